@@ -1,0 +1,189 @@
+#!/usr/bin/env python
+"""Benchmark: batched render-and-fit on one MI355X.
+
+Default workload (config 4): 16384 sources (30% point, 40% Exp, 30% Dev), each
+on its own P x P patch (P uniform on 16..128 step 8), 63x63 Moffat PSF
+(beta 3.5, FWHM 4 px), noise sigma 1, fit from truth perturbed by 0.5 px /
++-30% flux and r_e / +-0.1 in e until every source meets the reference's
+optimize_loop stopping rule.  One step = one complete batched fit from the
+start point.  value = sources fit per second.
+
+--workload c2: 4096 sources, one step = one LM iteration (Jacobian renders,
+normal equations, FP64 solve, line search); value = source-iterations/s.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ('sources fit/sec at χ² ≤1e-4 rel of FP64; surrogate-init vs plain; '
+          'HBM % peak')
+HBM_PEAK_GBS = 8000.0
+SLOT_NAMES = ['k_render_fft<32> (jacobian)', 'k_render_fft<64> (jacobian)',
+              'k_render_fft<128> (jacobian)', 'k_render_fft<256> (jacobian)',
+              'k_render_point (jacobian)', 'k_fit_normal_eq', 'k_fit_solve',
+              'k_render_fft<32> (line search chi2)', 'k_render_fft<64> (line search chi2)',
+              'k_render_fft<128> (line search chi2)', 'k_render_fft<256> (line search chi2)',
+              'k_render_point (line search chi2)', 'prep']
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name, torch):
+    from tractor_amd import batch as B
+    from tractor_amd import workloads as W
+    cfg = 4 if name == 'c4' else 2
+    n = 16384 if name == 'c4' else 4096
+    kind, P, truth = W.draw_sources(n, config=cfg)
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    fitter, data, inverr, pix_off = B.make_problem(kind, P, truth, psf, W.seed(cfg, 2))
+    start = W.perturb(kind, truth, np.random.default_rng(W.seed(cfg, 1)))
+    return dict(cfg=cfg, n=n, kind=kind, P=P, truth=truth, psf=psf, fitter=fitter, data=data,
+                inverr=inverr, pix_off=pix_off, start=start)
+
+
+def run_step(wl, torch):
+    f = wl['fitter']
+    if wl['cfg'] == 4:
+        f.set_params(wl['start'])
+        return f.fit(max_iter=100)
+    f.set_params(wl['start'])
+    f.iterate(0., 0.)
+    return 1
+
+
+def cpu_baseline(wl, budget_s=15.0, min_sources=4):
+    """The FP64 NumPy oracle (single core) fitting the same sources from the
+    same start, for about `budget_s` seconds."""
+    import oracle as O
+    from tractor_amd import batch as B
+    data = wl['data'].cpu().numpy()
+    inverr = wl['inverr'].cpu().numpy()
+    n = 0
+    t0 = time.time()
+    results = []
+    order = np.arange(wl['n'])
+    while (time.time() - t0 < budget_s or n < min_sources) and n < wl['n']:
+        s = int(order[n])
+        Pz, o = int(wl['P'][s]), int(wl['pix_off'][s])
+        prob = O.SourceProblem(int(wl['kind'][s]), wl['start'][s], Pz, wl['psf'],
+                               data=data[o:o + Pz * Pz].reshape(Pz, Pz),
+                               inverr=inverr[o:o + Pz * Pz].reshape(Pz, Pz), cdi=B.PIXEL_CDI)
+        if wl['cfg'] == 4:
+            p, nit, chi = O.fit_source(prob, wl['start'][s])
+        else:
+            X, _ = O.source_update(prob, wl['start'][s])
+            p, nit, chi = None, 1, None
+        results.append((s, p, chi))
+        n += 1
+    dt = time.time() - t0
+    return n, dt, results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='c4', choices=['c4', 'c2'])
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build runs the batched fit on a single '
+              'MI355X (multi-GPU sharding is designed, not built); use --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    log('building workload', args.workload)
+    wl = build_workload(args.workload, torch)
+    f = wl['fitter']
+    for _ in range(args.warmup):
+        run_step(wl, torch)
+    torch.cuda.synchronize()
+
+    f.profile(True)
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iters.append(run_step(wl, torch))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = f.profile_read()
+    f.profile(False)
+    dt = (t1 - t0) / args.steps
+    value = wl['n'] / dt
+    st = f.status()
+    log('per-step iterations', iters, 'max per-source iterations', int(st['iterations'].max()))
+
+    # dominant kernel (largest total time over the timed region)
+    k = int(np.argmax(prof[:, 0]))
+    ms, launches, jobs, nbytes = prof[k]
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = nbytes / max(launches, 1) / avg_s / 1e9
+    slots = {SLOT_NAMES[i]: dict(ms=round(float(prof[i, 0]), 3), launches=int(prof[i, 1]),
+                                 jobs=int(prof[i, 2]), gbytes=round(float(prof[i, 3]) / 1e9, 4))
+             for i in range(len(SLOT_NAMES)) if prof[i, 1] > 0}
+    log('per-slot totals over the timed region:', json.dumps(slots))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        n, cdt, res = cpu_baseline(wl, args.cpu_seconds)
+        cpu = dict(value=round(n / cdt, 4), unit='sources/s' if wl['cfg'] == 4 else 'source-iterations/s',
+                   cores=1, kind='port',
+                   sample='first %d sources of the same workload (same data and start), '
+                          'NumPy FP64 oracle, one process' % n)
+        if wl['cfg'] == 4:
+            gp = f.get_params()
+            errs = []
+            for (s, p, chi) in res:
+                errs.append(abs(st['chisq'][s] - chi) / chi)
+            parity = dict(sources=len(res), max_rel_chisq_vs_fp64=float(max(errs)))
+
+    out = {
+        'metric': METRIC,
+        'value': round(value, 3),
+        'unit': 'sources/s' if wl['cfg'] == 4 else 'source-iterations/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(dt * 1e3, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 (renders, J^T W J) + f64 (Cholesky, chi2)',
+        'data': 'synthetic',
+        'config': {
+            'workload': ('config4: batched LM fit of 16384 sources (30% point / 40% Exp / 30% Dev, '
+                         'P 16..128), 63x63 Moffat PSF, sigma=1, from perturbed start to convergence'
+                         if wl['cfg'] == 4 else
+                         'config2: one LM iteration over 4096 sources (30/40/30), P 16..128, '
+                         '63x63 Moffat PSF, sigma=1'),
+            'sources': wl['n'], 'lm_iterations_per_step': float(np.mean(iters)),
+            'parallelism': 'single GPU',
+        },
+        'roofline': {
+            'bound': 'hbm', 'kernel': SLOT_NAMES[k],
+            'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': round(achieved / HBM_PEAK_GBS, 5), 'traffic': None,
+            'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
+        },
+        'cpu_baseline': cpu,
+    }
+    if parity is not None:
+        out['parity'] = parity
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,245 @@
+/*
+ * tractor_hip.h -- C ABI of libtractor_hip.so, the MI355X (gfx950) render-and-fit
+ * library behind tractor_amd's Python API.
+ *
+ * Conventions (all entry points):
+ *   - every pointer argument that names an array is a DEVICE pointer owned by the
+ *     caller (hipMalloc / torch tensors' data_ptr), unless documented as host;
+ *   - `stream` is a hipStream_t passed as void* (0 = null stream); work is
+ *     enqueued on it and the call returns without synchronising;
+ *   - return 0 on success, a negative TRH_E* code on error; trh_last_error()
+ *     returns the calling thread's message for the last error.  Nothing aborts.
+ */
+#ifndef TRACTOR_HIP_H
+#define TRACTOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRH_OK            0
+#define TRH_E_ARG        -1   /* invalid argument / shape                   */
+#define TRH_E_HIP        -2   /* a HIP runtime call failed                   */
+#define TRH_E_NOMEM      -3   /* device allocation failed                    */
+#define TRH_E_UNSUPPORTED -4  /* size or mode outside what is implemented    */
+
+#define TRH_MAXK 16           /* max MoG components in one FFT render job    */
+
+/* Epilogue modes of a render job. */
+#define TRH_STORE   0         /* out[...] = scale * model                    */
+#define TRH_ADD     1         /* out[...] += scale * model (atomic)          */
+#define TRH_CHI2    2         /* chi2[slot] = sum(((data - scale*model)*inverr)^2), FP64 */
+
+/*
+ * One FFT-path render (ProfileGalaxy._realGetUnitFluxModelPatch FFT branch,
+ * tractor/galaxy.py:336-508): the zero-mean MoG's analytic Fourier transform
+ * (tractor/mp_fourier.i:431-478) on the sz x (sz/2+1) half grid, times the
+ * PSF spectrum (tractor/psf.py:249-280), inverse real FFT (galaxy.py:463),
+ * Lanczos-3 shift by (mux, muy) with truncated edge taps
+ * (mp_fourier.i:267-429), then the integer shift / crop into the output
+ * window:  out[oy][ox] = G[oy - sy][ox - sx]  (zero outside G)
+ * (galaxy.py:476-508).
+ */
+typedef struct trh_fft_job {
+    int64_t out_offset;      /* element offset of the window in out (or data/inverr for TRH_CHI2) */
+    int32_t out_stride;      /* row stride of the window, elements          */
+    int32_t mw, mh;          /* window width, height                         */
+    int32_t sx, sy;          /* integer shift                                */
+    int32_t psf;             /* index of the PSF spectrum in psf_spec        */
+    int32_t K;               /* number of MoG components (<= TRH_MAXK)       */
+    int32_t mode;            /* TRH_STORE / TRH_ADD / TRH_CHI2               */
+    int32_t slot;            /* chi2 output slot (TRH_CHI2)                  */
+    int32_t pad0;
+    double  mux, muy;        /* sub-pixel shift, |mu| <= 0.5                 */
+    float   scale;           /* counts (1 for a unit-flux patch)             */
+    float   pad1;
+    float   amp[TRH_MAXK];
+    float   var[TRH_MAXK][3];/* (var00, var01, var11) in pixels^2           */
+} trh_fft_job;
+
+/*
+ * One pixel-space MoG evaluation (c_gauss_2d_grid, tractor/mix.i:452-524, or
+ * c_gauss_2d_masked, tractor/gauss_masked.c:1-159 with eval_all_dxy_f,
+ * tractor/mix.i:85-142).  Component k is centred at
+ * (cxi[k] + cxf[k], cyi[k] + cyf[k]) in the window's pixel coordinates
+ * (integer + fractional parts, so large image coordinates keep FP32 precision).
+ */
+typedef struct trh_mog_job {
+    int64_t out_offset;      /* element offset of the window in out          */
+    int64_t mask_offset;     /* byte offset of a uint8 mask window, or -1    */
+    int32_t out_stride;      /* row stride of out / mask / derivs            */
+    int32_t x0, y0;          /* window origin in image pixels                */
+    int32_t w, h;            /* window size                                  */
+    int32_t comp0, K;        /* components [comp0, comp0+K) of the comp arrays */
+    int32_t cutoff_mode;     /* 0: skip Mahalanobis d>=100 (grid); 1: skip exponent < -30 (masked) */
+    int32_t mode;            /* TRH_STORE / TRH_ADD                          */
+    int32_t want_derivs;     /* write d/dmean_x, d/dmean_y into dx_out, dy_out */
+    float   scale;
+    float   pad;
+} trh_mog_job;
+
+/* Library handle: per-device twiddle tables and scratch. */
+typedef struct trh_handle trh_handle;
+
+const char *trh_last_error(void);
+int trh_version(void);
+
+/* sizeof(trh_fft_job), sizeof(trh_mog_job) as compiled (host only, no GPU
+ * call): lets bindings check their struct layouts. */
+int trh_struct_sizes(int *fft_job, int *mog_job);
+
+/* Device -> host copy of `bytes` bytes, synchronous on `stream` (binding
+ * plumbing for reading library-owned device buffers; no reference counterpart). */
+int trh_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream);
+
+/* Create / destroy a handle on `device` (no reference counterpart: replaces the
+ * module-level global scratch of tractor/psf.py:60-62). */
+int trh_create(trh_handle **h, int device);
+int trh_destroy(trh_handle *h);
+
+/*
+ * PSF spectra (PixelizedPSF.getFourierTransform, tractor/psf.py:249-280, with
+ * _padInImage 214-247): n float32 stamps [n][ph][pw] (ph, pw odd) are
+ * padded/cropped into sz x sz and transformed to half spectra
+ * out[n][sz][sz/2+1] (complex64 as float pairs).  sz in {32,64,128,256,512}.
+ * Writes the centre (cx, cy) of the padded stamp to *cx, *cy (host ints).
+ */
+int trh_psf_spectrum_batch(trh_handle *h, const float *stamps, int n, int ph, int pw,
+                           int sz, float *out, int *cx, int *cy, void *stream);
+
+/*
+ * Batched FFT-path renders (replaces mp_fourier.gaussian_fourier_transform_zero_mean
+ * + np.fft.irfftn + mp_fourier.lanczos_shift_3f, tractor/galaxy.py:459-472,
+ * called once per source from galaxy.py:256).  All jobs use FFT size sz.
+ * jobs: device array of njobs trh_fft_job.  psf_spec: [npsf][sz][sz/2+1]
+ * complex64.  out: float32 (TRH_STORE / TRH_ADD).  data, inverr: float32
+ * (TRH_CHI2).  chi2: double slots (TRH_CHI2).  Unused pointers may be NULL.
+ */
+int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njobs,
+                         const float *psf_spec, float *out, const float *data,
+                         const float *inverr, double *chi2, void *stream);
+
+/*
+ * Batched pixel-space MoG evaluation (replaces tractor.mix.c_gauss_2d_grid,
+ * tractor/mix.i:452-524 and c_gauss_2d_masked, tractor/gauss_masked.c:1-159).
+ * Component arrays (float32 except the integer centres): amp[c], cxi[c],
+ * cyi[c] (int32), cxf[c], cyf[c], var[c][3] = (v00, v01, v11).
+ * mask: uint8 base pointer (may be NULL if no job has a mask).  max_pixels:
+ * the largest w*h of any job (shapes the grid); njobs <= 65535 per call.
+ */
+int trh_render_mog_batch(trh_handle *h, const trh_mog_job *jobs, int njobs, int max_pixels,
+                         const float *amp, const int32_t *cxi, const int32_t *cyi,
+                         const float *cxf, const float *cyf, const float *var,
+                         const uint8_t *mask, float *out, float *dx_out, float *dy_out,
+                         void *stream);
+
+/*
+ * Lanczos-3 shift of float32 images (lanczos_shift_image, tractor/psf.py:26-58
+ * -> lanczos_shift_3f, tractor/mp_fourier.i:405-429), batched: n images of
+ * h x w at in + i*h*w, shifted by (dx[i], dy[i]) (device arrays of double)
+ * into out.  Replaces one SWIG call per point source (tractor/psf.py:180-187).
+ */
+int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n, int height,
+                            int width, const double *dx, const double *dy, void *stream);
+
+/*
+ * Batched pixelized-PSF point-source renders (PixelizedPSF.getPointSourcePatch
+ * with a modelMask, tractor/psf.py:137-190; one SWIG lanczos_shift_3f call per
+ * source in the reference).  Job fields as trh_fft_job with: (sx, sy) = window
+ * coordinates of the stamp's pixel (0,0); (mux, muy) = the sub-pixel shift
+ * px - round(px); psf, K, amp, var unused.  stamp: float32 [ph][pw];
+ * maxw >= every job's mw.
+ */
+int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, int maxw,
+                           const float *stamp, int ph, int pw, float *out, const float *data,
+                           const float *inverr, double *chi2, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched single-source fitting (config workloads: thousands of sources, one
+ * modelMask patch each).  Replaces GpuOptimizer's per-source
+ * gpu_one_source_update / gpu_one_source_try_updates
+ * (tractor/gpu_optimizer.py:184-811) and LsqrOptimizer.getUpdateDirection +
+ * ConstrainedOptimizer.tryUpdates (tractor/lsqr_optimizer.py:328-644,
+ * tractor/constrained_optimizer.py:36-159) for one-source problems.
+ * ------------------------------------------------------------------------ */
+typedef struct trh_fitter trh_fitter;
+
+/*
+ * Create a fitter for S sources.  Host arrays:
+ *   kind[S]      0 PointSource, 1 ExpGalaxy, 2 DevGalaxy
+ *   P[S]         square modelMask size (patch is (0,0,P,P) in patch pixels)
+ *   psf_stamp    float32 [ph][pw] PixelizedPSF stamp shared by all sources
+ *   cdi[4]       position-derivative matrix (ducks.py:413-423): row-major 2x2
+ * Device arrays (caller-owned, must outlive the fitter):
+ *   data, inverr float32, patch s at pix_off[s] (host int64 array), P*P row-major
+ */
+int trh_fitter_create(trh_handle *h, trh_fitter **f, int S, const int32_t *kind,
+                      const int32_t *P, const int64_t *pix_off, const float *data,
+                      const float *inverr, const float *psf_stamp, int ph, int pw,
+                      const double *cdi);
+int trh_fitter_destroy(trh_fitter *f);
+
+/* Copy parameters in/out: host double [S][6] = (x, y, flux, logre, ee1, ee2);
+ * point sources use the first 3. */
+int trh_fitter_set_params(trh_fitter *f, const double *params, void *stream);
+int trh_fitter_get_params(trh_fitter *f, double *params, void *stream);
+
+/*
+ * One LM iteration over all still-active sources (Jacobian renders,
+ * J^T W J in FP32 with FP64 reduction, FP64 damped Cholesky with damping
+ * `lambda` on the column-scaled normal equations (lambda=0: Gauss-Newton, the
+ * reference's direction), batched line search over the reference's alphas
+ * with its acceptance rule).  Sources whose step satisfies the optimize_loop
+ * stopping rule (constrained_optimizer.py:12-34) with `dchisq` become
+ * inactive.  Enqueued on `stream`.
+ */
+int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream);
+
+/*
+ * Render every source's model (counts x unit-flux patch at the current
+ * parameters, as getModelPatch, tractor/pointsource.py:36-60) into `out`,
+ * laid out like `data` (patch s at pix_off[s], P*P).  Enqueued on `stream`.
+ */
+int trh_fitter_render_models(trh_fitter *f, float *out, void *stream);
+
+/*
+ * Per-source status, host arrays (NULL to skip): active[S] (int32),
+ * iterations[S] (int32), chisq[S] (double, at the current parameters as of the
+ * last iteration), dlnp[S] (double, last accepted improvement).
+ * Synchronises `stream`.
+ */
+int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations, double *chisq,
+                      double *dlnp, int32_t *n_active, void *stream);
+
+/* Device pointer to the fitter's Jacobian renders (float32, per source
+ * 1 + n_shape planes of P*P at render_off[s]) and normal equations (double
+ * [S][32]: 21 upper-triangle entries, 6 gradient entries, chi^2).  For
+ * testing. */
+int trh_fitter_debug(trh_fitter *f, const float **renders, const int64_t **render_off,
+                     const double **normal_eq, const double **step_x);
+
+/*
+ * Per-launch profiling.  trh_fitter_profile(f, 1) resets the counters and
+ * makes every later trh_fitter_iterate record HIP events around each launch
+ * slot (and synchronise its stream at the end).  trh_fitter_profile_read
+ * fills out[13][4] = (total ms, launches, jobs, algorithmic bytes) per slot:
+ * 0-3 Jacobian FFT renders at sz 32/64/128/256 (bytes: renders written + PSF
+ * spectrum read), 4 Jacobian point renders, 5 normal equations (renders +
+ * data + inverr read), 6 solve, 7-10 line-search FFT renders (data + inverr
+ * read), 11 line-search point renders, 12 job preparation.
+ */
+int trh_fitter_profile(trh_fitter *f, int enable);
+int trh_fitter_profile_read(trh_fitter *f, double *out);
+
+/* Kernel timing of the last trh_fitter_iterate (milliseconds, by HIP events
+ * on the fitter's stream): [0] jacobian renders, [1] normal equations,
+ * [2] solve + steps, [3] line-search renders, [4] accept, [5] whole iteration. */
+int trh_fitter_timing(trh_fitter *f, float *ms6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRACTOR_HIP_H */

@@ -1,0 +1,124 @@
+"""GPU parity of the batched fitter (trh_fitter_*) against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tractor_amd import batch as B
+from tractor_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope='module')
+def problem():
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    rng = np.random.default_rng(W.seed(2, 0))
+    kind, P, truth = W.draw_sources(48, rng=rng, P_choices=np.array([16, 32, 48, 64, 80, 104, 128]))
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    fitter, data, inverr, pix_off = B.make_problem(kind, P, truth, psf, W.seed(2, 2))
+    start = W.perturb(kind, truth, np.random.default_rng(W.seed(2, 1)))
+    return dict(kind=kind, P=P, truth=truth, psf=psf, fitter=fitter, data=data.cpu().numpy(),
+                inverr=inverr.cpu().numpy(), pix_off=pix_off, start=start)
+
+
+def _prob(pb, s):
+    P = int(pb['P'][s])
+    o = int(pb['pix_off'][s])
+    return O.SourceProblem(int(pb['kind'][s]), pb['start'][s], P, pb['psf'],
+                           data=pb['data'][o:o + P * P].reshape(P, P),
+                           inverr=pb['inverr'][o:o + P * P].reshape(P, P), cdi=B.PIXEL_CDI)
+
+
+def test_truth_models_match_oracle(problem):
+    # data - noise = model at truth; check the model render path through the
+    # fitter against the oracle's renders of the truth
+    say('render models vs oracle')
+    pb = problem
+    f = pb['fitter']
+    out = torch.zeros_like(torch.from_numpy(pb['data'])).cuda()
+    f.set_params(pb['truth'])
+    f.render_models(out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    for s in range(len(pb['kind'])):
+        P, o = int(pb['P'][s]), int(pb['pix_off'][s])
+        prob = _prob(pb, s)
+        ref = O.source_render(prob, pb['truth'][s]) * np.float32(pb['truth'][s, 2])
+        g = got[o:o + P * P].reshape(P, P)
+        e = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+        assert e < 1e-5, (s, pb['kind'][s], P, e)
+
+
+def test_one_iteration_normal_equations_and_step(problem):
+    say('one iteration vs oracle')
+    pb = problem
+    f = pb['fitter']
+    f.set_params(pb['start'])
+    f.iterate(0., 0.)
+    dbg = f.debug_arrays()
+    S = len(pb['kind'])
+    for s in range(S):
+        prob = _prob(pb, s)
+        p0, derivs = O.source_jacobian(prob, pb['start'][s])
+        P = prob.P
+        # Jacobian renders
+        r = dbg['renders'][dbg['render_off'][s]:dbg['render_off'][s] + P * P].reshape(P, P)
+        assert np.max(np.abs(r - p0)) / np.max(np.abs(p0)) < 1e-5
+        # normal equations
+        A = np.stack([(d * prob.inverr).astype(np.float32).ravel() for d in derivs], 1).astype(np.float64)
+        mod = (p0 * np.float32(pb['start'][s, 2])).astype(np.float32)
+        b = ((prob.data - mod) * prob.inverr).astype(np.float32).ravel().astype(np.float64)
+        n = A.shape[1]
+        N = A.T @ A
+        iu = [(i, j) for i in range(6) for j in range(i, 6)]
+        gotN = np.zeros((6, 6))
+        for q, (i, j) in enumerate(iu):
+            gotN[i, j] = gotN[j, i] = dbg['neq'][s, q]
+        scale = np.sqrt(np.outer(np.diag(N), np.diag(N)))
+        assert np.max(np.abs(gotN[:n, :n] - N) / scale) < 1e-4, s
+        g = A.T @ b
+        assert np.max(np.abs(dbg['neq'][s, 21:21 + n] - g) / (np.sqrt(np.diag(N)) * np.sqrt(b @ b))) < 1e-4
+        chi2 = b @ b
+        assert abs(dbg['neq'][s, 27] - chi2) / chi2 < 1e-6
+        # update direction: column-scaled least squares
+        X = O.column_scaled_lstsq(A, b)
+        sig = 1. / np.sqrt(np.diag(N))
+        err = np.abs(dbg['X'][s, :n] - X)
+        assert np.all((err <= 1e-4 * np.abs(X)) | (err <= 0.01 * sig)), (s, dbg['X'][s, :n], X)
+
+
+def test_full_fit_vs_oracle(problem):
+    say('full fit vs oracle')
+    pb = problem
+    f = pb['fitter']
+    f.set_params(pb['start'])
+    nit = f.fit(max_iter=100)
+    st = f.status()
+    got = f.get_params()
+    say('gpu iterations', nit, 'per-source max', st['iterations'].max())
+    S = len(pb['kind'])
+    nbad = 0
+    for s in range(S):
+        prob = _prob(pb, s)
+        ref, nref, chi = O.fit_source(prob, pb['start'][s])
+        np_ = prob.nparams
+        # 0.01 sigma from the Fisher matrix at the oracle's solution
+        _, derivs = O.source_jacobian(prob, ref)
+        A = np.stack([(d * prob.inverr).astype(np.float64).ravel() for d in derivs], 1)
+        sig = np.sqrt(np.diag(np.linalg.pinv(A.T @ A)))
+        err = np.abs(got[s, :np_] - ref[:np_])
+        ok = np.all((err <= 1e-4 * np.abs(ref[:np_])) | (err <= 0.01 * sig))
+        chi_ok = abs(st['chisq'][s] - chi) <= 1e-4 * chi
+        if not (ok and chi_ok):
+            nbad += 1
+            say('source', s, 'kind', prob.kind, 'P', prob.P, 'gpu', got[s, :np_], 'oracle', ref[:np_],
+                'sig', sig, 'chi', st['chisq'][s], chi, 'iters', st['iterations'][s], nref)
+    assert nbad == 0

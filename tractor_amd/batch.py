@@ -1,0 +1,156 @@
+"""Batched single-source fitting on the GPU (trh_fitter_* in libtractor_hip).
+
+The batched counterpart of running GpuOptimizer / ConstrainedOptimizer
+(tractor/gpu_optimizer.py:184-811, tractor/constrained_optimizer.py:12-159)
+on thousands of one-source Tractors: one persistent device state, one stream,
+no host round trips inside an iteration.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr
+
+NULLWCS_CDI = np.eye(2) * 3600.   # NullWCS(pixscale=1).cd_inverse (tractor/wcs.py:24-28)
+PIXEL_CDI = np.eye(2)             # position parameters in pixels
+
+
+class BatchFitter(object):
+    """S sources, each on its own P x P patch of `data` / `inverr`
+    (device float32 tensors, patch s at pix_off[s]).
+
+    kind: 0 PointSource, 1 ExpGalaxy, 2 DevGalaxy;  params (S, 6):
+    x, y, flux, logre, ee1, ee2 (EllipseESoft) in patch pixels.
+    cdi: 2x2 position-derivative matrix (tractor/ducks.py:413-423).
+    """
+    def __init__(self, kind, P, pix_off, data, inverr, psf_stamp, cdi=PIXEL_CDI, device=0):
+        self.lib = _lib.load()
+        self.h = _lib.handle(device)
+        self.S = len(kind)
+        self.kind = np.ascontiguousarray(kind, np.int32)
+        self.P = np.ascontiguousarray(P, np.int32)
+        self.pix_off = np.ascontiguousarray(pix_off, np.int64)
+        self.data = data
+        self.inverr = inverr
+        psf = np.ascontiguousarray(psf_stamp, np.float32)
+        self.psf = psf
+        cdi = np.ascontiguousarray(np.asarray(cdi, np.float64).ravel())
+        assert data.dtype == torch.float32 and inverr.dtype == torch.float32
+        assert data.is_cuda and inverr.is_cuda
+        need = int(np.max(self.pix_off + self.P.astype(np.int64) ** 2))
+        if data.numel() < need or inverr.numel() < need:
+            raise ValueError('data/inverr hold %d/%d pixels, patches need %d' % (
+                data.numel(), inverr.numel(), need))
+        f = ctypes.c_void_p()
+        check(self.lib.trh_fitter_create(self.h.value, ctypes.byref(f), self.S, ptr(self.kind),
+                                         ptr(self.P), ptr(self.pix_off), ptr(data), ptr(inverr),
+                                         ptr(psf), psf.shape[0], psf.shape[1], ptr(cdi)),
+              'trh_fitter_create')
+        self.f = f
+        self.stream = torch.cuda.current_stream().cuda_stream
+
+    def close(self):
+        if self.f:
+            self.lib.trh_fitter_destroy(self.f)
+            self.f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float64)
+        assert p.shape == (self.S, 6)
+        check(self.lib.trh_fitter_set_params(self.f, ptr(p), self.stream), 'set_params')
+
+    def get_params(self):
+        p = np.zeros((self.S, 6))
+        check(self.lib.trh_fitter_get_params(self.f, ptr(p), self.stream), 'get_params')
+        return p
+
+    def render_models(self, out):
+        check(self.lib.trh_fitter_render_models(self.f, ptr(out), self.stream), 'render_models')
+
+    def iterate(self, lam=0., dchisq=0.):
+        check(self.lib.trh_fitter_iterate(self.f, float(lam), float(dchisq), self.stream),
+              'iterate')
+
+    def status(self):
+        active = np.zeros(self.S, np.int32)
+        iters = np.zeros(self.S, np.int32)
+        chisq = np.zeros(self.S)
+        dlnp = np.zeros(self.S)
+        n = ctypes.c_int32()
+        check(self.lib.trh_fitter_status(self.f, ptr(active), ptr(iters), ptr(chisq), ptr(dlnp),
+                                         ctypes.byref(n), self.stream), 'status')
+        return dict(active=active, iterations=iters, chisq=chisq, dlnp=dlnp, n_active=n.value)
+
+    def n_active(self):
+        n = ctypes.c_int32()
+        check(self.lib.trh_fitter_status(self.f, None, None, None, None, ctypes.byref(n),
+                                         self.stream), 'status')
+        return n.value
+
+    def timing(self):
+        ms = np.zeros(6, np.float32)
+        check(self.lib.trh_fitter_timing(self.f, ptr(ms)), 'timing')
+        return ms
+
+    def profile(self, enable=True):
+        check(self.lib.trh_fitter_profile(self.f, 1 if enable else 0), 'profile')
+
+    def profile_read(self):
+        out = np.zeros((13, 4))
+        check(self.lib.trh_fitter_profile_read(self.f, ptr(out)), 'profile_read')
+        return out
+
+    def fit(self, max_iter=100, dchisq=0., lam=0., check_every=1):
+        """Iterate until every source meets the optimize_loop stopping rule."""
+        it = 0
+        while it < max_iter:
+            for _ in range(check_every):
+                self.iterate(lam, dchisq)
+                it += 1
+            if self.n_active() == 0:
+                break
+        return it
+
+    # ---- debugging views of device state (tests) ----
+    def debug_arrays(self):
+        r = ctypes.c_void_p()
+        ro = ctypes.c_void_p()
+        ne = ctypes.c_void_p()
+        x = ctypes.c_void_p()
+        check(self.lib.trh_fitter_debug(self.f, ctypes.byref(r), ctypes.byref(ro), ctypes.byref(ne),
+                                        ctypes.byref(x)), 'debug')
+        nr = np.where(self.kind > 0, 4, 1) * self.P.astype(np.int64) ** 2
+        total = int(nr.sum())
+        renders = np.zeros(total, np.float32)
+        neq = np.zeros((self.S, 32))
+        X = np.zeros((self.S, 8))
+        roff = np.zeros(self.S, np.int64)
+        for dst, src, nb in [(renders, r, renders.nbytes), (neq, ne, neq.nbytes), (X, x, X.nbytes),
+                             (roff, ro, roff.nbytes)]:
+            check(self.lib.trh_copy_to_host(ptr(dst), src.value, nb, self.stream), 'copy')
+        return dict(renders=renders, render_off=roff, neq=neq, X=X)
+
+
+def make_problem(kind, P, truth, psf_stamp, noise_seed, device='cuda', sigma=1.0, cdi=PIXEL_CDI):
+    """Synthesize data = render(truth) + N(0, sigma^2) on the device, inverr =
+    1/sigma, and return (fitter, data, inverr, pix_off)."""
+    from .workloads import patch_offsets
+    pix_off, total = patch_offsets(np.asarray(P))
+    data = torch.zeros(total, dtype=torch.float32, device=device)
+    inverr = torch.full((total,), 1.0 / sigma, dtype=torch.float32, device=device)
+    fitter = BatchFitter(kind, P, pix_off, data, inverr, psf_stamp, cdi=cdi)
+    fitter.set_params(truth)
+    fitter.render_models(data)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(noise_seed))
+    data.add_(torch.randn(total, generator=g, device=device, dtype=torch.float32) * sigma)
+    torch.cuda.synchronize()
+    return fitter, data, inverr, pix_off

@@ -1,0 +1,937 @@
+// Batched single-source fitter: one LM iteration over thousands of sources,
+// each on its own P x P modelMask patch, entirely on the device.
+//
+// Per iteration (all launches on one stream, no host synchronisation):
+//   prep_jac  -> build 1 (point) or 4 (galaxy: current + 3 shape steps)
+//                render jobs per active source, bucketed by FFT size
+//   render    -> k_render_fft<sz> / k_render_point write the unit-flux renders
+//   normal_eq -> J^T W J, J^T W r, chi^2 per source (FP32 products, FP64 sums)
+//   solve     -> column-scaled, damped FP64 Cholesky; the reference's line
+//                search steps (ConstrainedOptimizer.getParameterSteps)
+//   prep_ls   -> one chi^2 render job per (source, step)
+//   render    -> fused render + chi^2 (no model written to HBM)
+//   accept    -> tryUpdates acceptance rule + optimize_loop stopping rule
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <vector>
+
+#include "../../include/tractor_hip.h"
+#include "internal.h"
+#include "render_fft.h"
+
+using namespace trh;
+
+#define NALPHA 13
+#define NBUCKET 5  // FFT sizes 32, 64, 128, 256 + point sources
+// launch slots: 0-3 jacobian FFT renders (sz 32..256), 4 jacobian point
+// renders, 5 normal equations, 6 solve, 7-10 line-search FFT renders,
+// 11 line-search point renders, 12 job preparation + accept
+#define TRH_NSLOT 13
+
+struct FitProfiles {
+    double amp[2][TRH_MAXK];
+    double var[2][TRH_MAXK];
+    int K[2];
+};
+
+struct trh_fitter {
+    trh_handle *h = nullptr;
+    int S = 0, n_gal = 0, n_pt = 0, ph = 0, pw = 0;
+    double cdi[4] = {1, 0, 0, 1};
+    int maxrows[4] = {8, 8, 8, 8};
+    int maxP_pt = 1;
+    int cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
+    // device
+    int32_t *kind = nullptr, *P = nullptr;
+    int64_t *pix_off = nullptr, *render_off = nullptr;
+    const float *data = nullptr, *inverr = nullptr;
+    float *stamp = nullptr;
+    float *spec[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *renders = nullptr;
+    double *params = nullptr, *trial = nullptr, *X = nullptr, *neq = nullptr, *chi2ls = nullptr;
+    int32_t *nsteps = nullptr, *steplim = nullptr, *active = nullptr, *iters = nullptr;
+    double *chisq = nullptr, *dlnp = nullptr;
+    int32_t *err = nullptr;
+    FitProfiles *prof = nullptr;
+    trh_fft_job *jobs_jac = nullptr, *jobs_ls = nullptr;
+    int32_t *list_jac = nullptr, *list_ls = nullptr, *cnt = nullptr;  // cnt[0..4] jac, [8..12] ls
+    hipEvent_t ev[6];
+    bool have_ev = false;
+    // per-launch profiling (trh_fitter_profile): events around every launch
+    // slot, device accumulators of jobs and algorithmic bytes per slot
+    bool profiling = false;
+    hipEvent_t pev[TRH_NSLOT][2];
+    unsigned long long *acc = nullptr;  // [TRH_NSLOT][2]: jobs, bytes
+    double tot_ms[TRH_NSLOT];
+    double tot_launch[TRH_NSLOT];
+    double tot_extra_bytes[TRH_NSLOT];   // PSF spectrum reads, counted per launch
+    std::vector<void *> allocs;
+};
+
+static const int kSizes[4] = {32, 64, 128, 256};
+
+__device__ __forceinline__ int sz_bucket(int sz) {
+    switch (sz) {
+    case 32: return 0;
+    case 64: return 1;
+    case 128: return 2;
+    case 256: return 3;
+    }
+    return -1;
+}
+
+// Build the FFT render job of a galaxy at (x, y) with EllipseESoft shape
+// (logre, ee1, ee2) on the modelMask (0, 0, P, P) (galaxy.py:336-406,
+// 659-669; ellipses.py:126-155, 232-255; NullWCS pixscale 1).  Returns the
+// FFT size.
+__device__ int build_gal_job(trh_fft_job &J, const FitProfiles &pr, int kind, double x, double y,
+                             double logre, double ee1, double ee2, int P, int ph, int pw) {
+    const int pi = kind - 1;
+    const double re = exp(logre);
+    const double ee = hypot(ee1, ee2);
+    const double e = 1.0 - exp(-ee);
+    const double theta = atan2(ee2, ee1) / 2.0;
+    const double ct = cos(theta), st = sin(theta);
+    const double ab = e >= 1.0 ? 1000.0 : fmin(1000.0, (1.0 + e) / (1.0 - e));
+    const double r_deg = re / 3600.0;
+    const double cdinv = 1.0 / (1.0 / 3600.0);
+    // T = cdinv * G, G = r_deg [[ct/ab, st], [-st/ab, ct]]
+    const double T00 = cdinv * r_deg * ct / ab, T01 = cdinv * r_deg * st;
+    const double T10 = -cdinv * r_deg * st / ab, T11 = cdinv * r_deg * ct;
+    const double M00 = T00 * T00 + T01 * T01;
+    const double M01 = T00 * T10 + T01 * T11;
+    const double M11 = T10 * T10 + T11 * T11;
+    J.K = pr.K[pi];
+    for (int k = 0; k < pr.K[pi]; ++k) {
+        const double v = pr.var[pi][k];
+        J.amp[k] = (float)pr.amp[pi][k];
+        J.var[k][0] = (float)(v * M00);
+        J.var[k][1] = (float)(v * M01);
+        J.var[k][2] = (float)(v * M11);
+    }
+    // halfsize with a modelMask (galaxy.py:337-351)
+    double hs = fmax(P / 2.0, P / 2.0);
+    hs = fmax(hs, fmax(fmax(1.0 + x - 0.0, 1.0 + P - x), fmax(1.0 + y - 0.0, 1.0 + P - y)));
+    hs = fmax(hs, fmax(pw / 2.0, ph / 2.0));
+    const int ihs = (int)ceil(hs);
+    int sz = 1;
+    while (sz < 2 * ihs) sz <<= 1;
+    const int cx = sz >= pw ? (sz - pw) / 2 + pw / 2 : pw / 2 - (pw - sz) / 2;
+    const int cy = sz >= ph ? (sz - ph) / 2 + ph / 2 : ph / 2 - (ph - sz) / 2;
+    double mux = x - cx;
+    double muy = y - cy;
+    const double sx = rint(mux), sy = rint(muy);
+    J.sx = (int)sx;
+    J.sy = (int)sy;
+    J.mux = mux - sx;
+    J.muy = muy - sy;
+    J.mw = P;
+    J.mh = P;
+    J.out_stride = P;
+    J.psf = 0;
+    return sz;
+}
+
+// Point source with PixelizedPSF on the modelMask (0, 0, P, P)
+// (psf.py:137-190): the stamp's (0,0) lands at (ix - pw//2, iy - ph//2).
+__device__ void build_point_job(trh_fft_job &J, double x, double y, int P, int ph, int pw) {
+    const double ix = rint(x), iy = rint(y);
+    J.mux = x - ix;
+    J.muy = y - iy;
+    J.sx = (int)ix - pw / 2;
+    J.sy = (int)iy - ph / 2;
+    J.mw = P;
+    J.mh = P;
+    J.out_stride = P;
+    J.psf = 0;
+    J.K = 0;
+}
+
+__global__ void k_fit_reset(int32_t *cnt, int n) {
+    if ((int)threadIdx.x < n) cnt[threadIdx.x] = 0;
+}
+
+__global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
+                               const int32_t *__restrict__ Parr, const int64_t *__restrict__ roff,
+                               const double *__restrict__ params,
+                               const int32_t *__restrict__ active, const FitProfiles *__restrict__ pr,
+                               int ph, int pw, trh_fft_job *__restrict__ jobs,
+                               int32_t *__restrict__ lists, int cap, int32_t *__restrict__ cnt,
+                               int32_t *__restrict__ err, unsigned long long *__restrict__ acc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !active[s]) return;
+    const int kd = kind[s];
+    const int P = Parr[s];
+    const double *p = params + 8 * s;
+    const unsigned long long PP = (unsigned long long)P * P;
+    if (acc) {
+        // normal equations read N_r renders + data + inverr, write 28 doubles
+        atomicAdd(&acc[2 * 5], 1ull);
+        atomicAdd(&acc[2 * 5 + 1], (kd ? 4ull * 4 + 8 : 4ull + 8) * PP + 28 * 8);
+        atomicAdd(&acc[2 * 6], 1ull);
+        atomicAdd(&acc[2 * 6 + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
+    }
+    if (kd == 0) {
+        trh_fft_job &J = jobs[4 * s];
+        build_point_job(J, p[0], p[1], P, ph, pw);
+        J.out_offset = roff[s];
+        J.mode = TRH_STORE;
+        J.scale = 1.f;
+        J.slot = 0;
+        const int i = atomicAdd(&cnt[4], 1);
+        lists[4 * cap + i] = 4 * s;
+        if (acc) {
+            atomicAdd(&acc[2 * 4], 1ull);
+            atomicAdd(&acc[2 * 4 + 1], 4ull * PP);
+        }
+        return;
+    }
+    for (int k = 0; k < 4; ++k) {
+        double sh[3] = {p[3], p[4], p[5]};
+        if (k > 0) sh[k - 1] += 0.01;  // EllipseE stepsizes (ellipses.py:44)
+        trh_fft_job &J = jobs[4 * s + k];
+        const int sz = build_gal_job(J, *pr, kd, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
+        J.out_offset = roff[s] + (int64_t)k * P * P;
+        J.mode = TRH_STORE;
+        J.scale = 1.f;
+        J.slot = 0;
+        const int b = sz_bucket(sz);
+        if (b < 0) {
+            atomicExch(err, sz);
+            continue;
+        }
+        const int i = atomicAdd(&cnt[b], 1);
+        lists[b * cap + i] = 4 * s + k;
+        if (acc) {
+            atomicAdd(&acc[2 * b], 1ull);
+            atomicAdd(&acc[2 * b + 1], 4ull * PP);  // render written
+        }
+    }
+}
+
+// Model renders (counts x unit-flux patch at the current parameters) into an
+// image buffer laid out like `data` (patch s at pix_off[s]).
+__global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
+                                 const int32_t *__restrict__ Parr,
+                                 const int64_t *__restrict__ pix_off,
+                                 const double *__restrict__ params,
+                                 const FitProfiles *__restrict__ pr, int ph, int pw,
+                                 trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
+                                 int cap, int32_t *__restrict__ cnt, int32_t *__restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int kd = kind[s];
+    const int P = Parr[s];
+    const double *p = params + 8 * s;
+    trh_fft_job &J = jobs[4 * s];
+    int b;
+    if (kd == 0) {
+        build_point_job(J, p[0], p[1], P, ph, pw);
+        b = 4;
+    } else {
+        const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
+        b = sz_bucket(sz);
+        if (b < 0) {
+            atomicExch(err, sz);
+            return;
+        }
+    }
+    J.out_offset = pix_off[s];
+    J.mode = TRH_STORE;
+    J.scale = (float)p[2];
+    J.slot = 0;
+    const int i = atomicAdd(&cnt[b], 1);
+    lists[b * cap + i] = 4 * s;
+}
+
+// J^T W J, J^T W r and chi^2 for one source per block (grid-stride).
+// Columns (galaxy.py:137-223, pointsource.py:130-241, lsqr_optimizer.py:386-431):
+//   pos_i = ((dxp*cdi[0][i] + dyp*cdi[1][i]) * counts) * inverr  (central
+//           differences of the unit render, zero 1-px border)
+//   flux  = render * inverr
+//   shape = ((R_k - R_0) * (counts / 0.01)) * inverr
+//   b     = (data - counts * R_0) * inverr
+__global__ void __launch_bounds__(256)
+k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
+                const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
+                const float *__restrict__ data, const float *__restrict__ inverr,
+                const float *__restrict__ renders, const double *__restrict__ params,
+                const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
+                double cdi11, double *__restrict__ neq) {
+    __shared__ double red[4][28];
+    const float c00 = (float)cdi00, c01 = (float)cdi01, c10 = (float)cdi10, c11 = (float)cdi11;
+    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+        if (!active[s]) continue;
+        const int kd = kind[s];
+        const int P = Parr[s];
+        const int np = kd ? 6 : 3;
+        const float counts = (float)params[8 * s + 2];
+        const float cstep = (float)(params[8 * s + 2] / 0.01);
+        const float *R0 = renders + roff[s];
+        const float *D = data + pix_off[s];
+        const float *E = inverr + pix_off[s];
+        const int PP = P * P;
+        float acc[27];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) acc[i] = 0.f;
+        double chi2 = 0.0;
+        for (int idx = threadIdx.x; idx < PP; idx += blockDim.x) {
+            const int y = idx / P;
+            const int x = idx - y * P;
+            const float r0 = R0[idx];
+            float dxp = 0.f, dyp = 0.f;
+            if (x > 0 && x < P - 1 && y > 0 && y < P - 1) {
+                dxp = (R0[idx - 1] - R0[idx + 1]) / 2.f;
+                dyp = (R0[idx - P] - R0[idx + P]) / 2.f;
+            }
+            const float e = E[idx];
+            float col[6];
+            col[0] = ((dxp * c00 + dyp * c10) * counts) * e;
+            col[1] = ((dxp * c01 + dyp * c11) * counts) * e;
+            col[2] = r0 * e;
+            if (np == 6) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) col[3 + k] = ((R0[(k + 1) * PP + idx] - r0) * cstep) * e;
+            } else {
+                col[3] = col[4] = col[5] = 0.f;
+            }
+            const float b = (D[idx] - r0 * counts) * e;
+            int q = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+#pragma unroll
+                for (int j = i; j < 6; ++j) acc[q++] += col[i] * col[j];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) acc[21 + i] += col[i] * b;
+            chi2 += (double)b * (double)b;
+        }
+        // block reduction in FP64
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        double v[28];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) v[i] = wave_sum((double)acc[i]);
+        v[27] = wave_sum(chi2);
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < 28; ++i) red[wid][i] = v[i];
+        }
+        __syncthreads();
+        if (threadIdx.x < 28) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
+            neq[32 * s + threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// Column-scaled damped Cholesky solve (FP64) + getParameterSteps
+// (constrained_optimizer.py:84-159).  One thread per source.
+__global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
+                            const double *__restrict__ neq, const double *__restrict__ params,
+                            const int32_t *__restrict__ active, double lambda,
+                            double *__restrict__ X, double *__restrict__ trial,
+                            int32_t *__restrict__ nsteps, int32_t *__restrict__ steplim) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    if (!active[s]) {
+        nsteps[s] = 0;
+        return;
+    }
+    const int np = kind[s] ? 6 : 3;
+    const double *q = neq + 32 * s;
+    double A[6][6], g[6], D[6];
+    int t = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            A[i][j] = q[t];
+            A[j][i] = q[t];
+            ++t;
+        }
+    for (int i = 0; i < 6; ++i) g[i] = q[21 + i];
+    bool any = false;
+    for (int i = 0; i < np; ++i) {
+        D[i] = sqrt(A[i][i]);
+        if (D[i] > 0.0) any = true;
+    }
+    double x[6] = {0, 0, 0, 0, 0, 0};
+    if (!any) {
+        // all derivatives zero: no update (gpu_optimizer.py:762-767)
+        nsteps[s] = 0;
+        steplim[s] = 0;
+        for (int i = 0; i < 8; ++i) X[8 * s + i] = 0.0;
+        return;
+    }
+    double lam = lambda;
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        double L[6][6], rhs[6];
+        bool ok = true;
+        for (int i = 0; i < np; ++i) {
+            for (int j = 0; j < np; ++j) {
+                if (D[i] > 0.0 && D[j] > 0.0)
+                    L[i][j] = A[i][j] / (D[i] * D[j]);
+                else
+                    L[i][j] = (i == j) ? 1.0 : 0.0;
+            }
+            if (D[i] > 0.0) L[i][i] += lam;
+            rhs[i] = D[i] > 0.0 ? g[i] / D[i] : 0.0;
+        }
+        for (int j = 0; j < np && ok; ++j) {
+            double d = L[j][j];
+            for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+            if (!(d > 1e-300)) {
+                ok = false;
+                break;
+            }
+            d = sqrt(d);
+            L[j][j] = d;
+            for (int i = j + 1; i < np; ++i) {
+                double v = L[i][j];
+                for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+                L[i][j] = v / d;
+            }
+        }
+        if (ok) {
+            double y[6];
+            for (int i = 0; i < np; ++i) {
+                double v = rhs[i];
+                for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+                y[i] = v / L[i][i];
+            }
+            for (int i = np - 1; i >= 0; --i) {
+                double v = y[i];
+                for (int k = i + 1; k < np; ++k) v -= L[k][i] * x[k];
+                x[i] = v / L[i][i];
+            }
+            for (int i = 0; i < np; ++i) x[i] = D[i] > 0.0 ? x[i] / D[i] : 0.0;
+            break;
+        }
+        lam = lam > 0.0 ? lam * 10.0 : 1e-12;
+    }
+    for (int i = 0; i < 8; ++i) X[8 * s + i] = i < np ? x[i] : 0.0;
+
+    // getParameterSteps: maxstep PixPos [1,1] (wcs.py:384), Flux none,
+    // EllipseESoft [1, 0.1, 0.1] (ellipses.py:48); no bounds for ESoft.
+    const double maxstep[6] = {1.0, 1.0, -1.0, 1.0, 0.1, 0.1};
+    const double *p0 = params + 8 * s;
+    int n = 0;
+    int anylim = 0;
+    for (int a = 0; a < NALPHA; ++a) {
+        double alpha = a <= 10 ? ldexp(1.0, a - 10) : (a == 11 ? sqrt(2.0) : 2.0);
+        bool do_break = false;
+        for (int i = 0; i < np; ++i) {
+            if (maxstep[i] < 0.0) continue;
+            const double step = alpha * fabs(x[i]);
+            if (step > maxstep[i]) {
+                do_break = true;
+                anylim = 1;
+                alpha = maxstep[i] / fabs(x[i]);
+            }
+        }
+        if (alpha < 1e-8) break;
+        double *tp = trial + ((size_t)s * NALPHA + n) * 8;
+        for (int i = 0; i < 8; ++i) tp[i] = i < np ? p0[i] + alpha * x[i] : 0.0;
+        ++n;
+        if (do_break) break;
+    }
+    nsteps[s] = n;
+    steplim[s] = anylim;
+}
+
+__global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
+                              const int32_t *__restrict__ Parr,
+                              const int64_t *__restrict__ pix_off,
+                              const double *__restrict__ trial,
+                              const int32_t *__restrict__ nsteps,
+                              const int32_t *__restrict__ active,
+                              const FitProfiles *__restrict__ pr, int ph, int pw,
+                              trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
+                              int cap, int32_t *__restrict__ cnt, int32_t *__restrict__ err,
+                              unsigned long long *__restrict__ acc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !active[s]) return;
+    const int kd = kind[s];
+    const int P = Parr[s];
+    const int n = nsteps[s];
+    for (int a = 0; a < n; ++a) {
+        const double *p = trial + ((size_t)s * NALPHA + a) * 8;
+        const int slot = s * NALPHA + a;
+        trh_fft_job &J = jobs[slot];
+        int b;
+        if (kd == 0) {
+            build_point_job(J, p[0], p[1], P, ph, pw);
+            b = 4;
+        } else {
+            const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
+            b = sz_bucket(sz);
+            if (b < 0) {
+                atomicExch(err, sz);
+                continue;
+            }
+        }
+        J.out_offset = pix_off[s];
+        J.mode = TRH_CHI2;
+        J.scale = (float)p[2];
+        J.slot = slot;
+        const int i = atomicAdd(&cnt[b], 1);
+        lists[b * cap + i] = slot;
+        if (acc) {
+            atomicAdd(&acc[2 * (7 + b)], 1ull);
+            atomicAdd(&acc[2 * (7 + b) + 1], 8ull * P * P);  // data + inverr read
+        }
+    }
+}
+
+// tryUpdates acceptance (constrained_optimizer.py:36-82) and the optimize_loop
+// stopping rule (constrained_optimizer.py:12-34).
+__global__ void k_fit_accept(int S, const double *__restrict__ neq,
+                             const double *__restrict__ trial,
+                             const int32_t *__restrict__ nsteps,
+                             const int32_t *__restrict__ steplim,
+                             const double *__restrict__ chi2ls, double *__restrict__ params,
+                             int32_t *__restrict__ active, int32_t *__restrict__ iters,
+                             double *__restrict__ chisq, double *__restrict__ dlnp_out,
+                             double dchisq, int max_iter) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || !active[s]) return;
+    const double chi0 = neq[32 * s + 27];
+    const double lnp0 = -0.5 * chi0;
+    double best = lnp0;
+    int ib = -1;
+    const int n = nsteps[s];
+    for (int a = 0; a < n; ++a) {
+        const double lnp = -0.5 * chi2ls[s * NALPHA + a];
+        if (!isfinite(lnp)) break;
+        if (lnp < best - 1.0) break;
+        if (lnp > best) {
+            best = lnp;
+            ib = a;
+        }
+    }
+    double dl = 0.0;
+    if (ib >= 0) {
+        const double *tp = trial + ((size_t)s * NALPHA + ib) * 8;
+        for (int i = 0; i < 8; ++i) params[8 * s + i] = tp[i];
+        dl = best - lnp0;
+        chisq[s] = chi2ls[s * NALPHA + ib];
+    } else {
+        chisq[s] = chi0;
+    }
+    dlnp_out[s] = dl;
+    const int it = iters[s] + 1;
+    iters[s] = it;
+    const bool limited = steplim[s] || n == 0;
+    const bool done = (!limited && dl <= dchisq) || (limited && dl <= 1e-6) || it >= max_iter;
+    if (done) active[s] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+static int falloc(trh_fitter *f, T **p, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (n ? n : 1)) != hipSuccess)
+        return set_err(TRH_E_NOMEM, "fitter allocation of %zu bytes failed", sizeof(T) * n);
+    f->allocs.push_back(q);
+    *p = (T *)q;
+    return TRH_OK;
+}
+
+static void host_profiles(FitProfiles &pr) {
+    // mixture_profiles.py:17-38, normalised as ExpGalaxy/DevGalaxy do
+    // (galaxy.py:729, 748)
+    const double ea[6] = {2.34853813e-03, 3.07995260e-02, 2.23364214e-01,
+                          1.17949102e+00, 4.33873750e+00, 5.99820770e+00};
+    const double ev[6] = {1.20078965e-03, 8.84526493e-03, 3.91463084e-02,
+                          1.39976817e-01, 4.60962500e-01, 1.50159566e+00};
+    const double da[8] = {4.26347652e-02, 2.40127183e-01, 6.85907632e-01, 1.51937350e+00,
+                          2.83627243e+00, 4.46467501e+00, 5.72440830e+00, 5.60989349e+00};
+    const double dv[8] = {2.23759216e-04, 1.00220099e-03, 4.18731126e-03, 1.69432589e-02,
+                          6.84850479e-02, 2.87207080e-01, 1.33320254e+00, 8.40215071e+00};
+    double s = 0;
+    for (int i = 0; i < 6; ++i) s += ea[i];
+    double a6[6], s2 = 0;
+    for (int i = 0; i < 6; ++i) {
+        a6[i] = ea[i] / s;
+        s2 += a6[i];
+    }
+    for (int i = 0; i < 6; ++i) {
+        pr.amp[0][i] = a6[i] / s2;
+        pr.var[0][i] = ev[i];
+    }
+    pr.K[0] = 6;
+    const double core = 0.010233;
+    double sd = 0;
+    for (int i = 0; i < 8; ++i) sd += da[i];
+    double a9[9];
+    a9[0] = core;
+    for (int i = 0; i < 8; ++i) a9[i + 1] = da[i] * (1.0 - core) / sd;
+    double s9 = 0;
+    for (int i = 0; i < 9; ++i) s9 += a9[i];
+    for (int i = 0; i < 9; ++i) pr.amp[1][i] = a9[i] / s9;
+    pr.var[1][0] = 0.0;
+    for (int i = 0; i < 8; ++i) pr.var[1][i + 1] = dv[i];
+    pr.K[1] = 9;
+}
+
+extern "C" int trh_fitter_destroy(trh_fitter *f) {
+    if (!f) return TRH_OK;
+    (void)hipSetDevice(f->h->device);
+    for (void *p : f->allocs) (void)hipFree(p);
+    if (f->have_ev) {
+        for (int i = 0; i < 6; ++i) (void)hipEventDestroy(f->ev[i]);
+        for (int i = 0; i < TRH_NSLOT; ++i)
+            for (int j = 0; j < 2; ++j) (void)hipEventDestroy(f->pev[i][j]);
+    }
+    delete f;
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const int32_t *kind,
+                                 const int32_t *P, const int64_t *pix_off, const float *data,
+                                 const float *inverr, const float *psf_stamp, int ph, int pw,
+                                 const double *cdi) {
+    if (!h || !fp || S <= 0 || !kind || !P || !pix_off || !data || !inverr || !psf_stamp)
+        return set_err(TRH_E_ARG, "trh_fitter_create: bad arguments");
+    if ((ph % 2) == 0 || (pw % 2) == 0)
+        return set_err(TRH_E_ARG, "trh_fitter_create: PSF stamp must have odd size");
+    int rc;
+    if ((rc = hip_check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    trh_fitter *f = new trh_fitter();
+    f->h = h;
+    f->S = S;
+    f->ph = ph;
+    f->pw = pw;
+    f->data = data;
+    f->inverr = inverr;
+    if (cdi)
+        for (int i = 0; i < 4; ++i) f->cdi[i] = cdi[i];
+    std::vector<int64_t> roff(S);
+    int64_t tot = 0;
+    int maxP_sz[4] = {0, 0, 0, 0};
+    for (int s = 0; s < S; ++s) {
+        if (kind[s] < 0 || kind[s] > 2 || P[s] < 3 || P[s] > 4096) {
+            trh_fitter_destroy(f);
+            return set_err(TRH_E_ARG, "trh_fitter_create: source %d has kind %d / P %d", s, kind[s],
+                           P[s]);
+        }
+        roff[s] = tot;
+        tot += (int64_t)(kind[s] ? 4 : 1) * P[s] * P[s];
+        if (kind[s]) {
+            ++f->n_gal;
+            for (int b = 0; b < 4; ++b)
+                if (P[s] < kSizes[b] && P[s] > maxP_sz[b]) maxP_sz[b] = P[s];
+        } else {
+            ++f->n_pt;
+            if (P[s] > f->maxP_pt) f->maxP_pt = P[s];
+        }
+    }
+    for (int b = 0; b < 4; ++b) {
+        int mr = max_rows_for(kSizes[b], h->lds_max);
+        if (maxP_sz[b] > 0 && maxP_sz[b] < mr) mr = maxP_sz[b];
+        if (mr < 8) mr = 8;
+        f->maxrows[b] = mr;
+    }
+    if (f->n_pt && point_lds_bytes(ph, pw, f->maxP_pt) + sizeof(RenderScratch) > (size_t)h->lds_max) {
+        trh_fitter_destroy(f);
+        return set_err(TRH_E_UNSUPPORTED, "point-source patch width %d too large for LDS", f->maxP_pt);
+    }
+    const int cap_jac = 4 * S, cap_ls = NALPHA * S;
+    if ((rc = falloc(f, &f->kind, S)) || (rc = falloc(f, &f->P, S)) ||
+        (rc = falloc(f, &f->pix_off, S)) || (rc = falloc(f, &f->render_off, S)) ||
+        (rc = falloc(f, &f->stamp, (size_t)ph * pw)) || (rc = falloc(f, &f->renders, tot)) ||
+        (rc = falloc(f, &f->params, 8 * (size_t)S)) ||
+        (rc = falloc(f, &f->trial, 8 * (size_t)S * NALPHA)) || (rc = falloc(f, &f->X, 8 * (size_t)S)) ||
+        (rc = falloc(f, &f->neq, 32 * (size_t)S)) || (rc = falloc(f, &f->chi2ls, (size_t)S * NALPHA)) ||
+        (rc = falloc(f, &f->nsteps, S)) || (rc = falloc(f, &f->steplim, S)) ||
+        (rc = falloc(f, &f->active, S)) || (rc = falloc(f, &f->iters, S)) ||
+        (rc = falloc(f, &f->chisq, S)) || (rc = falloc(f, &f->dlnp, S)) ||
+        (rc = falloc(f, &f->err, 1)) || (rc = falloc(f, &f->prof, 1)) ||
+        (rc = falloc(f, &f->jobs_jac, (size_t)cap_jac)) || (rc = falloc(f, &f->jobs_ls, (size_t)cap_ls)) ||
+        (rc = falloc(f, &f->list_jac, (size_t)NBUCKET * cap_jac)) ||
+        (rc = falloc(f, &f->list_ls, (size_t)NBUCKET * cap_ls)) || (rc = falloc(f, &f->cnt, 16))) {
+        trh_fitter_destroy(f);
+        return rc;
+    }
+    for (int b = 0; b < 4; ++b) {
+        const int sz = kSizes[b];
+        if ((rc = falloc(f, &f->spec[b], (size_t)2 * sz * (sz / 2 + 1)))) {
+            trh_fitter_destroy(f);
+            return rc;
+        }
+    }
+    FitProfiles pr;
+    host_profiles(pr);
+    std::vector<int32_t> ones(S, 1), zeros(S, 0);
+    const int32_t zero = 0;
+    if ((rc = hip_check(hipMemcpy(f->kind, kind, 4 * (size_t)S, hipMemcpyHostToDevice), "copy kind")) ||
+        (rc = hip_check(hipMemcpy(f->P, P, 4 * (size_t)S, hipMemcpyHostToDevice), "copy P")) ||
+        (rc = hip_check(hipMemcpy(f->pix_off, pix_off, 8 * (size_t)S, hipMemcpyHostToDevice), "copy pix_off")) ||
+        (rc = hip_check(hipMemcpy(f->render_off, roff.data(), 8 * (size_t)S, hipMemcpyHostToDevice), "copy roff")) ||
+        (rc = hip_check(hipMemcpy(f->stamp, psf_stamp, 4 * (size_t)ph * pw, hipMemcpyHostToDevice), "copy stamp")) ||
+        (rc = hip_check(hipMemcpy(f->prof, &pr, sizeof(pr), hipMemcpyHostToDevice), "copy profiles")) ||
+        (rc = hip_check(hipMemcpy(f->active, ones.data(), 4 * (size_t)S, hipMemcpyHostToDevice), "copy active")) ||
+        (rc = hip_check(hipMemcpy(f->iters, zeros.data(), 4 * (size_t)S, hipMemcpyHostToDevice), "copy iters")) ||
+        (rc = hip_check(hipMemcpy(f->err, &zero, 4, hipMemcpyHostToDevice), "copy err"))) {
+        trh_fitter_destroy(f);
+        return rc;
+    }
+    for (int b = 0; b < 4; ++b) {
+        if ((rc = trh_psf_spectrum_batch(h, f->stamp, 1, ph, pw, kSizes[b], f->spec[b], &f->cx[b],
+                                         &f->cy[b], 0))) {
+            trh_fitter_destroy(f);
+            return rc;
+        }
+    }
+    for (int i = 0; i < 6; ++i) (void)hipEventCreate(&f->ev[i]);
+    for (int i = 0; i < TRH_NSLOT; ++i)
+        for (int j = 0; j < 2; ++j) (void)hipEventCreate(&f->pev[i][j]);
+    f->have_ev = true;
+    if ((rc = falloc(f, &f->acc, 2 * TRH_NSLOT))) {
+        trh_fitter_destroy(f);
+        return rc;
+    }
+    if ((rc = hip_check(hipDeviceSynchronize(), "fitter create sync"))) {
+        trh_fitter_destroy(f);
+        return rc;
+    }
+    *fp = f;
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_set_params(trh_fitter *f, const double *params, void *stream) {
+    if (!f || !params) return set_err(TRH_E_ARG, "trh_fitter_set_params: null argument");
+    std::vector<double> p8((size_t)8 * f->S, 0.0);
+    for (int s = 0; s < f->S; ++s)
+        for (int i = 0; i < 6; ++i) p8[8 * (size_t)s + i] = params[6 * (size_t)s + i];
+    std::vector<int32_t> ones(f->S, 1), zeros(f->S, 0);
+    int rc;
+    if ((rc = hip_check(hipMemcpy(f->params, p8.data(), 8 * p8.size(), hipMemcpyHostToDevice), "set params")))
+        return rc;
+    if ((rc = hip_check(hipMemcpy(f->active, ones.data(), 4 * (size_t)f->S, hipMemcpyHostToDevice), "reset active")))
+        return rc;
+    return hip_check(hipMemcpy(f->iters, zeros.data(), 4 * (size_t)f->S, hipMemcpyHostToDevice), "reset iters");
+}
+
+extern "C" int trh_fitter_get_params(trh_fitter *f, double *params, void *stream) {
+    if (!f || !params) return set_err(TRH_E_ARG, "trh_fitter_get_params: null argument");
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "sync"))) return rc;
+    std::vector<double> p8((size_t)8 * f->S);
+    if ((rc = hip_check(hipMemcpy(p8.data(), f->params, 8 * p8.size(), hipMemcpyDeviceToHost), "get params")))
+        return rc;
+    for (int s = 0; s < f->S; ++s)
+        for (int i = 0; i < 6; ++i) params[6 * (size_t)s + i] = p8[8 * (size_t)s + i];
+    return TRH_OK;
+}
+
+static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *list,
+                        const int32_t *count, int bound, float *out, double *chi2,
+                        hipStream_t st) {
+    if (bound <= 0) return TRH_OK;
+    const size_t lds = point_lds_bytes(f->ph, f->pw, f->maxP_pt);
+    int wg = (int)((160 * 1024) / (lds + sizeof(RenderScratch)));
+    if (wg < 1) wg = 1;
+    if (wg > 8) wg = 8;
+    int grid = f->h->n_cu * wg;
+    if (bound < grid) grid = bound;
+    hipLaunchKernelGGL(k_render_point<0>, dim3(grid), dim3(256), lds, st, jobs, list, count, bound,
+                       f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data, f->inverr, chi2);
+    return hip_check(hipGetLastError(), "k_render_point");
+}
+
+// Record the start / end event of launch slot `k` when profiling.
+#define SLOT_BEGIN(k) do { if (f->profiling) (void)hipEventRecord(f->pev[k][0], st); } while (0)
+#define SLOT_END(k) do { if (f->profiling) (void)hipEventRecord(f->pev[k][1], st); } while (0)
+
+extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream) {
+    if (!f) return set_err(TRH_E_ARG, "trh_fitter_iterate: null fitter");
+    hipStream_t st = (hipStream_t)stream;
+    const int S = f->S;
+    const int cap_jac = 4 * S, cap_ls = NALPHA * S;
+    const int tb = 256, nb = (S + tb - 1) / tb;
+    unsigned long long *acc = f->profiling ? f->acc : nullptr;
+    int rc;
+    (void)hipEventRecord(f->ev[0], st);
+    SLOT_BEGIN(12);
+    hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
+    hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->render_off,
+                       f->params, f->active, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
+                       cap_jac, f->cnt, f->err, acc);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
+    SLOT_END(12);
+    for (int b = 0; b < 4 && f->n_gal; ++b) {
+        SLOT_BEGIN(b);
+        if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_jac, f->list_jac + (size_t)b * cap_jac,
+                                 f->cnt + b, 4 * f->n_gal, f->maxrows[b], f->spec[b], f->renders,
+                                 nullptr, nullptr, nullptr, st)))
+            return rc;
+        SLOT_END(b);
+    }
+    SLOT_BEGIN(4);
+    if ((rc = launch_point(f, f->jobs_jac, f->list_jac + (size_t)4 * cap_jac, f->cnt + 4, f->n_pt,
+                           f->renders, nullptr, st)))
+        return rc;
+    SLOT_END(4);
+    (void)hipEventRecord(f->ev[1], st);
+    SLOT_BEGIN(5);
+    {
+        int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
+        hipLaunchKernelGGL(k_fit_normal_eq, dim3(grid), dim3(256), 0, st, S, f->kind, f->P,
+                           f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
+                           f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq);
+        if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq"))) return rc;
+    }
+    SLOT_END(5);
+    (void)hipEventRecord(f->ev[2], st);
+    SLOT_BEGIN(6);
+    hipLaunchKernelGGL(k_fit_solve, dim3((S + 63) / 64), dim3(64), 0, st, S, f->kind, f->neq,
+                       f->params, f->active, lambda, f->X, f->trial, f->nsteps, f->steplim);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_solve"))) return rc;
+    SLOT_END(6);
+    (void)hipEventRecord(f->ev[3], st);
+    hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->pix_off,
+                       f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw, f->jobs_ls,
+                       f->list_ls, cap_ls, f->cnt + 8, f->err, acc);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
+    for (int b = 0; b < 4 && f->n_gal; ++b) {
+        SLOT_BEGIN(7 + b);
+        if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_ls, f->list_ls + (size_t)b * cap_ls,
+                                 f->cnt + 8 + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
+                                 nullptr, f->data, f->inverr, f->chi2ls, st)))
+            return rc;
+        SLOT_END(7 + b);
+    }
+    SLOT_BEGIN(11);
+    if ((rc = launch_point(f, f->jobs_ls, f->list_ls + (size_t)4 * cap_ls, f->cnt + 12,
+                           NALPHA * f->n_pt, nullptr, f->chi2ls, st)))
+        return rc;
+    SLOT_END(11);
+    (void)hipEventRecord(f->ev[4], st);
+    hipLaunchKernelGGL(k_fit_accept, dim3(nb), dim3(tb), 0, st, S, f->neq, f->trial, f->nsteps,
+                       f->steplim, f->chi2ls, f->params, f->active, f->iters, f->chisq, f->dlnp,
+                       dchisq, 100);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_accept"))) return rc;
+    (void)hipEventRecord(f->ev[5], st);
+    if (f->profiling) {
+        // accumulate this iteration's per-slot times (synchronises the stream)
+        if ((rc = hip_check(hipEventSynchronize(f->ev[5]), "profile sync"))) return rc;
+        int32_t cnt[16];
+        if ((rc = hip_check(hipMemcpy(cnt, f->cnt, sizeof(cnt), hipMemcpyDeviceToHost), "read counts")))
+            return rc;
+        for (int k = 0; k < TRH_NSLOT; ++k) {
+            const bool fft = (k < 4) || (k >= 7 && k < 11);
+            const bool launched = fft ? f->n_gal > 0 : ((k == 4 || k == 11) ? f->n_pt > 0 : true);
+            if (!launched) continue;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, f->pev[k][0], f->pev[k][1]) != hipSuccess) continue;
+            f->tot_ms[k] += ms;
+            f->tot_launch[k] += 1;
+            if (fft) {
+                const int b = k < 4 ? k : k - 7;
+                const int njobs = k < 4 ? cnt[b] : cnt[8 + b];
+                const int sz = kSizes[b];
+                if (njobs > 0) f->tot_extra_bytes[k] += 8.0 * sz * (sz / 2 + 1);  // PSF spectrum
+            }
+        }
+    }
+    return TRH_OK;
+}
+
+__global__ void k_fit_zero_u64(unsigned long long *a, int n) {
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0ull;
+}
+
+extern "C" int trh_fitter_profile(trh_fitter *f, int enable) {
+    if (!f) return set_err(TRH_E_ARG, "trh_fitter_profile: null fitter");
+    f->profiling = enable != 0;
+    for (int k = 0; k < TRH_NSLOT; ++k) f->tot_ms[k] = f->tot_launch[k] = f->tot_extra_bytes[k] = 0.0;
+    hipLaunchKernelGGL(k_fit_zero_u64, dim3(1), dim3(64), 0, 0, f->acc, 2 * TRH_NSLOT);
+    int rc = hip_check(hipGetLastError(), "k_fit_zero_u64");
+    if (rc) return rc;
+    return hip_check(hipDeviceSynchronize(), "profile reset sync");
+}
+
+extern "C" int trh_fitter_profile_read(trh_fitter *f, double *out) {
+    if (!f || !out) return set_err(TRH_E_ARG, "trh_fitter_profile_read: null argument");
+    unsigned long long a[2 * TRH_NSLOT];
+    int rc;
+    if ((rc = hip_check(hipDeviceSynchronize(), "profile read sync"))) return rc;
+    if ((rc = hip_check(hipMemcpy(a, f->acc, sizeof(a), hipMemcpyDeviceToHost), "read acc"))) return rc;
+    for (int k = 0; k < TRH_NSLOT; ++k) {
+        out[4 * k + 0] = f->tot_ms[k];
+        out[4 * k + 1] = f->tot_launch[k];
+        out[4 * k + 2] = (double)a[2 * k];
+        out[4 * k + 3] = (double)a[2 * k + 1] + f->tot_extra_bytes[k];
+    }
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, void *stream) {
+    if (!f || !out) return set_err(TRH_E_ARG, "trh_fitter_render_models: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int S = f->S;
+    const int cap_jac = 4 * S;
+    int rc;
+    hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
+    hipLaunchKernelGGL(k_fit_prep_model, dim3((S + 255) / 256), dim3(256), 0, st, S, f->kind, f->P,
+                       f->pix_off, f->params, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
+                       cap_jac, f->cnt, f->err);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_prep_model"))) return rc;
+    for (int b = 0; b < 4 && f->n_gal; ++b) {
+        if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_jac, f->list_jac + (size_t)b * cap_jac,
+                                 f->cnt + b, f->n_gal, f->maxrows[b], f->spec[b], out, nullptr,
+                                 nullptr, nullptr, st)))
+            return rc;
+    }
+    return launch_point(f, f->jobs_jac, f->list_jac + (size_t)4 * cap_jac, f->cnt + 4, f->n_pt, out,
+                        nullptr, st);
+}
+
+extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations,
+                                 double *chisq, double *dlnp, int32_t *n_active, void *stream) {
+    if (!f) return set_err(TRH_E_ARG, "trh_fitter_status: null fitter");
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "sync"))) return rc;
+    int32_t err = 0;
+    if ((rc = hip_check(hipMemcpy(&err, f->err, 4, hipMemcpyDeviceToHost), "read err"))) return rc;
+    if (err) return set_err(TRH_E_UNSUPPORTED, "a render needed FFT size %d (supported: 32..256)", err);
+    std::vector<int32_t> a(f->S);
+    if ((rc = hip_check(hipMemcpy(a.data(), f->active, 4 * (size_t)f->S, hipMemcpyDeviceToHost), "read active")))
+        return rc;
+    if (active) std::copy(a.begin(), a.end(), active);
+    if (n_active) {
+        int n = 0;
+        for (int v : a) n += v ? 1 : 0;
+        *n_active = n;
+    }
+    if (iterations &&
+        (rc = hip_check(hipMemcpy(iterations, f->iters, 4 * (size_t)f->S, hipMemcpyDeviceToHost), "read iters")))
+        return rc;
+    if (chisq && (rc = hip_check(hipMemcpy(chisq, f->chisq, 8 * (size_t)f->S, hipMemcpyDeviceToHost), "read chisq")))
+        return rc;
+    if (dlnp && (rc = hip_check(hipMemcpy(dlnp, f->dlnp, 8 * (size_t)f->S, hipMemcpyDeviceToHost), "read dlnp")))
+        return rc;
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_debug(trh_fitter *f, const float **renders, const int64_t **render_off,
+                                const double **normal_eq, const double **step_x) {
+    if (!f) return set_err(TRH_E_ARG, "trh_fitter_debug: null fitter");
+    if (renders) *renders = f->renders;
+    if (render_off) *render_off = f->render_off;
+    if (normal_eq) *normal_eq = f->neq;
+    if (step_x) *step_x = f->X;
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_timing(trh_fitter *f, float *ms6) {
+    if (!f || !ms6) return set_err(TRH_E_ARG, "trh_fitter_timing: null argument");
+    int rc;
+    if ((rc = hip_check(hipEventSynchronize(f->ev[5]), "event sync"))) return rc;
+    for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&ms6[i], f->ev[i], f->ev[i + 1]);
+    (void)hipEventElapsedTime(&ms6[5], f->ev[0], f->ev[5]);
+    return TRH_OK;
+}

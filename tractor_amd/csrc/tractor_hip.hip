@@ -1,0 +1,513 @@
+// libtractor_hip: C ABI over the MI355X render kernels (handle, PSF spectra,
+// batched FFT renders, pixel-space MoG evaluation, Lanczos shifts).
+// The batched fitter lives in fitter.hip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/tractor_hip.h"
+#include "internal.h"
+#include "render_fft.h"
+
+using namespace trh;
+
+static thread_local std::string g_err;
+
+namespace trh {
+int set_err(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return TRH_OK;
+    return set_err(TRH_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int tw_offset(int n) {
+    // tables for N = 32, 64, 128, 256, 512 stored back to back (N/2 entries)
+    switch (n) {
+    case 32: return 0;
+    case 64: return 16;
+    case 128: return 48;
+    case 256: return 112;
+    case 512: return 240;
+    }
+    return -1;
+}
+
+int pad_center(int sz, int p) {
+    // PixelizedPSF._padInImage (tractor/psf.py:225-238)
+    if (sz >= p) return (sz - p) / 2 + p / 2;
+    return p / 2 - (p - sz) / 2;
+}
+
+int cu_count(trh_handle *h) { return h->n_cu; }
+}  // namespace trh
+
+extern "C" const char *trh_last_error(void) { return g_err.c_str(); }
+extern "C" int trh_version(void) { return 1; }
+
+extern "C" int trh_struct_sizes(int *fft_job, int *mog_job) {
+    if (fft_job) *fft_job = (int)sizeof(trh_fft_job);
+    if (mog_job) *mog_job = (int)sizeof(trh_mog_job);
+    return TRH_OK;
+}
+
+extern "C" int trh_copy_to_host(void *dst, const void *src, uint64_t bytes, void *stream) {
+    if (!dst || !src) return set_err(TRH_E_ARG, "trh_copy_to_host: null pointer");
+    int rc = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream),
+                       "hipMemcpyAsync D2H");
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+__global__ void k_twiddles(float2 *tw) {
+    // exp(+2 pi i m / N) in FP64, for every table
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ns[5] = {32, 64, 128, 256, 512};
+    const int offs[5] = {0, 16, 48, 112, 240};
+    for (int q = 0; q < 5; ++q) {
+        const int N = Ns[q];
+        if (t < N / 2) {
+            double s, c;
+            sincospi(2.0 * t / N, &s, &c);
+            tw[offs[q] + t] = make_float2((float)c, (float)s);
+        }
+    }
+}
+
+extern "C" int trh_create(trh_handle **hp, int device) {
+    if (!hp) return set_err(TRH_E_ARG, "trh_create: null handle pointer");
+    int rc;
+    if ((rc = hip_check(hipSetDevice(device), "hipSetDevice"))) return rc;
+    trh_handle *h = new trh_handle();
+    h->device = device;
+    hipDeviceProp_t prop;
+    if ((rc = hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties"))) {
+        delete h;
+        return rc;
+    }
+    h->n_cu = prop.multiProcessorCount;
+    h->lds_max = (int)prop.sharedMemPerBlock;
+    if ((rc = hip_check(hipMalloc(&h->tw, sizeof(float2) * 496), "hipMalloc twiddles"))) {
+        delete h;
+        return rc;
+    }
+    hipLaunchKernelGGL(k_twiddles, dim3(1), dim3(256), 0, 0, h->tw);
+    if ((rc = hip_check(hipGetLastError(), "k_twiddles"))) return rc;
+    if ((rc = hip_check(hipDeviceSynchronize(), "twiddle sync"))) return rc;
+    *hp = h;
+    return TRH_OK;
+}
+
+extern "C" int trh_destroy(trh_handle *h) {
+    if (!h) return TRH_OK;
+    (void)hipSetDevice(h->device);
+    if (h->tw) (void)hipFree(h->tw);
+    if (h->scratch) (void)hipFree(h->scratch);
+    delete h;
+    return TRH_OK;
+}
+
+namespace trh {
+int ensure_scratch(trh_handle *h, size_t bytes) {
+    if (h->scratch_bytes >= bytes) return TRH_OK;
+    if (h->scratch) (void)hipFree(h->scratch);
+    h->scratch = nullptr;
+    h->scratch_bytes = 0;
+    if (hipMalloc(&h->scratch, bytes) != hipSuccess)
+        return set_err(TRH_E_NOMEM, "scratch allocation of %zu bytes failed", bytes);
+    h->scratch_bytes = bytes;
+    return TRH_OK;
+}
+}  // namespace trh
+
+// ---------------------------------------------------------------------------
+// PSF spectra: rfft2 of the padded stamp (psf.py:249-280), rows then columns.
+// ---------------------------------------------------------------------------
+template <int SZ>
+__global__ void __launch_bounds__(256)
+k_psf_rows(const float *__restrict__ stamps, int ph, int pw, float2 *__restrict__ out,
+           const float2 *__restrict__ twg, int rp) {
+    constexpr int NV = SZ / 2 + 1;
+    constexpr int LOG2 = Log2<SZ>::v;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *work = tw + SZ / 2;
+    for (int i = threadIdx.x; i < SZ / 2; i += blockDim.x) tw[i] = twg[i];
+    const int n = blockIdx.y;
+    const int pair0 = blockIdx.x * rp;
+    const int np = min(rp, SZ / 2 - pair0);
+    const float *st = stamps + (size_t)n * ph * pw;
+    const int y0 = SZ >= ph ? (SZ - ph) / 2 : -((ph - SZ) / 2);
+    const int x0 = SZ >= pw ? (SZ - pw) / 2 : -((pw - SZ) / 2);
+    for (int idx = threadIdx.x; idx < np * SZ; idx += blockDim.x) {
+        const int pp = idx >> LOG2;
+        const int x = idx & (SZ - 1);
+        const int ya = 2 * (pair0 + pp);
+        float v[2];
+        for (int r = 0; r < 2; ++r) {
+            const int sy = ya + r - y0, sx = x - x0;
+            v[r] = (sy >= 0 && sy < ph && sx >= 0 && sx < pw) ? st[sy * pw + sx] : 0.f;
+        }
+        work[pp * SZ + bitrev<LOG2>(x)] = make_float2(v[0], v[1]);
+    }
+    __syncthreads();
+    fft_dit<SZ, false>(work, np, tw);
+    for (int idx = threadIdx.x; idx < np * NV; idx += blockDim.x) {
+        const int pp = idx / NV;
+        const int k = idx - pp * NV;
+        const float2 zk = work[pp * SZ + (k & (SZ - 1))];
+        const float2 zn = work[pp * SZ + ((SZ - k) & (SZ - 1))];
+        const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+        const float2 B = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        const int ya = 2 * (pair0 + pp);
+        out[((size_t)n * SZ + ya) * NV + k] = A;
+        out[((size_t)n * SZ + ya + 1) * NV + k] = B;
+    }
+}
+
+template <int SZ>
+__global__ void __launch_bounds__(256)
+k_psf_cols(float2 *__restrict__ spec, const float2 *__restrict__ twg, int cc) {
+    constexpr int NV = SZ / 2 + 1;
+    constexpr int LOG2 = Log2<SZ>::v;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *work = tw + SZ / 2;
+    for (int i = threadIdx.x; i < SZ / 2; i += blockDim.x) tw[i] = twg[i];
+    const int n = blockIdx.y;
+    const int c0 = blockIdx.x * cc;
+    const int nc = min(cc, NV - c0);
+    float2 *S = spec + (size_t)n * SZ * NV;
+    for (int idx = threadIdx.x; idx < nc * SZ; idx += blockDim.x) {
+        const int j = idx / nc;
+        const int c = idx - j * nc;
+        work[c * SZ + bitrev<LOG2>(j)] = S[j * NV + c0 + c];
+    }
+    __syncthreads();
+    fft_dit<SZ, false>(work, nc, tw);
+    for (int idx = threadIdx.x; idx < nc * SZ; idx += blockDim.x) {
+        const int j = idx / nc;
+        const int c = idx - j * nc;
+        S[j * NV + c0 + c] = work[c * SZ + j];
+    }
+}
+
+template <int SZ>
+static int psf_spectrum_sz(trh_handle *h, const float *stamps, int n, int ph, int pw, float *out,
+                           hipStream_t st) {
+    const int rp = SZ >= 256 ? 8 : 16;
+    const int cc = SZ >= 256 ? 8 : 16;
+    const float2 *tw = h->tw + tw_offset(SZ);
+    size_t lds = sizeof(float2) * (SZ / 2 + (size_t)rp * SZ);
+    hipLaunchKernelGGL(k_psf_rows<SZ>, dim3((SZ / 2 + rp - 1) / rp, n), dim3(256), lds, st,
+                       stamps, ph, pw, (float2 *)out, tw, rp);
+    int rc = hip_check(hipGetLastError(), "k_psf_rows");
+    if (rc) return rc;
+    lds = sizeof(float2) * (SZ / 2 + (size_t)cc * SZ);
+    hipLaunchKernelGGL(k_psf_cols<SZ>, dim3((SZ / 2 + 1 + cc - 1) / cc, n), dim3(256), lds, st,
+                       (float2 *)out, tw, cc);
+    return hip_check(hipGetLastError(), "k_psf_cols");
+}
+
+extern "C" int trh_psf_spectrum_batch(trh_handle *h, const float *stamps, int n, int ph, int pw,
+                                      int sz, float *out, int *cx, int *cy, void *stream) {
+    if (!h || !stamps || !out || n <= 0 || ph <= 0 || pw <= 0)
+        return set_err(TRH_E_ARG, "trh_psf_spectrum_batch: bad arguments");
+    if ((ph % 2) == 0 || (pw % 2) == 0)
+        return set_err(TRH_E_ARG, "trh_psf_spectrum_batch: PSF stamp must have odd size (psf.py:91-92)");
+    hipStream_t st = (hipStream_t)stream;
+    if (cx) *cx = pad_center(sz, pw);
+    if (cy) *cy = pad_center(sz, ph);
+    switch (sz) {
+    case 32: return psf_spectrum_sz<32>(h, stamps, n, ph, pw, out, st);
+    case 64: return psf_spectrum_sz<64>(h, stamps, n, ph, pw, out, st);
+    case 128: return psf_spectrum_sz<128>(h, stamps, n, ph, pw, out, st);
+    case 256: return psf_spectrum_sz<256>(h, stamps, n, ph, pw, out, st);
+    case 512: return psf_spectrum_sz<512>(h, stamps, n, ph, pw, out, st);
+    }
+    return set_err(TRH_E_UNSUPPORTED, "trh_psf_spectrum_batch: sz=%d not in {32..512}", sz);
+}
+
+// ---------------------------------------------------------------------------
+// Batched FFT renders
+// ---------------------------------------------------------------------------
+namespace trh {
+void fft_launch_shape(int sz, int maxrows, int n_cu, int lds_max, int *ch, size_t *lds, int *grid,
+                      int njobs) {
+    int c;
+    switch (sz) {
+    case 32: c = 17; break;
+    case 64: c = 33; break;
+    case 128: c = 16; break;
+    default: c = 8; break;
+    }
+    size_t b = fft_lds_bytes(sz, c, maxrows);
+    while (b + sizeof(RenderScratch) > (size_t)lds_max && c > 1) {
+        c /= 2;
+        b = fft_lds_bytes(sz, c, maxrows);
+    }
+    *ch = c;
+    *lds = b;
+    const size_t per_cu = 160 * 1024;
+    int wg_per_cu = (int)(per_cu / (b + sizeof(RenderScratch)));
+    if (wg_per_cu < 1) wg_per_cu = 1;
+    if (wg_per_cu > 8) wg_per_cu = 8;
+    int g = n_cu * wg_per_cu;
+    if (njobs > 0 && njobs < g) g = njobs;
+    *grid = g < 1 ? 1 : g;
+}
+
+int max_rows_for(int sz, int lds_max) {
+    // rows of the y-Lanczos output kept in LDS per pass
+    const int NV = sz / 2 + 1;
+    int mr = sz;
+    while (mr > 8 && fft_lds_bytes(sz, 8, mr) + sizeof(RenderScratch) > (size_t)lds_max) mr /= 2;
+    (void)NV;
+    return mr;
+}
+
+template <int SZ>
+int launch_fft(trh_handle *h, const trh_fft_job *jobs, const int *list, const int *count,
+               int njobs_bound, int maxrows, const float *psf_spec, float *out, const float *data,
+               const float *inverr, double *chi2, hipStream_t st) {
+    int ch, grid;
+    size_t lds;
+    fft_launch_shape(SZ, maxrows, h->n_cu, h->lds_max, &ch, &lds, &grid, njobs_bound);
+    hipLaunchKernelGGL(k_render_fft<SZ>, dim3(grid), dim3(256), lds, st, jobs, list, count,
+                       njobs_bound, (const float2 *)psf_spec, h->tw + tw_offset(SZ), ch, maxrows,
+                       out, data, inverr, chi2);
+    return hip_check(hipGetLastError(), "k_render_fft");
+}
+
+template int launch_fft<32>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
+                            const float *, float *, const float *, const float *, double *,
+                            hipStream_t);
+template int launch_fft<64>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
+                            const float *, float *, const float *, const float *, double *,
+                            hipStream_t);
+template int launch_fft<128>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
+                             const float *, float *, const float *, const float *, double *,
+                             hipStream_t);
+template int launch_fft<256>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
+                             const float *, float *, const float *, const float *, double *,
+                             hipStream_t);
+
+int launch_fft_any(trh_handle *h, int sz, const trh_fft_job *jobs, const int *list,
+                   const int *count, int njobs_bound, int maxrows, const float *psf_spec,
+                   float *out, const float *data, const float *inverr, double *chi2,
+                   hipStream_t st) {
+    switch (sz) {
+    case 32:
+        return launch_fft<32>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
+                              inverr, chi2, st);
+    case 64:
+        return launch_fft<64>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
+                              inverr, chi2, st);
+    case 128:
+        return launch_fft<128>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
+                               inverr, chi2, st);
+    case 256:
+        return launch_fft<256>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
+                               inverr, chi2, st);
+    }
+    return set_err(TRH_E_UNSUPPORTED, "FFT render size %d not in {32,64,128,256}", sz);
+}
+}  // namespace trh
+
+extern "C" int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njobs,
+                                    const float *psf_spec, float *out, const float *data,
+                                    const float *inverr, double *chi2, void *stream) {
+    if (!h || (!jobs && njobs > 0) || !psf_spec)
+        return set_err(TRH_E_ARG, "trh_render_fft_batch: null argument");
+    if (njobs <= 0) return TRH_OK;
+    const int maxrows = max_rows_for(sz, h->lds_max);
+    return launch_fft_any(h, sz, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out, data,
+                          inverr, chi2, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Pixel-space MoG (c_gauss_2d_grid / c_gauss_2d_masked)
+// ---------------------------------------------------------------------------
+#define TRH_MOG_CHUNK 128
+
+__global__ void __launch_bounds__(256)
+k_render_mog(const trh_mog_job *__restrict__ jobs, int njobs, const float *__restrict__ amp,
+             const int32_t *__restrict__ cxi, const int32_t *__restrict__ cyi,
+             const float *__restrict__ cxf, const float *__restrict__ cyf,
+             const float *__restrict__ var, const uint8_t *__restrict__ mask,
+             float *__restrict__ out, float *__restrict__ dxo, float *__restrict__ dyo) {
+    // per component: scale, e0, e1, e2 (exponent = e0 dx^2 + e1 dx dy + e2 dy^2)
+    __shared__ float4 C[TRH_MOG_CHUNK];
+    __shared__ int2 CI[TRH_MOG_CHUNK];
+    __shared__ float2 CF[TRH_MOG_CHUNK];
+    const int jid = blockIdx.y;
+    if (jid >= njobs) return;
+    const trh_mog_job J = jobs[jid];
+    const int npix = J.w * J.h;
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = pix < npix;
+    const int oy = live ? pix / J.w : 0;
+    const int ox = live ? pix - oy * J.w : 0;
+    const int64_t idx = J.out_offset + (int64_t)oy * J.out_stride + ox;
+    bool on = live;
+    if (live && J.mask_offset >= 0 && mask) on = mask[J.mask_offset + (int64_t)oy * J.out_stride + ox] != 0;
+    const float cut = J.cutoff_mode == 0 ? -50.f : -30.f;
+    float r = 0.f, xd = 0.f, yd = 0.f;
+    for (int c0 = 0; c0 < J.K; c0 += TRH_MOG_CHUNK) {
+        const int nc = min(TRH_MOG_CHUNK, J.K - c0);
+        __syncthreads();
+        if (threadIdx.x < nc) {
+            const int c = J.comp0 + c0 + threadIdx.x;
+            float4 q;
+            if (J.cutoff_mode == 0) {
+                // c_gauss_2d_grid constants in FP64 (mix.i:482-491)
+                const double V0 = var[3 * c], V1 = var[3 * c + 1], V2 = var[3 * c + 2];
+                const double det = V0 * V2 - V1 * V1;
+                q.x = (float)(amp[c] / sqrt(4.0 * M_PI * M_PI * det));
+                q.y = (float)(-0.5 * V2 / det);
+                q.z = (float)(V1 / det);
+                q.w = (float)(-0.5 * V0 / det);
+                if (!(det > 0.0)) q = make_float4(0.f, -1.f, 0.f, -1.f);
+            } else {
+                // c_gauss_2d_masked constants in FP32 (gauss_masked.c:80-103)
+                const float V0 = var[3 * c], V1 = var[3 * c + 1], V2 = var[3 * c + 2];
+                const float det = V0 * V2 - V1 * V1;
+                const float isc = -0.5f / det;
+                q.y = V2 * isc;
+                q.z = -V1 * isc * 2.0f;
+                q.w = V0 * isc;
+                q.x = amp[c] / sqrtf((float)(4.0 * M_PI * M_PI) * det);
+                if (!(isfinite(V0) && isfinite(V1) && isfinite(V2) && isfinite(q.y) &&
+                      isfinite(q.z) && isfinite(q.w) && isfinite(q.x)))
+                    q = make_float4(0.f, -1.f, -1.f, -1.f);
+            }
+            C[threadIdx.x] = q;
+            CI[threadIdx.x] = make_int2(cxi[c], cyi[c]);
+            CF[threadIdx.x] = make_float2(cxf[c], cyf[c]);
+        }
+        __syncthreads();
+        if (on) {
+            const int gx = J.x0 + ox, gy = J.y0 + oy;
+            for (int k = 0; k < nc; ++k) {
+                const float4 q = C[k];
+                if (q.x == 0.f) continue;
+                const float dx = (float)(gx - CI[k].x) - CF[k].x;
+                const float dy = (float)(gy - CI[k].y) - CF[k].y;
+                const float e = q.y * dx * dx + q.z * dx * dy + q.w * dy * dy;
+                const bool keep = J.cutoff_mode == 0 ? (e > cut) : (e >= cut);
+                if (!keep || !isfinite(e)) continue;
+                const float G = q.x * __expf(e);
+                r += G;
+                if (J.want_derivs) {
+                    xd += -G * (2.f * q.y * dx + q.z * dy);
+                    yd += -G * (2.f * q.w * dy + q.z * dx);
+                }
+            }
+        }
+    }
+    if (!live) return;
+    if (J.mode == TRH_STORE) {
+        out[idx] = J.scale * r;
+        if (J.want_derivs) {
+            if (dxo) dxo[idx] = J.scale * xd;
+            if (dyo) dyo[idx] = J.scale * yd;
+        }
+    } else if (on) {
+        atomicAdd(out + idx, J.scale * r);
+        if (J.want_derivs) {
+            if (dxo) atomicAdd(dxo + idx, J.scale * xd);
+            if (dyo) atomicAdd(dyo + idx, J.scale * yd);
+        }
+    }
+}
+
+// The caller passes the largest window pixel count so the grid can be shaped
+// without reading jobs back.
+extern "C" int trh_render_mog_batch(trh_handle *h, const trh_mog_job *jobs, int njobs,
+                                    int max_pixels, const float *amp, const int32_t *cxi,
+                                    const int32_t *cyi, const float *cxf, const float *cyf,
+                                    const float *var, const uint8_t *mask, float *out,
+                                    float *dx_out, float *dy_out, void *stream) {
+    if (!h || (njobs > 0 && (!jobs || !amp || !cxi || !cyi || !cxf || !cyf || !var || !out)))
+        return set_err(TRH_E_ARG, "trh_render_mog_batch: null argument");
+    if (njobs <= 0 || max_pixels <= 0) return TRH_OK;
+    if (njobs > 65535) return set_err(TRH_E_ARG, "trh_render_mog_batch: at most 65535 jobs per call");
+    dim3 grid((max_pixels + 255) / 256, njobs);
+    hipLaunchKernelGGL(k_render_mog, grid, dim3(256), 0, (hipStream_t)stream, jobs, njobs, amp,
+                       cxi, cyi, cxf, cyf, var, mask, out, dx_out, dy_out);
+    return hip_check(hipGetLastError(), "k_render_mog");
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos-3 shift (lanczos_shift_image): x pass into scratch, y pass into out.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_lanczos_pass(const float *__restrict__ in, float *__restrict__ out, int height, int width,
+               const double *__restrict__ shifts, int along_y) {
+    __shared__ RenderScratch S;
+    const int n = blockIdx.y;
+    const double d = shifts[n];
+    compute_taps(along_y ? 0.0 : d, along_y ? d : 0.0, S);
+    const float *f = along_y ? S.taps + 7 : S.taps;
+    const size_t base = (size_t)n * height * width;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < height * width;
+         p += gridDim.x * blockDim.x) {
+        const int y = p / width, x = p - y * width;
+        float a = 0.f;
+#pragma unroll
+        for (int t = 0; t < 7; ++t) {
+            const int yy = along_y ? y - 3 + t : y;
+            const int xx = along_y ? x : x - 3 + t;
+            if (yy >= 0 && yy < height && xx >= 0 && xx < width)
+                a += f[t] * in[base + (size_t)yy * width + xx];
+        }
+        out[base + p] = a;
+    }
+}
+
+extern "C" int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n,
+                                       int height, int width, const double *dx, const double *dy,
+                                       void *stream) {
+    if (!h || !in || !out || !dx || !dy || n <= 0 || height <= 0 || width <= 0)
+        return set_err(TRH_E_ARG, "trh_lanczos_shift_batch: bad arguments");
+    const size_t bytes = sizeof(float) * (size_t)n * height * width;
+    int rc = ensure_scratch(h, bytes);
+    if (rc) return rc;
+    float *tmp = (float *)h->scratch;
+    const int gx = (height * width + 255) / 256 < 1024 ? (height * width + 255) / 256 : 1024;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_lanczos_pass, dim3(gx, n), dim3(256), 0, st, in, tmp, height, width, dx, 0);
+    if ((rc = hip_check(hipGetLastError(), "k_lanczos_pass x"))) return rc;
+    hipLaunchKernelGGL(k_lanczos_pass, dim3(gx, n), dim3(256), 0, st, tmp, out, height, width, dy, 1);
+    return hip_check(hipGetLastError(), "k_lanczos_pass y");
+}
+
+// ---------------------------------------------------------------------------
+// Batched pixelized-PSF point renders (generic API path)
+// ---------------------------------------------------------------------------
+extern "C" int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, int maxw,
+                                      const float *stamp, int ph, int pw, float *out,
+                                      const float *data, const float *inverr, double *chi2,
+                                      void *stream) {
+    if (!h || !jobs || !stamp || njobs < 0 || maxw <= 0)
+        return set_err(TRH_E_ARG, "trh_render_point_batch: bad arguments");
+    if (njobs == 0) return TRH_OK;
+    const size_t lds = point_lds_bytes(ph, pw, maxw);
+    if (lds + sizeof(RenderScratch) > (size_t)h->lds_max)
+        return set_err(TRH_E_UNSUPPORTED, "point render: stamp %dx%d with window width %d exceeds LDS",
+                       ph, pw, maxw);
+    int grid = njobs < h->n_cu * 4 ? njobs : h->n_cu * 4;
+    hipLaunchKernelGGL(k_render_point<0>, dim3(grid), dim3(256), lds, (hipStream_t)stream, jobs,
+                       nullptr, nullptr, njobs, stamp, ph, pw, maxw, out, data, inverr, chi2);
+    return hip_check(hipGetLastError(), "k_render_point");
+}
