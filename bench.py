@@ -30,9 +30,11 @@ METRIC = ('sources fit/sec at χ² ≤1e-4 rel of FP64; surrogate-init vs plain;
 HBM_PEAK_GBS = 8000.0
 SLOT_NAMES = ['k_render_fft<32> (jacobian)', 'k_render_fft<64> (jacobian)',
               'k_render_fft<128> (jacobian)', 'k_render_fft<256> (jacobian)',
+              'k_render_fft<512> (jacobian)',
               'k_render_point (jacobian)', 'k_fit_normal_eq', 'k_fit_solve',
               'k_render_fft<32> (line search chi2)', 'k_render_fft<64> (line search chi2)',
               'k_render_fft<128> (line search chi2)', 'k_render_fft<256> (line search chi2)',
+              'k_render_fft<512> (line search chi2)',
               'k_render_point (line search chi2)', 'prep']
 
 
@@ -83,8 +85,7 @@ def cpu_baseline(wl, budget_s=15.0, min_sources=4):
         if wl['cfg'] == 4:
             p, nit, chi = O.fit_source(prob, wl['start'][s])
         else:
-            X, _ = O.source_update(prob, wl['start'][s])
-            p, nit, chi = None, 1, None
+            p, nit, chi = O.fit_source(prob, wl['start'][s], steps=1)
         results.append((s, p, chi))
         n += 1
     dt = time.time() - t0

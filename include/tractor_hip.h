@@ -224,11 +224,11 @@ int trh_fitter_debug(trh_fitter *f, const float **renders, const int64_t **rende
  * Per-launch profiling.  trh_fitter_profile(f, 1) resets the counters and
  * makes every later trh_fitter_iterate record HIP events around each launch
  * slot (and synchronise its stream at the end).  trh_fitter_profile_read
- * fills out[13][4] = (total ms, launches, jobs, algorithmic bytes) per slot:
- * 0-3 Jacobian FFT renders at sz 32/64/128/256 (bytes: renders written + PSF
- * spectrum read), 4 Jacobian point renders, 5 normal equations (renders +
- * data + inverr read), 6 solve, 7-10 line-search FFT renders (data + inverr
- * read), 11 line-search point renders, 12 job preparation.
+ * fills out[15][4] = (total ms, launches, jobs, algorithmic bytes) per slot:
+ * 0-4 Jacobian FFT renders at sz 32/64/128/256/512 (bytes: renders written +
+ * PSF spectrum read), 5 Jacobian point renders, 6 normal equations (renders +
+ * data + inverr read), 7 solve, 8-12 line-search FFT renders (data + inverr
+ * read), 13 line-search point renders, 14 job preparation.
  */
 int trh_fitter_profile(trh_fitter *f, int enable);
 int trh_fitter_profile_read(trh_fitter *f, double *out);

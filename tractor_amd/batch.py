@@ -104,7 +104,7 @@ class BatchFitter(object):
         check(self.lib.trh_fitter_profile(self.f, 1 if enable else 0), 'profile')
 
     def profile_read(self):
-        out = np.zeros((13, 4))
+        out = np.zeros((15, 4))
         check(self.lib.trh_fitter_profile_read(self.f, ptr(out)), 'profile_read')
         return out
 

@@ -23,11 +23,20 @@
 using namespace trh;
 
 #define NALPHA 13
-#define NBUCKET 5  // FFT sizes 32, 64, 128, 256 + point sources
-// launch slots: 0-3 jacobian FFT renders (sz 32..256), 4 jacobian point
-// renders, 5 normal equations, 6 solve, 7-10 line-search FFT renders,
-// 11 line-search point renders, 12 job preparation + accept
-#define TRH_NSLOT 13
+#define NFFT 5     // FFT sizes 32, 64, 128, 256, 512
+#define PTB NFFT   // bucket index of point sources
+#define NBUCKET (NFFT + 1)
+#define LS_CNT 8   // line-search counters start at cnt[8]
+// launch slots: 0-4 jacobian FFT renders (sz 32..512), 5 jacobian point
+// renders, 6 normal equations, 7 solve, 8-12 line-search FFT renders,
+// 13 line-search point renders, 14 job preparation
+#define SL_JPT 5
+#define SL_NEQ 6
+#define SL_SOLVE 7
+#define SL_LS 8
+#define SL_LPT 13
+#define SL_PREP 14
+#define TRH_NSLOT 15
 
 struct FitProfiles {
     double amp[2][TRH_MAXK];
@@ -39,15 +48,15 @@ struct trh_fitter {
     trh_handle *h = nullptr;
     int S = 0, n_gal = 0, n_pt = 0, ph = 0, pw = 0;
     double cdi[4] = {1, 0, 0, 1};
-    int maxrows[4] = {8, 8, 8, 8};
+    int maxrows[NFFT] = {8, 8, 8, 8, 8};
     int maxP_pt = 1;
-    int cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
+    int cx[NFFT] = {0, 0, 0, 0, 0}, cy[NFFT] = {0, 0, 0, 0, 0};
     // device
     int32_t *kind = nullptr, *P = nullptr;
     int64_t *pix_off = nullptr, *render_off = nullptr;
     const float *data = nullptr, *inverr = nullptr;
     float *stamp = nullptr;
-    float *spec[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *spec[NFFT] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float *renders = nullptr;
     double *params = nullptr, *trial = nullptr, *X = nullptr, *neq = nullptr, *chi2ls = nullptr;
     int32_t *nsteps = nullptr, *steplim = nullptr, *active = nullptr, *iters = nullptr;
@@ -69,7 +78,7 @@ struct trh_fitter {
     std::vector<void *> allocs;
 };
 
-static const int kSizes[4] = {32, 64, 128, 256};
+static const int kSizes[NFFT] = {32, 64, 128, 256, 512};
 
 __device__ __forceinline__ int sz_bucket(int sz) {
     switch (sz) {
@@ -77,6 +86,7 @@ __device__ __forceinline__ int sz_bucket(int sz) {
     case 64: return 1;
     case 128: return 2;
     case 256: return 3;
+    case 512: return 4;
     }
     return -1;
 }
@@ -167,10 +177,10 @@ __global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
     const unsigned long long PP = (unsigned long long)P * P;
     if (acc) {
         // normal equations read N_r renders + data + inverr, write 28 doubles
-        atomicAdd(&acc[2 * 5], 1ull);
-        atomicAdd(&acc[2 * 5 + 1], (kd ? 4ull * 4 + 8 : 4ull + 8) * PP + 28 * 8);
-        atomicAdd(&acc[2 * 6], 1ull);
-        atomicAdd(&acc[2 * 6 + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
+        atomicAdd(&acc[2 * SL_NEQ], 1ull);
+        atomicAdd(&acc[2 * SL_NEQ + 1], (kd ? 4ull * 4 + 8 : 4ull + 8) * PP + 28 * 8);
+        atomicAdd(&acc[2 * SL_SOLVE], 1ull);
+        atomicAdd(&acc[2 * SL_SOLVE + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
     }
     if (kd == 0) {
         trh_fft_job &J = jobs[4 * s];
@@ -179,11 +189,11 @@ __global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
         J.mode = TRH_STORE;
         J.scale = 1.f;
         J.slot = 0;
-        const int i = atomicAdd(&cnt[4], 1);
-        lists[4 * cap + i] = 4 * s;
+        const int i = atomicAdd(&cnt[PTB], 1);
+        lists[PTB * cap + i] = 4 * s;
         if (acc) {
-            atomicAdd(&acc[2 * 4], 1ull);
-            atomicAdd(&acc[2 * 4 + 1], 4ull * PP);
+            atomicAdd(&acc[2 * SL_JPT], 1ull);
+            atomicAdd(&acc[2 * SL_JPT + 1], 4ull * PP);
         }
         return;
     }
@@ -228,7 +238,7 @@ __global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
     int b;
     if (kd == 0) {
         build_point_job(J, p[0], p[1], P, ph, pw);
-        b = 4;
+        b = PTB;
     } else {
         const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
         b = sz_bucket(sz);
@@ -461,7 +471,7 @@ __global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
         int b;
         if (kd == 0) {
             build_point_job(J, p[0], p[1], P, ph, pw);
-            b = 4;
+            b = PTB;
         } else {
             const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
             b = sz_bucket(sz);
@@ -477,8 +487,9 @@ __global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
         const int i = atomicAdd(&cnt[b], 1);
         lists[b * cap + i] = slot;
         if (acc) {
-            atomicAdd(&acc[2 * (7 + b)], 1ull);
-            atomicAdd(&acc[2 * (7 + b) + 1], 8ull * P * P);  // data + inverr read
+            const int sl = b == PTB ? SL_LPT : SL_LS + b;
+            atomicAdd(&acc[2 * sl], 1ull);
+            atomicAdd(&acc[2 * sl + 1], 8ull * P * P);  // data + inverr read
         }
     }
 }
@@ -610,7 +621,7 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         for (int i = 0; i < 4; ++i) f->cdi[i] = cdi[i];
     std::vector<int64_t> roff(S);
     int64_t tot = 0;
-    int maxP_sz[4] = {0, 0, 0, 0};
+    int maxP_sz[NFFT] = {0, 0, 0, 0, 0};
     for (int s = 0; s < S; ++s) {
         if (kind[s] < 0 || kind[s] > 2 || P[s] < 3 || P[s] > 4096) {
             trh_fitter_destroy(f);
@@ -621,14 +632,14 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         tot += (int64_t)(kind[s] ? 4 : 1) * P[s] * P[s];
         if (kind[s]) {
             ++f->n_gal;
-            for (int b = 0; b < 4; ++b)
+            for (int b = 0; b < NFFT; ++b)
                 if (P[s] < kSizes[b] && P[s] > maxP_sz[b]) maxP_sz[b] = P[s];
         } else {
             ++f->n_pt;
             if (P[s] > f->maxP_pt) f->maxP_pt = P[s];
         }
     }
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NFFT; ++b) {
         int mr = max_rows_for(kSizes[b], h->lds_max);
         if (maxP_sz[b] > 0 && maxP_sz[b] < mr) mr = maxP_sz[b];
         if (mr < 8) mr = 8;
@@ -655,7 +666,7 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         trh_fitter_destroy(f);
         return rc;
     }
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NFFT; ++b) {
         const int sz = kSizes[b];
         if ((rc = falloc(f, &f->spec[b], (size_t)2 * sz * (sz / 2 + 1)))) {
             trh_fitter_destroy(f);
@@ -678,7 +689,7 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         trh_fitter_destroy(f);
         return rc;
     }
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NFFT; ++b) {
         if ((rc = trh_psf_spectrum_batch(h, f->stamp, 1, ph, pw, kSizes[b], f->spec[b], &f->cx[b],
                                          &f->cy[b], 0))) {
             trh_fitter_destroy(f);
@@ -755,14 +766,14 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
     unsigned long long *acc = f->profiling ? f->acc : nullptr;
     int rc;
     (void)hipEventRecord(f->ev[0], st);
-    SLOT_BEGIN(12);
+    SLOT_BEGIN(SL_PREP);
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
     hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->render_off,
                        f->params, f->active, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
                        cap_jac, f->cnt, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
-    SLOT_END(12);
-    for (int b = 0; b < 4 && f->n_gal; ++b) {
+    SLOT_END(SL_PREP);
+    for (int b = 0; b < NFFT && f->n_gal; ++b) {
         SLOT_BEGIN(b);
         if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_jac, f->list_jac + (size_t)b * cap_jac,
                                  f->cnt + b, 4 * f->n_gal, f->maxrows[b], f->spec[b], f->renders,
@@ -770,13 +781,13 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
             return rc;
         SLOT_END(b);
     }
-    SLOT_BEGIN(4);
-    if ((rc = launch_point(f, f->jobs_jac, f->list_jac + (size_t)4 * cap_jac, f->cnt + 4, f->n_pt,
+    SLOT_BEGIN(SL_JPT);
+    if ((rc = launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt,
                            f->renders, nullptr, st)))
         return rc;
-    SLOT_END(4);
+    SLOT_END(SL_JPT);
     (void)hipEventRecord(f->ev[1], st);
-    SLOT_BEGIN(5);
+    SLOT_BEGIN(SL_NEQ);
     {
         int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
         hipLaunchKernelGGL(k_fit_normal_eq, dim3(grid), dim3(256), 0, st, S, f->kind, f->P,
@@ -784,31 +795,31 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
                            f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq);
         if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq"))) return rc;
     }
-    SLOT_END(5);
+    SLOT_END(SL_NEQ);
     (void)hipEventRecord(f->ev[2], st);
-    SLOT_BEGIN(6);
+    SLOT_BEGIN(SL_SOLVE);
     hipLaunchKernelGGL(k_fit_solve, dim3((S + 63) / 64), dim3(64), 0, st, S, f->kind, f->neq,
                        f->params, f->active, lambda, f->X, f->trial, f->nsteps, f->steplim);
     if ((rc = hip_check(hipGetLastError(), "k_fit_solve"))) return rc;
-    SLOT_END(6);
+    SLOT_END(SL_SOLVE);
     (void)hipEventRecord(f->ev[3], st);
     hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->pix_off,
                        f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw, f->jobs_ls,
-                       f->list_ls, cap_ls, f->cnt + 8, f->err, acc);
+                       f->list_ls, cap_ls, f->cnt + LS_CNT, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
-    for (int b = 0; b < 4 && f->n_gal; ++b) {
-        SLOT_BEGIN(7 + b);
+    for (int b = 0; b < NFFT && f->n_gal; ++b) {
+        SLOT_BEGIN(SL_LS + b);
         if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_ls, f->list_ls + (size_t)b * cap_ls,
-                                 f->cnt + 8 + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
+                                 f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
                                  nullptr, f->data, f->inverr, f->chi2ls, st)))
             return rc;
-        SLOT_END(7 + b);
+        SLOT_END(SL_LS + b);
     }
-    SLOT_BEGIN(11);
-    if ((rc = launch_point(f, f->jobs_ls, f->list_ls + (size_t)4 * cap_ls, f->cnt + 12,
+    SLOT_BEGIN(SL_LPT);
+    if ((rc = launch_point(f, f->jobs_ls, f->list_ls + (size_t)PTB * cap_ls, f->cnt + LS_CNT + PTB,
                            NALPHA * f->n_pt, nullptr, f->chi2ls, st)))
         return rc;
-    SLOT_END(11);
+    SLOT_END(SL_LPT);
     (void)hipEventRecord(f->ev[4], st);
     hipLaunchKernelGGL(k_fit_accept, dim3(nb), dim3(tb), 0, st, S, f->neq, f->trial, f->nsteps,
                        f->steplim, f->chi2ls, f->params, f->active, f->iters, f->chisq, f->dlnp,
@@ -822,16 +833,16 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
         if ((rc = hip_check(hipMemcpy(cnt, f->cnt, sizeof(cnt), hipMemcpyDeviceToHost), "read counts")))
             return rc;
         for (int k = 0; k < TRH_NSLOT; ++k) {
-            const bool fft = (k < 4) || (k >= 7 && k < 11);
-            const bool launched = fft ? f->n_gal > 0 : ((k == 4 || k == 11) ? f->n_pt > 0 : true);
+            const bool fft = (k < NFFT) || (k >= SL_LS && k < SL_LS + NFFT);
+            const bool launched = fft ? f->n_gal > 0 : ((k == SL_JPT || k == SL_LPT) ? f->n_pt > 0 : true);
             if (!launched) continue;
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, f->pev[k][0], f->pev[k][1]) != hipSuccess) continue;
             f->tot_ms[k] += ms;
             f->tot_launch[k] += 1;
             if (fft) {
-                const int b = k < 4 ? k : k - 7;
-                const int njobs = k < 4 ? cnt[b] : cnt[8 + b];
+                const int b = k < NFFT ? k : k - SL_LS;
+                const int njobs = k < NFFT ? cnt[b] : cnt[LS_CNT + b];
                 const int sz = kSizes[b];
                 if (njobs > 0) f->tot_extra_bytes[k] += 8.0 * sz * (sz / 2 + 1);  // PSF spectrum
             }
@@ -880,13 +891,13 @@ extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, void *stream)
                        f->pix_off, f->params, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
                        cap_jac, f->cnt, f->err);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_model"))) return rc;
-    for (int b = 0; b < 4 && f->n_gal; ++b) {
+    for (int b = 0; b < NFFT && f->n_gal; ++b) {
         if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_jac, f->list_jac + (size_t)b * cap_jac,
                                  f->cnt + b, f->n_gal, f->maxrows[b], f->spec[b], out, nullptr,
                                  nullptr, nullptr, st)))
             return rc;
     }
-    return launch_point(f, f->jobs_jac, f->list_jac + (size_t)4 * cap_jac, f->cnt + 4, f->n_pt, out,
+    return launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt, out,
                         nullptr, st);
 }
 
@@ -897,7 +908,7 @@ extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterat
     if ((rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "sync"))) return rc;
     int32_t err = 0;
     if ((rc = hip_check(hipMemcpy(&err, f->err, 4, hipMemcpyDeviceToHost), "read err"))) return rc;
-    if (err) return set_err(TRH_E_UNSUPPORTED, "a render needed FFT size %d (supported: 32..256)", err);
+    if (err) return set_err(TRH_E_UNSUPPORTED, "a render needed FFT size %d (supported: 32..512)", err);
     std::vector<int32_t> a(f->S);
     if ((rc = hip_check(hipMemcpy(a.data(), f->active, 4 * (size_t)f->S, hipMemcpyDeviceToHost), "read active")))
         return rc;

@@ -301,6 +301,9 @@ template int launch_fft<128>(trh_handle *, const trh_fft_job *, const int *, con
 template int launch_fft<256>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
                              const float *, float *, const float *, const float *, double *,
                              hipStream_t);
+template int launch_fft<512>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
+                             const float *, float *, const float *, const float *, double *,
+                             hipStream_t);
 
 int launch_fft_any(trh_handle *h, int sz, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
@@ -319,8 +322,11 @@ int launch_fft_any(trh_handle *h, int sz, const trh_fft_job *jobs, const int *li
     case 256:
         return launch_fft<256>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
                                inverr, chi2, st);
+    case 512:
+        return launch_fft<512>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
+                               inverr, chi2, st);
     }
-    return set_err(TRH_E_UNSUPPORTED, "FFT render size %d not in {32,64,128,256}", sz);
+    return set_err(TRH_E_UNSUPPORTED, "FFT render size %d not in {32,64,128,256,512}", sz);
 }
 }  // namespace trh
 
