@@ -55,17 +55,17 @@ def build_workload(name, torch):
                 inverr=inverr, pix_off=pix_off, start=start)
 
 
-def run_step(wl, torch):
+def run_step(wl, torch, dchisq=0.):
     f = wl['fitter']
     if wl['cfg'] == 4:
         f.set_params(wl['start'])
-        return f.fit(max_iter=100)
+        return f.fit(max_iter=100, dchisq=dchisq)
     f.set_params(wl['start'])
     f.iterate(0., 0.)
     return 1
 
 
-def cpu_baseline(wl, budget_s=15.0, min_sources=4):
+def cpu_baseline(wl, budget_s=15.0, min_sources=4, dchisq=0.):
     """The FP64 NumPy oracle (single core) fitting the same sources from the
     same start, for about `budget_s` seconds."""
     import oracle as O
@@ -83,10 +83,10 @@ def cpu_baseline(wl, budget_s=15.0, min_sources=4):
                                data=data[o:o + Pz * Pz].reshape(Pz, Pz),
                                inverr=inverr[o:o + Pz * Pz].reshape(Pz, Pz), cdi=B.PIXEL_CDI)
         if wl['cfg'] == 4:
-            p, nit, chi = O.fit_source(prob, wl['start'][s])
+            p, nit, chi = O.fit_source(prob, wl['start'][s], dchisq=dchisq)
         else:
             p, nit, chi = O.fit_source(prob, wl['start'][s], steps=1)
-        results.append((s, p, chi))
+        results.append((s, p, chi, nit))
         n += 1
     dt = time.time() - t0
     return n, dt, results
@@ -100,6 +100,8 @@ def main():
     ap.add_argument('--workload', default='c4', choices=['c4', 'c2'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--dchisq', type=float, default=0.,
+                    help='optimize_loop stopping tolerance on dlnp (reference default 0)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build runs the batched fit on a single '
@@ -112,7 +114,7 @@ def main():
     wl = build_workload(args.workload, torch)
     f = wl['fitter']
     for _ in range(args.warmup):
-        run_step(wl, torch)
+        run_step(wl, torch, args.dchisq)
     torch.cuda.synchronize()
 
     f.profile(True)
@@ -120,7 +122,7 @@ def main():
     iters = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        iters.append(run_step(wl, torch))
+        iters.append(run_step(wl, torch, args.dchisq))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = f.profile_read()
@@ -129,6 +131,9 @@ def main():
     value = wl['n'] / dt
     st = f.status()
     log('per-step iterations', iters, 'max per-source iterations', int(st['iterations'].max()))
+    hist = np.bincount(np.minimum(st['iterations'], 100), minlength=101)
+    log('per-source iteration histogram (iterations: sources):',
+        {int(i): int(c) for i, c in enumerate(hist) if c})
 
     # dominant kernel (largest total time over the timed region)
     k = int(np.argmax(prof[:, 0]))
@@ -143,7 +148,7 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
-        n, cdt, res = cpu_baseline(wl, args.cpu_seconds)
+        n, cdt, res = cpu_baseline(wl, args.cpu_seconds, dchisq=args.dchisq)
         cpu = dict(value=round(n / cdt, 4), unit='sources/s' if wl['cfg'] == 4 else 'source-iterations/s',
                    cores=1, kind='port',
                    sample='first %d sources of the same workload (same data and start), '
@@ -151,9 +156,17 @@ def main():
         if wl['cfg'] == 4:
             gp = f.get_params()
             errs = []
-            for (s, p, chi) in res:
-                errs.append(abs(st['chisq'][s] - chi) / chi)
-            parity = dict(sources=len(res), max_rel_chisq_vs_fp64=float(max(errs)))
+            for (s, p, chi, nit) in res:
+                e = abs(st['chisq'][s] - chi) / chi
+                errs.append(e)
+                if e > 1e-4:
+                    log('chi2 parity miss: source %d kind %d P %d flux %.1f gpu chi2 %.4f (iters %d) '
+                        'oracle %.4f (iters %d) gpu %s oracle %s' % (
+                            s, wl['kind'][s], wl['P'][s], wl['truth'][s, 2], st['chisq'][s],
+                            st['iterations'][s], chi, nit, np.round(gp[s], 4), np.round(p, 4)))
+            errs = np.array(errs)
+            parity = {'sources': len(res), 'max_rel_chisq_vs_fp64': float(errs.max()),
+                      'frac_within_1e-4': float(np.mean(errs <= 1e-4))}
 
     out = {
         'metric': METRIC,
@@ -171,6 +184,7 @@ def main():
                          'config2: one LM iteration over 4096 sources (30/40/30), P 16..128, '
                          '63x63 Moffat PSF, sigma=1'),
             'sources': wl['n'], 'lm_iterations_per_step': float(np.mean(iters)),
+            'dchisq': args.dchisq,
             'parallelism': 'single GPU',
         },
         'roofline': {

@@ -122,6 +122,14 @@ int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njo
                          const float *inverr, double *chi2, void *stream);
 
 /*
+ * Launch shape of the FFT render kernel for FFT size sz (32..512): threads
+ * per block (multiple of 64, <= 1024) and columns per phase-A chunk; 0 keeps
+ * the default.  The chunk is reduced automatically to fit LDS.  Tuning knob
+ * of the render schedule (no reference counterpart).
+ */
+int trh_set_fft_config(trh_handle *h, int sz, int threads, int ch);
+
+/*
  * Batched pixel-space MoG evaluation (replaces tractor.mix.c_gauss_2d_grid,
  * tractor/mix.i:452-524 and c_gauss_2d_masked, tractor/gauss_masked.c:1-159).
  * Component arrays (float32 except the integer centres): amp[c], cxi[c],

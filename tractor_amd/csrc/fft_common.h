@@ -53,6 +53,60 @@ __device__ __forceinline__ void fft_dit(float2 *buf, int nseq, const float2 *tw)
     }
 }
 
+// Same transform as fft_dit, with pairs of radix-2 stages fused into one
+// radix-4 pass (4 elements per thread in registers): ceil(log2 N / 2)
+// barriers and half the LDS traffic.
+template <int N, bool INVERSE>
+__device__ __forceinline__ void fft_dit_r4(float2 *buf, int nseq, const float2 *tw) {
+    constexpr int L = Log2<N>::v;
+    int lg = 0;
+    if (L & 1) {
+        // leading radix-2 stage (span 1, twiddle 1)
+        const int nb = nseq * (N / 2);
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int i0 = 2 * b;
+            const float2 x0 = buf[i0], x1 = buf[i0 + 1];
+            buf[i0] = cadd(x0, x1);
+            buf[i0 + 1] = csub(x0, x1);
+        }
+        __syncthreads();
+        lg = 1;
+    }
+#pragma unroll 1
+    for (; lg < L; lg += 2) {
+        const int h = 1 << lg;
+        const int nb = nseq * (N / 4);
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int s = b >> (L - 2);
+            const int r = b & (N / 4 - 1);
+            const int k = r & (h - 1);
+            const int grp = r >> lg;
+            const int base = s * N + grp * 4 * h + k;
+            const float2 a = buf[base], bq = buf[base + h], c = buf[base + 2 * h],
+                         d = buf[base + 3 * h];
+            float2 w1 = tw[k << (L - 1 - lg)];
+            float2 w2 = tw[k << (L - 2 - lg)];
+            float2 w3 = tw[(k + h) << (L - 2 - lg)];
+            if (!INVERSE) {
+                w1.y = -w1.y;
+                w2.y = -w2.y;
+                w3.y = -w3.y;
+            }
+            float2 t = cmul(w1, bq);
+            const float2 a1 = cadd(a, t), b1 = csub(a, t);
+            t = cmul(w1, d);
+            const float2 c1 = cadd(c, t), d1 = csub(c, t);
+            t = cmul(w2, c1);
+            buf[base] = cadd(a1, t);
+            buf[base + 2 * h] = csub(a1, t);
+            t = cmul(w3, d1);
+            buf[base + h] = cadd(b1, t);
+            buf[base + 3 * h] = csub(b1, t);
+        }
+        __syncthreads();
+    }
+}
+
 // lanczos_filter_3 (tractor/mp_fourier.i:355-402): 7 taps L3(i-3+dx), the one
 // beyond +-3 set to zero, normalised to unit sum; computed in FP64, stored FP32.
 __device__ __forceinline__ void lanczos3_taps(double dx, float *f) {

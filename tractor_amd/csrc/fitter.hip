@@ -775,8 +775,8 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
     SLOT_END(SL_PREP);
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         SLOT_BEGIN(b);
-        if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_jac, f->list_jac + (size_t)b * cap_jac,
-                                 f->cnt + b, 4 * f->n_gal, f->maxrows[b], f->spec[b], f->renders,
+        if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
+                                 f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal, f->maxrows[b], f->spec[b], f->renders,
                                  nullptr, nullptr, nullptr, st)))
             return rc;
         SLOT_END(b);
@@ -809,8 +809,8 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         SLOT_BEGIN(SL_LS + b);
-        if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_ls, f->list_ls + (size_t)b * cap_ls,
-                                 f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
+        if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
+                                 f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
                                  nullptr, f->data, f->inverr, f->chi2ls, st)))
             return rc;
         SLOT_END(SL_LS + b);
@@ -892,8 +892,8 @@ extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, void *stream)
                        cap_jac, f->cnt, f->err);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_model"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
-        if ((rc = launch_fft_any(f->h, kSizes[b], f->jobs_jac, f->list_jac + (size_t)b * cap_jac,
-                                 f->cnt + b, f->n_gal, f->maxrows[b], f->spec[b], out, nullptr,
+        if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
+                                 f->list_jac + (size_t)b * cap_jac, f->cnt + b, f->n_gal, f->maxrows[b], f->spec[b], out, nullptr,
                                  nullptr, nullptr, st)))
             return rc;
     }

@@ -12,6 +12,8 @@ struct trh_handle {
     float2 *tw = nullptr;      // twiddle tables for N = 32..512
     void *scratch = nullptr;   // grown on demand by entry points that need it
     size_t scratch_bytes = 0;
+    int fft_threads[5] = {0, 0, 0, 0, 0};  // per FFT size 32..512; 0 = default
+    int fft_ch[5] = {0, 0, 0, 0, 0};
 };
 
 namespace trh {
@@ -21,9 +23,9 @@ int tw_offset(int n);
 int pad_center(int sz, int p);
 int ensure_scratch(trh_handle *h, size_t bytes);
 int max_rows_for(int sz, int lds_max);
-void fft_launch_shape(int sz, int maxrows, int n_cu, int lds_max, int *ch, size_t *lds, int *grid,
-                      int njobs);
-int launch_fft_any(trh_handle *h, int sz, const trh_fft_job *jobs, const int *list,
+void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *ch, size_t *lds,
+                      int *grid, int njobs);
+int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
                    float *out, const float *data, const float *inverr, double *chi2,
                    hipStream_t st);

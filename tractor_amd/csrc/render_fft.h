@@ -71,7 +71,7 @@ __device__ __forceinline__ double epilogue(int mode, float val, float scale, int
 // Render one FFT job (see trh_fft_job).  Every thread of the block calls this.
 // tw (LDS) must hold exp(+2 pi i m/SZ), m < SZ/2.  work: ch*SZ float2,
 // hbuf: maxrows*(SZ/2+1) float2.
-template <int SZ>
+template <int SZ, int MODE>
 __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                                const float2 *__restrict__ psf_spec, const float2 *tw,
                                float2 *work, float2 *hbuf, int ch, int maxrows,
@@ -85,7 +85,8 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     const int64_t off = Jp->out_offset;
     const int stride = Jp->out_stride;
     const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy;
-    const int K = min(max(Jp->K, 0), TRH_MAXK), mode = Jp->mode;
+    // MODE >= 0: epilogue fixed at compile time; -1: read from the job
+    const int K = min(max(Jp->K, 0), TRH_MAXK), mode = MODE >= 0 ? MODE : Jp->mode;
     const float scale = Jp->scale;
     const float2 *__restrict__ P = psf_spec + (size_t)Jp->psf * SZ * NV;
 
@@ -137,7 +138,7 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 work[cc * SZ + bitrev<LOG2>(j)] = make_float2(F * p.x, F * p.y);
             }
             __syncthreads();
-            fft_dit<SZ, true>(work, nc, tw);
+            fft_dit_r4<SZ, true>(work, nc, tw);
             for (int idx = tid; idx < nrb * nc; idx += nthr) {
                 const int q = idx / nc;
                 const int cc = idx - q * nc;
@@ -183,7 +184,7 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 work[pp * SZ + bitrev<LOG2>(k)] = Z;
             }
             __syncthreads();
-            fft_dit<SZ, true>(work, npp, tw);
+            fft_dit_r4<SZ, true>(work, npp, tw);
             const int rowlen = 2 * mw;
             for (int idx = tid; idx < npp * rowlen; idx += nthr) {
                 const int pp = idx / rowlen;
@@ -273,8 +274,8 @@ static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 // Grid-stride render kernel over a job list.  `count` (device, may be NULL:
 // then njobs) bounds the list; `list` (may be NULL: identity) maps list
 // entries to job indices.
-template <int SZ>
-__global__ void __launch_bounds__(256)
+template <int SZ, int MODE>
+__global__ void __launch_bounds__(1024)
 k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
              const int *__restrict__ count, int njobs, const float2 *__restrict__ psf_spec,
              const float2 *__restrict__ twg, int ch, int maxrows, float *__restrict__ out,
@@ -290,8 +291,8 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
     const int n = count ? *count : njobs;
     for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
         const int jid = list ? list[jj] : jj;
-        render_fft_one<SZ>(jobs + jid, psf_spec, tw, work, hbuf, ch, maxrows, S, out, data,
-                           inverr, chi2);
+        render_fft_one<SZ, MODE>(jobs + jid, psf_spec, tw, work, hbuf, ch, maxrows, S, out, data,
+                                 inverr, chi2);
     }
 }
 

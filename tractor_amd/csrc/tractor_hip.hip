@@ -242,93 +242,117 @@ extern "C" int trh_psf_spectrum_batch(trh_handle *h, const float *stamps, int n,
 // Batched FFT renders
 // ---------------------------------------------------------------------------
 namespace trh {
-void fft_launch_shape(int sz, int maxrows, int n_cu, int lds_max, int *ch, size_t *lds, int *grid,
-                      int njobs) {
-    int c;
+static int sz_index(int sz) {
     switch (sz) {
-    case 32: c = 17; break;
-    case 64: c = 33; break;
-    case 128: c = 16; break;
-    default: c = 8; break;
+    case 32: return 0;
+    case 64: return 1;
+    case 128: return 2;
+    case 256: return 3;
+    case 512: return 4;
     }
+    return -1;
+}
+
+// Default launch shape per FFT size: threads per block and columns per
+// phase-A chunk (all columns where LDS allows).  trh_set_fft_config
+// overrides them (the autotuner's knob).
+// from a sweep over (threads, ch) on MI355X with the config-2 workload
+static const int kDefThreads[5] = {256, 256, 768, 1024, 1024};
+static const int kDefCh[5] = {17, 17, 16, 8, 8};
+
+void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *ch, size_t *lds,
+                      int *grid, int njobs) {
+    const int si = sz_index(sz);
+    int c = h->fft_ch[si] > 0 ? h->fft_ch[si] : kDefCh[si];
+    int nt = h->fft_threads[si] > 0 ? h->fft_threads[si] : kDefThreads[si];
+    if (c > sz / 2 + 1) c = sz / 2 + 1;
     size_t b = fft_lds_bytes(sz, c, maxrows);
-    while (b + sizeof(RenderScratch) > (size_t)lds_max && c > 1) {
+    while (b + sizeof(RenderScratch) > (size_t)h->lds_max && c > 1) {
         c /= 2;
         b = fft_lds_bytes(sz, c, maxrows);
     }
     *ch = c;
     *lds = b;
+    *threads = nt;
     const size_t per_cu = 160 * 1024;
     int wg_per_cu = (int)(per_cu / (b + sizeof(RenderScratch)));
+    const int by_threads = 2048 / nt;    // 32 waves per CU
+    if (wg_per_cu > by_threads) wg_per_cu = by_threads;
     if (wg_per_cu < 1) wg_per_cu = 1;
-    if (wg_per_cu > 8) wg_per_cu = 8;
-    int g = n_cu * wg_per_cu;
+    int g = h->n_cu * wg_per_cu;
     if (njobs > 0 && njobs < g) g = njobs;
     *grid = g < 1 ? 1 : g;
 }
 
 int max_rows_for(int sz, int lds_max) {
     // rows of the y-Lanczos output kept in LDS per pass
-    const int NV = sz / 2 + 1;
     int mr = sz;
     while (mr > 8 && fft_lds_bytes(sz, 8, mr) + sizeof(RenderScratch) > (size_t)lds_max) mr /= 2;
-    (void)NV;
     return mr;
 }
 
 template <int SZ>
-int launch_fft(trh_handle *h, const trh_fft_job *jobs, const int *list, const int *count,
-               int njobs_bound, int maxrows, const float *psf_spec, float *out, const float *data,
-               const float *inverr, double *chi2, hipStream_t st) {
-    int ch, grid;
+int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list,
+               const int *count, int njobs_bound, int maxrows, const float *psf_spec, float *out,
+               const float *data, const float *inverr, double *chi2, hipStream_t st) {
+    int nt, ch, grid;
     size_t lds;
-    fft_launch_shape(SZ, maxrows, h->n_cu, h->lds_max, &ch, &lds, &grid, njobs_bound);
-    hipLaunchKernelGGL(k_render_fft<SZ>, dim3(grid), dim3(256), lds, st, jobs, list, count,
-                       njobs_bound, (const float2 *)psf_spec, h->tw + tw_offset(SZ), ch, maxrows,
-                       out, data, inverr, chi2);
+    fft_launch_shape(h, SZ, maxrows, &nt, &ch, &lds, &grid, njobs_bound);
+    const float2 *tw = h->tw + tw_offset(SZ);
+    const float2 *ps = (const float2 *)psf_spec;
+    switch (mode) {
+    case TRH_STORE:
+        hipLaunchKernelGGL((k_render_fft<SZ, TRH_STORE>), dim3(grid), dim3(nt), lds, st, jobs, list,
+                           count, njobs_bound, ps, tw, ch, maxrows, out, data, inverr, chi2);
+        break;
+    case TRH_CHI2:
+        hipLaunchKernelGGL((k_render_fft<SZ, TRH_CHI2>), dim3(grid), dim3(nt), lds, st, jobs, list,
+                           count, njobs_bound, ps, tw, ch, maxrows, out, data, inverr, chi2);
+        break;
+    default:
+        hipLaunchKernelGGL((k_render_fft<SZ, -1>), dim3(grid), dim3(nt), lds, st, jobs, list, count,
+                           njobs_bound, ps, tw, ch, maxrows, out, data, inverr, chi2);
+        break;
+    }
     return hip_check(hipGetLastError(), "k_render_fft");
 }
 
-template int launch_fft<32>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
-                            const float *, float *, const float *, const float *, double *,
-                            hipStream_t);
-template int launch_fft<64>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
-                            const float *, float *, const float *, const float *, double *,
-                            hipStream_t);
-template int launch_fft<128>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
-                             const float *, float *, const float *, const float *, double *,
-                             hipStream_t);
-template int launch_fft<256>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
-                             const float *, float *, const float *, const float *, double *,
-                             hipStream_t);
-template int launch_fft<512>(trh_handle *, const trh_fft_job *, const int *, const int *, int, int,
-                             const float *, float *, const float *, const float *, double *,
-                             hipStream_t);
-
-int launch_fft_any(trh_handle *h, int sz, const trh_fft_job *jobs, const int *list,
+int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
                    float *out, const float *data, const float *inverr, double *chi2,
                    hipStream_t st) {
     switch (sz) {
     case 32:
-        return launch_fft<32>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
-                              inverr, chi2, st);
+        return launch_fft<32>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
+                              data, inverr, chi2, st);
     case 64:
-        return launch_fft<64>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
-                              inverr, chi2, st);
+        return launch_fft<64>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
+                              data, inverr, chi2, st);
     case 128:
-        return launch_fft<128>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
-                               inverr, chi2, st);
+        return launch_fft<128>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
+                               data, inverr, chi2, st);
     case 256:
-        return launch_fft<256>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
-                               inverr, chi2, st);
+        return launch_fft<256>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
+                               data, inverr, chi2, st);
     case 512:
-        return launch_fft<512>(h, jobs, list, count, njobs_bound, maxrows, psf_spec, out, data,
-                               inverr, chi2, st);
+        return launch_fft<512>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
+                               data, inverr, chi2, st);
     }
     return set_err(TRH_E_UNSUPPORTED, "FFT render size %d not in {32,64,128,256,512}", sz);
 }
 }  // namespace trh
+
+extern "C" int trh_set_fft_config(trh_handle *h, int sz, int threads, int ch) {
+    if (!h) return set_err(TRH_E_ARG, "trh_set_fft_config: null handle");
+    const int si = sz_index(sz);
+    if (si < 0) return set_err(TRH_E_ARG, "trh_set_fft_config: sz %d not in {32..512}", sz);
+    if (threads != 0 && (threads % 64 != 0 || threads < 64 || threads > 1024))
+        return set_err(TRH_E_ARG, "trh_set_fft_config: threads must be a multiple of 64 in [64, 1024]");
+    if (ch < 0) return set_err(TRH_E_ARG, "trh_set_fft_config: ch must be >= 0");
+    h->fft_threads[si] = threads;
+    h->fft_ch[si] = ch;
+    return TRH_OK;
+}
 
 extern "C" int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njobs,
                                     const float *psf_spec, float *out, const float *data,
@@ -337,7 +361,7 @@ extern "C" int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jo
         return set_err(TRH_E_ARG, "trh_render_fft_batch: null argument");
     if (njobs <= 0) return TRH_OK;
     const int maxrows = max_rows_for(sz, h->lds_max);
-    return launch_fft_any(h, sz, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out, data,
+    return launch_fft_any(h, sz, -1, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out, data,
                           inverr, chi2, (hipStream_t)stream);
 }
 
