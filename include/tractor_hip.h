@@ -164,6 +164,27 @@ int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, in
                            const float *stamp, int ph, int pw, float *out, const float *data,
                            const float *inverr, double *chi2, void *stream);
 
+/*
+ * Normal equations of a dense least-squares problem (the update direction of
+ * LsqrOptimizer.getUpdateDirection, tractor/lsqr_optimizer.py:328-644, for
+ * the object API).  Ab: column-major float32 [ncols+1][nrows], column ncols
+ * is b.  out: double [(ncols+1)^2] = Ab^T Ab (FP32 products, FP64 sums).
+ * ncols <= 512.
+ */
+int trh_normal_eq_dense(trh_handle *h, const float *Ab, int nrows, int ncols, double *out,
+                        void *stream);
+
+/*
+ * FP64 Cholesky solve of the column-scaled, damped normal equations from
+ * trh_normal_eq_dense: (D^-1 N D^-1 + damp^2 I) y = D^-1 g, x = D^-1 y
+ * (D = sqrt(diag N) if scale_columns, else I; zero columns give x = 0).  This
+ * is the least-squares solution scipy's LSQR converges to
+ * (tractor/lsqr_optimizer.py:584-591, damp as LSQR's).  *info (host, may be
+ * NULL; synchronises) = 0 or the failing pivot + 1.  n <= 512.
+ */
+int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, int scale_columns,
+                         double *x, int *info, void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched single-source fitting (config workloads: thousands of sources, one
  * modelMask patch each).  Replaces GpuOptimizer's per-source

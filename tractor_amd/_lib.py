@@ -48,6 +48,8 @@ SIGNATURES = {
     'trh_psf_spectrum_batch': (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P]),
     'trh_render_fft_batch': (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
     'trh_set_fft_config': (_I, [_P, _I, _I, _I]),
+    'trh_normal_eq_dense': (_I, [_P, _P, _I, _I, _P, _P]),
+    'trh_solve_normal_f64': (_I, [_P, _P, _I, _D, _I, _P, _P, _P]),
     'trh_render_mog_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     'trh_render_point_batch': (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P]),

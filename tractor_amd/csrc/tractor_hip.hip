@@ -541,3 +541,128 @@ extern "C" int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, in
                        nullptr, nullptr, njobs, stamp, ph, pw, maxw, out, data, inverr, chi2);
     return hip_check(hipGetLastError(), "k_render_point");
 }
+
+// ---------------------------------------------------------------------------
+// Dense normal equations and FP64 solve (the least-squares step of
+// LsqrOptimizer.getUpdateDirection, tractor/lsqr_optimizer.py:328-644, for
+// the object API path: A restricted to the rows its derivatives touch).
+// ---------------------------------------------------------------------------
+// Ab: column-major [ncols+1][nrows] float32 (the last column is b).
+// out[(ncols+1)^2] += sum over rows of Ab[:, i] * Ab[:, j] (FP64 atomics).
+__global__ void __launch_bounds__(256)
+k_normal_eq_dense(const float *__restrict__ Ab, int nrows, int nc1, double *__restrict__ out) {
+    __shared__ double red[4];
+    const int p = blockIdx.y;          // pair index over the upper triangle
+    int i = 0, rem = p;
+    while (rem >= nc1 - i) {
+        rem -= nc1 - i;
+        ++i;
+    }
+    const int j = i + rem;
+    const float *ci = Ab + (size_t)i * nrows;
+    const float *cj = Ab + (size_t)j * nrows;
+    double acc = 0.0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += gridDim.x * blockDim.x)
+        acc += (double)(ci[r] * cj[r]);
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        atomicAdd(out + i * nc1 + j, t);
+        if (i != j) atomicAdd(out + j * nc1 + i, t);
+    }
+}
+
+__global__ void k_zero_f64(double *a, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+extern "C" int trh_normal_eq_dense(trh_handle *h, const float *Ab, int nrows, int ncols, double *out,
+                                   void *stream) {
+    if (!h || !Ab || !out || nrows <= 0 || ncols <= 0 || ncols > 512)
+        return set_err(TRH_E_ARG, "trh_normal_eq_dense: bad arguments");
+    const int nc1 = ncols + 1;
+    const int npairs = nc1 * (nc1 + 1) / 2;
+    if (npairs > 65535) return set_err(TRH_E_ARG, "trh_normal_eq_dense: too many columns");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_f64, dim3(64), dim3(256), 0, st, out, nc1 * nc1);
+    int gx = (nrows + 255) / 256;
+    if (gx > 64) gx = 64;
+    hipLaunchKernelGGL(k_normal_eq_dense, dim3(gx, npairs), dim3(256), 0, st, Ab, nrows, nc1, out);
+    return hip_check(hipGetLastError(), "k_normal_eq_dense");
+}
+
+// Solve (D^-1 N D^-1 + damp^2 I) y = D^-1 g, x = D^-1 y, with N = out[:n,:n],
+// g = out[:n, n] from trh_normal_eq_dense; D = sqrt(diag N) when
+// scale_columns, else I.  Columns with N_ii == 0 get x_i = 0.  One block, FP64
+// Cholesky in LDS-free global scratch.  info: 0 ok, k>0 not positive definite
+// at column k (x = 0).
+__global__ void k_solve_normal_f64(const double *__restrict__ Nb, int n, double damp, int scale,
+                                   double *__restrict__ L, double *__restrict__ x,
+                                   int *__restrict__ info) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n1 = n + 1;
+    for (int i = 0; i < n; ++i) {
+        const double Dii = sqrt(Nb[i * n1 + i]);
+        x[i] = 0.0;
+        L[n * n + i] = (scale ? Dii : (Nb[i * n1 + i] > 0.0 ? 1.0 : 0.0));  // D
+    }
+    const double *D = L + n * n;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double v;
+            if (D[i] > 0.0 && D[j] > 0.0)
+                v = Nb[i * n1 + j] / (D[i] * D[j]);
+            else
+                v = (i == j) ? 1.0 : 0.0;
+            if (i == j && D[i] > 0.0) v += damp * damp;
+            L[i * n + j] = v;
+        }
+    for (int j = 0; j < n; ++j) {
+        double d = L[j * n + j];
+        for (int k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
+        if (!(d > 0.0)) {
+            *info = j + 1;
+            return;
+        }
+        d = sqrt(d);
+        L[j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double v = L[i * n + j];
+            for (int k = 0; k < j; ++k) v -= L[i * n + k] * L[j * n + k];
+            L[i * n + j] = v / d;
+        }
+    }
+    // forward: L y = D^-1 g  (y stored in x)
+    for (int i = 0; i < n; ++i) {
+        double v = D[i] > 0.0 ? Nb[i * n1 + n] / D[i] : 0.0;
+        for (int k = 0; k < i; ++k) v -= L[i * n + k] * x[k];
+        x[i] = v / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double v = x[i];
+        for (int k = i + 1; k < n; ++k) v -= L[k * n + i] * x[k];
+        x[i] = v / L[i * n + i];
+    }
+    for (int i = 0; i < n; ++i) x[i] = D[i] > 0.0 ? x[i] / D[i] : 0.0;
+    *info = 0;
+}
+
+extern "C" int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp,
+                                    int scale_columns, double *x, int *info_host, void *stream) {
+    if (!h || !Nb || !x || n <= 0 || n > 512)
+        return set_err(TRH_E_ARG, "trh_solve_normal_f64: bad arguments (1 <= n <= 512)");
+    int rc = ensure_scratch(h, sizeof(double) * ((size_t)n * n + n) + 64);
+    if (rc) return rc;
+    double *L = (double *)h->scratch;
+    int *info = (int *)(L + (size_t)n * n + n);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_solve_normal_f64, dim3(1), dim3(64), 0, st, Nb, n, damp, scale_columns, L,
+                       x, info);
+    if ((rc = hip_check(hipGetLastError(), "k_solve_normal_f64"))) return rc;
+    if (info_host) {
+        if ((rc = hip_check(hipMemcpyAsync(info_host, info, sizeof(int), hipMemcpyDeviceToHost, st),
+                            "copy info")))
+            return rc;
+        return hip_check(hipStreamSynchronize(st), "solve sync");
+    }
+    return TRH_OK;
+}

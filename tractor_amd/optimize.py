@@ -1,0 +1,204 @@
+"""Optimizer plugin base: line search (tryUpdates) and forced photometry
+(tractor/optimize.py:7-520)."""
+import numpy as np
+
+from .engine import logverb, logmsg, OptResult
+
+
+ALPHAS = np.append(2. ** np.arange(-10, 1), [np.sqrt(2.), 2.])
+
+
+class Optimizer(object):
+    """optimize(tractor, alphas, damp, priors, scale_columns, shared_params,
+    variance, just_variance) -> (dlnp, X, alpha); optimize_loop -> dict;
+    forced_photometry -> OptResult."""
+
+    def optimize(self, tractor, alphas=None, damp=0, priors=True, scale_columns=True,
+                 shared_params=True, variance=False, just_variance=False):
+        raise NotImplementedError
+
+    def optimize_loop(self, tractor, **kwargs):
+        raise NotImplementedError
+
+    def tryUpdates(self, tractor, X, alphas=None):
+        """Sequential line search (tractor/optimize.py:425-470)."""
+        if alphas is None:
+            alphas = ALPHAS
+        pBefore = tractor.getLogProb()
+        pBest = pBefore
+        alphaBest = None
+        p0 = tractor.getParams()
+        for alpha in alphas:
+            pa = [p + alpha * d for p, d in zip(p0, X)]
+            tractor.setParams(pa)
+            pAfter = tractor.getLogProb()
+            if not np.isfinite(pAfter):
+                logmsg('  Got bad log-prob', pAfter)
+                break
+            if pAfter < (pBest - 1.):
+                break
+            if pAfter > pBest:
+                alphaBest = alpha
+                pBest = pAfter
+        if alphaBest is None:
+            tractor.setParams(p0)
+            return 0, 0.
+        tractor.setParams([p + alphaBest * d for p, d in zip(p0, X)])
+        return pBest - pBefore, alphaBest
+
+    # ---- forced photometry (tractor/optimize.py:16-222, lsqr_optimizer.py:11-251)
+    def forced_photometry(self, tractor, alphas=None, damp=0, priors=False, minsb=0., mindlnp=1.,
+                          rois=None, sky=False, minFlux=None, fitstats=False, fitstat_extras=None,
+                          justims0=False, variance=False, skyvariance=False, shared_params=True,
+                          nonneg=False, nilcounts=-1e30, wantims=True, negfluxval=None, **kwargs):
+        from .calib import LinearPhotoCal
+        if rois is not None or sky:
+            raise NotImplementedError('forced_photometry with rois or sky fitting')
+        result = OptResult()
+        imgs = tractor.getImages()
+        scales = []
+        for img in imgs:
+            assert isinstance(img.getPhotoCal(), LinearPhotoCal)
+            scales.append(img.getPhotoCal().getScale())
+        Nsourceparams = tractor.catalog.numberOfParams()
+        srcs = list(tractor.catalog.getThawedSources())
+        umodels = self._get_umodels(tractor, srcs, imgs, minsb, **kwargs)
+        fsrcs = list(tractor.catalog.getFrozenSources())
+        mod0 = [tractor.getModelImage(img, fsrcs, minsb=minsb, sky=True) for img in imgs]
+        derivs = [[] for i in range(Nsourceparams)]
+        for tim, umods, scale in zip(imgs, umodels, scales):
+            for um, dd in zip(umods, derivs):
+                if um is None:
+                    continue
+                dd.append((um * scale, tim))
+        assert len(derivs) == tractor.numberOfParams()
+        lnp0 = None
+        chis0 = None
+        damp0 = 1e-3
+        damping = damp
+        ims0 = imsBest = None
+        while True:
+            tryAgain = False
+            quitNow = False
+            p0 = tractor.getParams()
+            if lnp0 is None:
+                lnp0, chis0, ims0 = self._lnp_for_update(tractor, mod0, imgs, umodels, None, None,
+                                                         p0, scales, priors, minFlux)
+            if justims0:
+                result.lnp0, result.chis0, result.ims0 = lnp0, chis0, ims0
+                return result
+            X = self.getUpdateDirection(tractor, derivs, damp=damping, priors=priors,
+                                        scale_columns=False, chiImages=chis0,
+                                        shared_params=shared_params)
+            if X is None or len(X) == 0:
+                break
+            als = ALPHAS if alphas is None else alphas
+            if (minFlux is None) or np.all((np.array(p0) + X) >= minFlux):
+                als = [1.]
+                quitNow = True
+            elif damp == 0.0:
+                damping = damp0
+                damp0 *= 10.
+                if damp0 < 1e3:
+                    tryAgain = True
+            lnpBest = lnp0
+            alphaBest = None
+            chiBest = None
+            for alpha in als:
+                lnp, chis, ims = self._lnp_for_update(tractor, mod0, imgs, umodels, X, alpha, p0,
+                                                      scales, priors, minFlux)
+                if lnp < (lnpBest - 1.):
+                    break
+                if not np.isfinite(lnp):
+                    break
+                if lnp > lnpBest:
+                    alphaBest, lnpBest, chiBest, imsBest = alpha, lnp, chis, ims
+            if alphaBest is not None:
+                if minFlux is not None:
+                    pa = [max(minFlux, p + alphaBest * d) for p, d in zip(p0, X)]
+                else:
+                    pa = [p + alphaBest * d for p, d in zip(p0, X)]
+                tractor.catalog.setParams(pa)
+                dlogprob = lnpBest - lnp0
+                lnp0, chis0 = lnpBest, chiBest
+            else:
+                dlogprob = 0.
+            if dlogprob < mindlnp and not tryAgain:
+                break
+            if quitNow:
+                break
+        result.ims0 = ims0
+        result.ims1 = imsBest
+        if variance:
+            result.IV = self._get_iv(imgs, umodels, scales, Nsourceparams)
+        if fitstats:
+            result.fitstats = None
+        return result
+
+    def _get_umodels(self, tractor, srcs, imgs, minsb, **kwargs):
+        """Unit-flux patches per (image, source brightness) (optimize.py:147-222)."""
+        umodels = []
+        for img in imgs:
+            pcal = img.getPhotoCal()
+            umods = []
+            for src in srcs:
+                counts = sum([pcal.brightnessToCounts(b) for b in src.getBrightnesses()])
+                mv = 1e-3 if counts <= 0 else minsb / counts
+                mask = tractor._getModelMaskFor(img, src)
+                ums = src.getUnitFluxModelPatches(img, minval=mv, modelMask=mask, **kwargs)
+                for um in ums:
+                    if um is not None and um.patch is not None:
+                        assert np.all(np.isfinite(um.patch))
+                umods.extend(ums)
+            umodels.append(umods)
+        return umodels
+
+    def _getims(self, fluxes, imgs, umodels, mod0, scales, minFlux):
+        """Model = mod0 + sum flux*scale*umodel; chi (optimize.py:472-520)."""
+        ims = []
+        for img, umods, m0, scale in zip(imgs, umodels, mod0, scales):
+            mod = m0.copy()
+            for f, um in zip(fluxes, umods):
+                if um is None or um.patch is None:
+                    continue
+                if minFlux is not None:
+                    f = max(f, minFlux)
+                counts = f * scale
+                if counts == 0.:
+                    continue
+                (um * counts).addTo(mod)
+            ie = img.getInvError()
+            im = img.getImage()
+            chi = (im - mod) * ie
+            ims.append((im, mod, ie, chi, None))
+        return ims
+
+    def _lnp_for_update(self, tractor, mod0, imgs, umodels, X, alpha, p0, scales, priors,
+                        minFlux):
+        pa = p0 if X is None else [p + alpha * d for p, d in zip(p0, X)]
+        lnp = 0.
+        if priors:
+            lnp += tractor.getLogPrior()
+            if not np.isfinite(lnp):
+                return lnp, None, None
+        ims = self._getims(pa, imgs, umodels, mod0, scales, minFlux)
+        chisq = 0.
+        chis = []
+        for _, _, _, chi, _ in ims:
+            chis.append(chi)
+            chisq += (chi.astype(np.float64) ** 2).sum()
+        return lnp - 0.5 * chisq, chis, ims
+
+    def _get_iv(self, imgs, umodels, scales, Nsourceparams):
+        """Inverse variances sum((umodel*scale*inverr)^2) (optimize.py:378-423)."""
+        IV = np.zeros(Nsourceparams)
+        for img, umods, scale in zip(imgs, umodels, scales):
+            ie = img.getInvError()
+            H, W = img.shape
+            for ui, um in enumerate(umods):
+                if um is None or um.patch is None:
+                    continue
+                mm = np.zeros((H, W))
+                um.addTo(mm)
+                IV[ui] += np.sum((mm * scale * ie) ** 2)
+        return IV

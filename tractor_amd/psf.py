@@ -1,0 +1,312 @@
+"""PSF models (tractor/psf.py): PixelizedPSF (Lanczos-shifted stamp; FFT
+spectrum for galaxies), GaussianMixturePSF, NCircularGaussianPSF,
+HybridPixelizedPSF.  Pixels are rendered by libtractor_hip."""
+import numpy as np
+
+from .utils import BaseParams, ParamList, MultiParams
+from .mixture_profiles import MixtureOfGaussians
+from .patch import Patch
+from . import gpu_render as G
+from . import render as R
+
+
+def lanczos_shift_image(img, dx, dy, inplace=False, force_python=False):
+    """tractor/psf.py:26-58 -- Lanczos-3 shift on the device."""
+    out = G.lanczos_shift(img, dx, dy)
+    if inplace:
+        img[:, :] = out
+    return out
+
+
+class HybridPSF(object):
+    """Marker: galaxies render wide components with the MoG (psf.py:65-66)."""
+    pass
+
+
+class PixelizedPSF(BaseParams):
+    """PSF from an odd-sized image stamp (tractor/psf.py:68-367), sampling 1."""
+
+    def __init__(self, img, sampling=1., Lorder=3):
+        img = np.asarray(img).astype(np.float32)
+        H, W = img.shape
+        assert (H % 2) == 1 and (W % 2) == 1
+        if sampling != 1.:
+            raise NotImplementedError('PixelizedPSF sampling != 1 is not supported')
+        self.img = img
+        self.radius = np.hypot(H / 2., W / 2.)
+        self.H, self.W = H, W
+        self.Lorder = Lorder
+        self.sampling = sampling
+        self.fftcache = {}
+
+    def __str__(self):
+        return 'PixelizedPSF'
+
+    def __getstate__(self):
+        d = self.__dict__.copy()
+        d.pop('_trh_spectra', None)
+        d['fftcache'] = {}
+        return d
+
+    def clear_cache(self):
+        self.fftcache = {}
+        self.__dict__.pop('_trh_spectra', None)
+
+    @property
+    def shape(self):
+        return (self.H, self.W)
+
+    def hashkey(self):
+        return ('PixelizedPSF', tuple(self.img.ravel()))
+
+    def copy(self):
+        return self.__class__(self.img.copy())
+
+    def getShifted(self, x0, y0):
+        return self
+
+    def constantPsfAt(self, x, y):
+        return self
+
+    def getRadius(self):
+        return self.radius
+
+    def getImage(self, px, py):
+        return self.img
+
+    def getPointSourcePatch(self, px, py, minval=0., modelMask=None, radius=None, derivs=False,
+                            **kwargs):
+        """tractor/psf.py:137-207."""
+        img = self.getImage(px, py)
+        if radius is not None:
+            R_ = int(np.ceil(radius))
+            H, W = img.shape
+            cx, cy = W // 2, H // 2
+            img = img[max(cy - R_, 0):min(cy + R_ + 1, H - 1), max(cx - R_, 0):min(cx + R_ + 1, W - 1)]
+        H, W = img.shape
+        ix = int(round(float(px)))
+        iy = int(round(float(py)))
+        dx = px - ix
+        dy = py - iy
+        x0 = ix - W // 2
+        y0 = iy - H // 2
+        if modelMask is None:
+            return Patch(x0, y0, G.lanczos_shift(img, dx, dy))
+        mm = (modelMask.x0, modelMask.y0, modelMask.w, modelMask.h)
+        if img is self.img:
+            p = G.point_render(self, px, py, mm)
+        else:
+            p = G.point_render(PixelizedPSF(img), px, py, mm)
+        if p is None:
+            return None
+        patch = Patch(modelMask.x0, modelMask.y0, p)
+        if not derivs:
+            return patch
+        ddx = np.zeros_like(p)
+        ddx[1:-1, 1:-1] = -(p[1:-1, 2:] - p[1:-1, :-2]) / 2.
+        ddy = np.zeros_like(p)
+        ddy[1:-1, 1:-1] = -(p[2:, 1:-1] - p[:-2, 1:-1]) / 2.
+        return patch, Patch(patch.x0, patch.y0, ddx), Patch(patch.x0, patch.y0, ddy)
+
+    def getFourierTransformSize(self, radius):
+        return R.fft_size(radius)
+
+    def getFourierTransform(self, px, py, radius):
+        """(P, (cx, cy), (pH, pW), (v, w)) of psf.py:249-280, with P the
+        device half spectrum (torch complex64 view) instead of NumPy."""
+        sz = self.getFourierTransformSize(radius)
+        if sz not in self.fftcache:
+            spec, cx, cy = G._spectra(self).get(sz)
+            self.fftcache[sz] = (spec, (cx, cy), (sz, sz),
+                                 (np.fft.rfftfreq(sz), np.fft.fftfreq(sz)))
+        return self.fftcache[sz]
+
+
+class GaussianMixturePSF(BaseParams):
+    """PSF as a mixture of Gaussians (tractor/psf.py:369-528):
+    GaussianMixturePSF(amp, mean, var) or (a0.., mx0, my0, .., vxx, vyy, vxy ...)."""
+
+    def __init__(self, *args):
+        if len(args) == 3 and not np.isscalar(args[0]):
+            amp, mean, var = args
+            self.mog = MixtureOfGaussians(np.atleast_1d(amp), np.atleast_2d(mean), var)
+        else:
+            n = len(args)
+            K = n // 6
+            a = np.array(args, float)
+            amp = a[:K]
+            mean = a[K:3 * K].reshape(K, 2)
+            v = a[3 * K:].reshape(K, 3)
+            var = np.zeros((K, 2, 2))
+            var[:, 0, 0] = v[:, 0]
+            var[:, 1, 1] = v[:, 1]
+            var[:, 0, 1] = var[:, 1, 0] = v[:, 2]
+            self.mog = MixtureOfGaussians(amp, mean, var)
+        self.radius = 25
+        K = self.mog.K
+        self.stepsizes = [0.01] * K + [0.01] * (K * 2) + [0.1] * (K * 3)
+
+    def __str__(self):
+        return ('GaussianMixturePSF: amps=' + str(tuple(self.mog.amp.ravel())) +
+                ', means=' + str(tuple(self.mog.mean.ravel())) +
+                ', var=' + str(tuple(self.mog.var.ravel())))
+
+    def copy(self):
+        return GaussianMixturePSF(self.mog.amp.copy(), self.mog.mean.copy(), self.mog.var.copy())
+
+    def hashkey(self):
+        return ('GaussianMixturePSF', tuple(self.mog.amp), tuple(self.mog.mean.ravel()),
+                tuple(self.mog.var.ravel()))
+
+    def getShifted(self, x0, y0):
+        return self
+
+    def constantPsfAt(self, x, y):
+        return self
+
+    def getMixtureOfGaussians(self, px=None, py=None):
+        return self.mog
+
+    def getRadius(self):
+        return self.radius
+
+    def getPointSourcePatch(self, px, py, minval=0., radius=None, derivs=False, minradius=None,
+                            modelMask=None, clipExtent=None, **kwargs):
+        """tractor/psf.py:458-528 (minval = 0 paths: exact grid / masked)."""
+        if modelMask is not None:
+            if modelMask.mask is None:
+                return Patch(modelMask.x0, modelMask.y0,
+                             G.mog_render(self.mog.amp, self.mog.mean, self.mog.var,
+                                          modelMask.x0, modelMask.y0, modelMask.w, modelMask.h,
+                                          px, py))
+            r = G.mog_render(self.mog.amp, self.mog.mean, self.mog.var, modelMask.x0,
+                             modelMask.y0, modelMask.w, modelMask.h, px, py,
+                             mask=modelMask.mask, derivs=derivs, masked_rules=True)
+            if derivs:
+                return tuple(Patch(modelMask.x0, modelMask.y0, q) for q in r)
+            return Patch(modelMask.x0, modelMask.y0, r)
+        if minval is not None and minval > 0.:
+            raise NotImplementedError('approximate (minval > 0) MoG rendering is not supported')
+        r = self.getRadius() if radius is None else radius
+        x0 = int(np.floor(px - r))
+        x1 = int(np.ceil(px + r)) + 1
+        y0 = int(np.floor(py - r))
+        y1 = int(np.ceil(py + r)) + 1
+        if clipExtent is not None:
+            xl, xh, yl, yh = clipExtent
+            x0, x1, y0, y1 = max(x0, xl), min(x1, xh), max(y0, yl), min(y1, yh)
+        if x0 >= x1 or y0 >= y1:
+            return None
+        return Patch(x0, y0, G.mog_render(self.mog.amp, self.mog.mean, self.mog.var, x0, y0,
+                                          x1 - x0, y1 - y0, px, py))
+
+
+class NCircularGaussianPSF(MultiParams):
+    """N concentric circular Gaussians (tractor/psf.py:792-929)."""
+
+    def __init__(self, sigmas, weights):
+        assert len(sigmas) == len(weights)
+        ps = ParamList(*sigmas)
+        ps.stepsizes = [0.01] * len(sigmas)
+        pw = ParamList(*weights)
+        pw.stepsizes = [0.01] * len(weights)
+        super(NCircularGaussianPSF, self).__init__(ps, pw)
+        self.minradius = 1.
+
+    @staticmethod
+    def getNamedParams():
+        return dict(sigmas=0, weights=1)
+
+    def __str__(self):
+        return ('NCircularGaussianPSF: sigmas [ ' + ', '.join('%.3f' % s for s in self.mysigmas) +
+                ' ], weights [ ' + ', '.join('%.3f' % w for w in self.myweights) + ' ]')
+
+    def copy(self):
+        return NCircularGaussianPSF(list(self.mysigmas), list(self.myweights))
+
+    def getShifted(self, x0, y0):
+        return self
+
+    def constantPsfAt(self, x, y):
+        return self
+
+    @property
+    def mysigmas(self):
+        return self.sigmas.vals
+
+    @property
+    def myweights(self):
+        return self.weights.vals
+
+    def getMixtureOfGaussians(self, px=None, py=None):
+        """getCircularMog (psf.py:931-938): float32 amps and variances."""
+        K = len(self.mysigmas)
+        amps = np.array(self.myweights).astype(np.float32)
+        var = np.zeros((K, 2, 2), np.float32)
+        for k, s in enumerate(self.mysigmas):
+            var[k, 0, 0] = var[k, 1, 1] = s ** 2
+        return MixtureOfGaussians(amps, np.zeros((K, 2), np.float32), var, quick=True)
+
+    def getNSigma(self):
+        return 5.
+
+    def getRadius(self):
+        if hasattr(self, 'radius'):
+            return self.radius
+        return max(self.minradius, max(self.mysigmas) * self.getNSigma())
+
+    def getPointSourcePatch(self, px, py, minval=0., radius=None, modelMask=None,
+                            clipExtent=None, **kwargs):
+        """tractor/psf.py:898-929 (minval = 0: exact grid)."""
+        if modelMask is not None:
+            x0, x1, y0, y1 = modelMask.extent
+        else:
+            ix, iy = int(round(px)), int(round(py))
+            rad = int(np.ceil(self.getRadius())) if radius is None else radius
+            x0, x1, y0, y1 = ix - rad, ix + rad + 1, iy - rad, iy + rad + 1
+        if clipExtent is not None:
+            xl, xh, yl, yh = clipExtent
+            x0, x1, y0, y1 = max(x0, xl), min(x1, xh), max(y0, yl), min(y1, yh)
+        if minval is not None and minval > 0.:
+            raise NotImplementedError('approximate (minval > 0) MoG rendering is not supported')
+        mix = self.getMixtureOfGaussians()
+        return Patch(x0, y0, G.mog_render(mix.amp, mix.mean, mix.var, x0, y0, x1 - x0, y1 - y0,
+                                          px, py))
+
+
+class HybridPixelizedPSF(HybridPSF):
+    """PixelizedPSF + a Gaussian-mixture approximation (tractor/psf.py:600-655).
+    gauss=None would need an EM fit of the stamp (tractor/emfit.i), which is
+    not built: pass the MoG explicitly."""
+
+    def __init__(self, pix, gauss=None, N=2, cx=0., cy=0.):
+        if gauss is None:
+            raise NotImplementedError('HybridPixelizedPSF needs an explicit gauss= MoG PSF')
+        self.__dict__['pix'] = pix
+        self.__dict__['gauss'] = gauss
+
+    def __str__(self):
+        return 'HybridPixelizedPSF: Pix %s' % str(self.pix)
+
+    def copy(self):
+        return HybridPixelizedPSF(self.pix.copy(), self.gauss.copy())
+
+    def getMixtureOfGaussians(self, **kwargs):
+        return self.gauss.getMixtureOfGaussians(**kwargs)
+
+    def getShifted(self, dx, dy):
+        return HybridPixelizedPSF(self.pix.getShifted(dx, dy), self.gauss)
+
+    def constantPsfAt(self, x, y):
+        return HybridPixelizedPSF(self.pix.constantPsfAt(x, y), self.gauss)
+
+    def __getattr__(self, name):
+        if name in ('pix', 'gauss') or name.startswith('__'):
+            raise AttributeError(name)
+        return getattr(self.__dict__['pix'], name)
+
+    def __getstate__(self):
+        return (self.gauss, self.pix)
+
+    def __setstate__(self, state):
+        self.__dict__['gauss'], self.__dict__['pix'] = state

@@ -12,7 +12,7 @@ import torch
 from . import _lib
 from ._lib import FFT_JOB, MOG_JOB, TRH_STORE, TRH_ADD, TRH_CHI2, TRH_MAXK, check, ptr
 
-SUPPORTED_FFT_SIZES = (32, 64, 128, 256)
+SUPPORTED_FFT_SIZES = (32, 64, 128, 256, 512)
 
 
 def fft_size(radius):

@@ -699,3 +699,7 @@ class MultiParams(BaseParams, NamedParams):
         if rA == []:
             return None
         return rA, cA, vA, pb, mub
+
+
+def _isint(i):
+    return isinstance(i, (int, np.integer))
