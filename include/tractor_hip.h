@@ -80,15 +80,38 @@ typedef struct trh_mog_job {
     float   pad;
 } trh_mog_job;
 
+/*
+ * One adaptive ring-growth MoG evaluation (c_gauss_2d_approx3,
+ * tractor/approx3.c:1-465, called by MixtureOfGaussians.evaluate_grid_approx3,
+ * tractor/mixture_profiles.py:345-411).  FP64 throughout.  The box is
+ * [x0, x0+w) x [y0, y0+h); components [comp0, comp0+K) are evaluated at
+ * (pixel - (fx, fy)); growth starts at (xc, yc) (moved to the brightest edge
+ * pixel when outside the box); pixels never reached are written as 0.
+ */
+#define TRH_APPROX_MAXK   64      /* max components per approx3 job           */
+#define TRH_APPROX_MAX_WH 8192    /* max w + h of an approx3 job              */
+typedef struct trh_mog_approx_job {
+    int64_t out_offset;      /* element offset of the window in out / derivs  */
+    int32_t out_stride;      /* row stride (elements)                          */
+    int32_t x0, y0, w, h;    /* box                                            */
+    int32_t comp0, K;        /* components of the amp/mean/var arrays          */
+    int32_t xc, yc;          /* start pixel (image coordinates)                */
+    int32_t minradius;       /* rings <= minradius always grow                 */
+    int32_t want_derivs;     /* write d/dmean_x, d/dmean_y                     */
+    int32_t pad;
+    double  fx, fy;          /* offset of the component means                  */
+    double  minval;          /* growth stops below this value                  */
+} trh_mog_approx_job;
+
 /* Library handle: per-device twiddle tables and scratch. */
 typedef struct trh_handle trh_handle;
 
 const char *trh_last_error(void);
 int trh_version(void);
 
-/* sizeof(trh_fft_job), sizeof(trh_mog_job) as compiled (host only, no GPU
- * call): lets bindings check their struct layouts. */
-int trh_struct_sizes(int *fft_job, int *mog_job);
+/* sizeof(trh_fft_job), sizeof(trh_mog_job), sizeof(trh_mog_approx_job) as
+ * compiled (host only, no GPU call): lets bindings check their struct layouts. */
+int trh_struct_sizes(int *fft_job, int *mog_job, int *mog_approx_job);
 
 /* Device -> host copy of `bytes` bytes, synchronous on `stream` (binding
  * plumbing for reading library-owned device buffers; no reference counterpart). */
@@ -142,6 +165,21 @@ int trh_render_mog_batch(trh_handle *h, const trh_mog_job *jobs, int njobs, int 
                          const float *cxf, const float *cyf, const float *var,
                          const uint8_t *mask, float *out, float *dx_out, float *dy_out,
                          void *stream);
+
+/*
+ * Batched ring-growth MoG evaluation (replaces tractor.mix.c_gauss_2d_approx3,
+ * tractor/approx3.c:1-465; one SWIG call per evaluate_grid_approx3,
+ * tractor/mixture_profiles.py:382-386).  Device arrays: jobs[njobs];
+ * amp[c], mean[c][2], var[c][2][2] (float64); out (and dx_out, dy_out when a
+ * job wants derivatives) float64; bbox[njobs][4] = (sx0, sx1, sy0, sy1), the
+ * half-open box-relative bounds of the evaluated pixels (all -1 for a job
+ * whose w, h or K are out of range).  max_wh >= every job's w + h, and
+ * <= TRH_APPROX_MAX_WH (sizes the LDS mark buffers).
+ */
+int trh_render_mog_approx3_batch(trh_handle *h, const trh_mog_approx_job *jobs, int njobs,
+                                 int max_wh, const double *amp, const double *mean,
+                                 const double *var, double *out, double *dx_out, double *dy_out,
+                                 int32_t *bbox, void *stream);
 
 /*
  * Lanczos-3 shift of float32 images (lanczos_shift_image, tractor/psf.py:26-58

@@ -20,7 +20,7 @@ __all__ = [
     'mog_fourier_zero_mean', 'fft_size', 'pad_psf', 'psf_spectrum',
     'lanczos_taps3', 'correlate7f', 'lanczos_shift_image',
     'render_galaxy_fft', 'galaxy_patch_halfsize', 'mog_grid', 'mog_masked',
-    'mog_convolve', 'render_galaxy_mog', 'mog_psf_point_patch',
+    'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch',
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
@@ -348,6 +348,179 @@ def mog_masked(x0, y0, mask, fx, fy, amp, mean, var, derivs=False):
         yd[~mask] = 0
         return result, xd, yd
     return result
+
+
+def _approx3_prep(amp, var, minval):
+    """Per-component constants of c_gauss_2d_approx3 (approx3.c:96-126):
+    icov with -1/2 and the cross factor 2 folded in, scale = amp/(2 pi sqrt
+    det), and the exponent floor maxD = log(minval 2 pi sqrt(det) / sum amp),
+    clamped at -100 (or -100 when minval <= 0)."""
+    tpd = (2. * np.pi) ** 2
+    ampsum = float(np.sum(amp))
+    comps = []
+    for k in range(len(amp)):
+        V0 = var[k][0][0]
+        V1 = (var[k][0][1] + var[k][1][0]) * 0.5
+        V2 = var[k][1][1]
+        det = V0 * V2 - V1 * V1
+        isc = -0.5 / det
+        icov = (V2 * isc, -V1 * isc * 2.0, V0 * isc)
+        scale = amp[k] / math.sqrt(tpd * det)
+        if minval > 0.:
+            md = math.log(minval * math.sqrt(tpd * det) / ampsum)
+            md = max(md, -100.)
+        else:
+            md = -100.
+        comps.append((icov, scale, md))
+    return comps
+
+
+def _approx3_eval(comps, mean, x, y, cutoff):
+    """eval_all (mix.i:31-42, exponent floor -100) when cutoff is False,
+    eval_all_dxy (mix.i:44-82, per-component floor maxD, non-finite skipped)
+    when True.  Returns (value, d/dmean_x, d/dmean_y)."""
+    r = dxs = dys = 0.
+    for k, (I, scale, md) in enumerate(comps):
+        dx = x - mean[k][0]
+        dy = y - mean[k][1]
+        dsq = I[0] * dx * dx + I[1] * dx * dy + I[2] * dy * dy
+        if cutoff:
+            if not (dsq >= md) or not math.isfinite(dsq):
+                continue
+        elif dsq < -100:
+            continue
+        G = scale * math.exp(dsq)
+        r += G
+        dxs += -G * (2. * I[0] * dx + I[1] * dy)
+        dys += -G * (2. * I[2] * dy + I[1] * dx)
+    return r, dxs, dys
+
+
+def mog_grid_approx3(x0, x1, y0, y1, fx, fy, minval, amp, mean, var, xc, yc, minradius=3):
+    """c_gauss_2d_approx3 (approx3.c:1-465), FP64, pure Python (small cases).
+
+    Evaluation grows outward in square rings around the start pixel.  Each
+    ring has four sides: top row (y = yc+R), bottom row (y = yc-R), left
+    column (x = xc-R) and right column (x = xc+R); a side pixel is evaluated
+    only if it was marked by the previous ring.  An evaluated pixel whose
+    value is >= minval, or any pixel while R <= minradius, marks its outward
+    neighbours for the next ring: a top-row pixel marks the three pixels
+    above it and, at the ring's left/right end, two rows of the next left /
+    right column; bottom mirrors top; a left-column pixel marks three rows of
+    the next left column; right mirrors left.  Growth stops at the first ring
+    with no marked pixel.  If (xc, yc) lies outside the box, the start pixel
+    is the maximum of eval_all along the nearest edge(s) (first maximum in
+    scan order; box centre when every value is 0).
+
+    Returns (result, xderiv, yderiv, (sx0, sx1, sy0, sy1)) with the
+    half-open bounding box of evaluated pixels (box coordinates).
+    """
+    W = x1 - x0
+    H = y1 - y0
+    comps = _approx3_prep(amp, var, minval)
+    res = np.zeros((H, W))
+    xd = np.zeros((H, W))
+    yd = np.zeros((H, W))
+    if xc < x0 or yc < y0 or xc >= x1 or yc >= y1:
+        best = 0.
+        nx, ny = (x0 + x1) // 2, (y0 + y1) // 2
+        cand = []
+        if xc < x0 or xc >= x1:
+            ex = x0 if xc < x0 else x1 - 1
+            cand += [(ex, iy) for iy in range(y0, y1)]
+        if yc < y0 or yc >= y1:
+            ey = y0 if yc < y0 else y1 - 1
+            cand += [(ix, ey) for ix in range(x0, x1)]
+        for (ix, iy) in cand:
+            v = _approx3_eval(comps, mean, ix - fx, iy - fy, False)[0]
+            if v > best:
+                best, nx, ny = v, ix, iy
+        xc, yc = nx, ny
+
+    def ev(x, y):
+        r, a, b = _approx3_eval(comps, mean, x - fx, y - fy, True)
+        res[y - y0, x - x0] = r
+        xd[y - y0, x - x0] = a
+        yd[y - y0, x - x0] = b
+        return r
+
+    # marks for the next ring, in image coordinates: columns for top/bottom
+    # rows, rows for left/right columns; kept inside the box.
+    def inx(v):
+        return x0 <= v < x1
+
+    def iny(v):
+        return y0 <= v < y1
+
+    nT, nB, nL, nR = set(), set(), set(), set()
+    if yc > y0:
+        nB |= {v for v in (xc - 1, xc, xc + 1) if inx(v)}
+    if yc < y1 - 1:
+        nT |= {v for v in (xc - 1, xc, xc + 1) if inx(v)}
+    if xc > x0 and iny(yc):
+        nL.add(yc)
+    if xc < x1 - 1 and iny(yc):
+        nR.add(yc)
+    ev(xc, yc)
+    bx0 = bx1 = xc - x0
+    by0 = by1 = yc - y0
+    R = 0
+    while True:
+        R += 1
+        T, B, L, Rt = nT, nB, nL, nR
+        nT, nB, nL, nR = set(), set(), set(), set()
+        anything = False
+        yy = yc + R
+        if yy < y1 and T:
+            anything = True
+            by1 = yy - y0
+            for xx in sorted(T):
+                r = ev(xx, yy)
+                if R > minradius and r < minval:
+                    continue
+                if xx == xc - R and xx > x0:
+                    nL |= {v for v in (yy, yy - 1) if iny(v)}
+                if xx == xc + R and xx < x1 - 1:
+                    nR |= {v for v in (yy, yy - 1) if iny(v)}
+                if yy < y1 - 1:
+                    nT |= {v for v in (xx - 1, xx, xx + 1) if inx(v)}
+        yy = yc - R
+        if yy >= y0 and B:
+            anything = True
+            by0 = yy - y0
+            for xx in sorted(B):
+                r = ev(xx, yy)
+                if R > minradius and r < minval:
+                    continue
+                if xx == xc - R and xx > x0:
+                    nL |= {v for v in (yy, yy + 1) if iny(v)}
+                if xx == xc + R and xx < x1 - 1:
+                    nR |= {v for v in (yy, yy + 1) if iny(v)}
+                if yy > y0:
+                    nB |= {v for v in (xx - 1, xx, xx + 1) if inx(v)}
+        xx = xc - R
+        if xx >= x0 and L:
+            anything = True
+            bx0 = xx - x0
+            for yy in sorted(L):
+                r = ev(xx, yy)
+                if R > minradius and r < minval:
+                    continue
+                if xx > x0:
+                    nL |= {v for v in (yy - 1, yy, yy + 1) if iny(v)}
+        xx = xc + R
+        if xx < x1 and Rt:
+            anything = True
+            bx1 = xx - x0
+            for yy in sorted(Rt):
+                r = ev(xx, yy)
+                if R > minradius and r < minval:
+                    continue
+                if xx < x1 - 1:
+                    nR |= {v for v in (yy - 1, yy, yy + 1) if iny(v)}
+        if not anything:
+            break
+    return res, xd, yd, (bx0, bx1 + 1, by0, by1 + 1)
 
 
 def mog_convolve(amp, mean, var, pamp, pmean, pvar):

@@ -34,10 +34,11 @@ def test_exports_match_header():
 
 def test_struct_layouts():
     lib = _lib.load()      # checks sizes against the numpy dtypes
-    fs, ms = ctypes.c_int(), ctypes.c_int()
-    assert lib.trh_struct_sizes(ctypes.byref(fs), ctypes.byref(ms)) == 0
+    fs, ms, As = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.trh_struct_sizes(ctypes.byref(fs), ctypes.byref(ms), ctypes.byref(As)) == 0
     assert fs.value == _lib.FFT_JOB.itemsize == 328
     assert ms.value == _lib.MOG_JOB.itemsize == 64
+    assert As.value == _lib.MOG_APPROX_JOB.itemsize == 80
 
 
 def test_error_path_without_gpu():

@@ -32,6 +32,16 @@ MOG_JOB = np.dtype([
     ('scale', '<f4'), ('pad', '<f4'),
 ], align=True)
 
+# struct trh_mog_approx_job
+MOG_APPROX_JOB = np.dtype([
+    ('out_offset', '<i8'), ('out_stride', '<i4'), ('x0', '<i4'), ('y0', '<i4'),
+    ('w', '<i4'), ('h', '<i4'), ('comp0', '<i4'), ('K', '<i4'), ('xc', '<i4'), ('yc', '<i4'),
+    ('minradius', '<i4'), ('want_derivs', '<i4'), ('pad', '<i4'), ('fx', '<f8'), ('fy', '<f8'),
+    ('minval', '<f8'),
+], align=True)
+TRH_APPROX_MAXK = 64
+TRH_APPROX_MAX_WH = 8192
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -41,7 +51,7 @@ _PP = ctypes.POINTER(ctypes.c_void_p)
 SIGNATURES = {
     'trh_last_error': (ctypes.c_char_p, []),
     'trh_version': (_I, []),
-    'trh_struct_sizes': (_I, [_P, _P]),
+    'trh_struct_sizes': (_I, [_P, _P, _P]),
     'trh_copy_to_host': (_I, [_P, _P, ctypes.c_uint64, _P]),
     'trh_create': (_I, [_PP, _I]),
     'trh_destroy': (_I, [_P]),
@@ -51,6 +61,7 @@ SIGNATURES = {
     'trh_normal_eq_dense': (_I, [_P, _P, _I, _I, _P, _P]),
     'trh_solve_normal_f64': (_I, [_P, _P, _I, _D, _I, _P, _P, _P]),
     'trh_render_mog_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'trh_render_mog_approx3_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     'trh_render_point_batch': (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     'trh_fitter_create': (_I, [_P, _PP, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
@@ -86,11 +97,14 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    fs, ms = ctypes.c_int(0), ctypes.c_int(0)
-    lib.trh_struct_sizes(ctypes.byref(fs), ctypes.byref(ms))
-    if fs.value != FFT_JOB.itemsize or ms.value != MOG_JOB.itemsize:
-        raise TractorHipError('struct layout mismatch: fft_job %d vs %d, mog_job %d vs %d' % (
-            fs.value, FFT_JOB.itemsize, ms.value, MOG_JOB.itemsize))
+    fs, ms, As = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    lib.trh_struct_sizes(ctypes.byref(fs), ctypes.byref(ms), ctypes.byref(As))
+    if (fs.value != FFT_JOB.itemsize or ms.value != MOG_JOB.itemsize
+            or As.value != MOG_APPROX_JOB.itemsize):
+        raise TractorHipError(
+            'struct layout mismatch: fft_job %d vs %d, mog_job %d vs %d, mog_approx_job %d vs %d'
+            % (fs.value, FFT_JOB.itemsize, ms.value, MOG_JOB.itemsize, As.value,
+               MOG_APPROX_JOB.itemsize))
     _lib = lib
     return lib
 

@@ -55,9 +55,10 @@ int cu_count(trh_handle *h) { return h->n_cu; }
 extern "C" const char *trh_last_error(void) { return g_err.c_str(); }
 extern "C" int trh_version(void) { return 1; }
 
-extern "C" int trh_struct_sizes(int *fft_job, int *mog_job) {
+extern "C" int trh_struct_sizes(int *fft_job, int *mog_job, int *mog_approx_job) {
     if (fft_job) *fft_job = (int)sizeof(trh_fft_job);
     if (mog_job) *mog_job = (int)sizeof(trh_mog_job);
+    if (mog_approx_job) *mog_approx_job = (int)sizeof(trh_mog_approx_job);
     return TRH_OK;
 }
 

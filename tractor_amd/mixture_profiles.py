@@ -69,6 +69,51 @@ class MixtureOfGaussians(object):
             var[sl] = self.var + other.var[k]
         return MixtureOfGaussians(amp, mean, var, quick=True)
 
+    def evaluate_grid(self, x0, x1, y0, y1, cx, cy):
+        """Exact grid (evaluate_grid_dstn, tractor/mixture_profiles.py:270-283,
+        c_gauss_2d_grid) on the device -> Patch."""
+        from .patch import Patch
+        from . import gpu_render as G
+        return Patch(x0, y0, G.mog_render(self.amp, self.mean, self.var, x0, y0, x1 - x0, y1 - y0,
+                                          cx, cy))
+
+    def evaluate_grid_approx3(self, x0, x1, y0, y1, fx, fy, minval, derivs=False, minradius=3,
+                              doslice=True, maxmargin=100):
+        """Ring-growth evaluation that stops below minval
+        (tractor/mixture_profiles.py:345-411 -> c_gauss_2d_approx3) on the
+        device.  Returns a Patch (or (patch, dx, dy) with derivs), sliced to
+        the evaluated pixels when doslice; None when the first component lies
+        more than maxmargin pixels outside the box."""
+        from .patch import Patch
+        from . import gpu_render as G
+        cx = int(self.mean[0, 0] + fx)
+        cy = int(self.mean[0, 1] + fy)
+        if (cx < x0 - maxmargin or cx > x1 + maxmargin or
+                cy < y0 - maxmargin or cy > y1 + maxmargin):
+            return None
+        res, xd, yd, (sx0, sx1, sy0, sy1) = G.mog_render_approx3(
+            self.amp, self.mean, self.var, int(x0), int(x1), int(y0), int(y1), float(fx), float(fy),
+            float(minval), cx, cy, int(minradius), derivs=derivs)
+        if doslice:
+            slc = slice(sy0, sy1), slice(sx0, sx1)
+            res = res[slc].copy()
+            if derivs:
+                xd = xd[slc].copy()
+                yd = yd[slc].copy()
+            x0 += sx0
+            y0 += sy0
+        if derivs:
+            return Patch(x0, y0, res), Patch(x0, y0, xd), Patch(x0, y0, yd)
+        return Patch(x0, y0, res)
+
+
+def mixture_to_patch(mixture, x0, x1, y0, y1, minval=0., exactExtent=False):
+    """tractor/mixture_profiles.py:425-438."""
+    if minval == 0. or minval is None:
+        return mixture.evaluate_grid(x0, x1, y0, y1, 0., 0.)
+    return mixture.evaluate_grid_approx3(x0, x1, y0, y1, 0., 0., minval,
+                                         doslice=not exactExtent)
+
 
 def get_exp_mixture():
     amp = _exp_amp / np.sum(_exp_amp)

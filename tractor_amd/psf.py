@@ -4,7 +4,7 @@ HybridPixelizedPSF.  Pixels are rendered by libtractor_hip."""
 import numpy as np
 
 from .utils import BaseParams, ParamList, MultiParams
-from .mixture_profiles import MixtureOfGaussians
+from .mixture_profiles import MixtureOfGaussians, mixture_to_patch
 from .patch import Patch
 from . import gpu_render as G
 from . import render as R
@@ -172,7 +172,8 @@ class GaussianMixturePSF(BaseParams):
 
     def getPointSourcePatch(self, px, py, minval=0., radius=None, derivs=False, minradius=None,
                             modelMask=None, clipExtent=None, **kwargs):
-        """tractor/psf.py:458-528 (minval = 0 paths: exact grid / masked)."""
+        """tractor/psf.py:458-528: exact grid, masked (with derivatives) or ring-growth
+        (minval > 0) evaluation."""
         if modelMask is not None:
             if modelMask.mask is None:
                 return Patch(modelMask.x0, modelMask.y0,
@@ -185,8 +186,35 @@ class GaussianMixturePSF(BaseParams):
             if derivs:
                 return tuple(Patch(modelMask.x0, modelMask.y0, q) for q in r)
             return Patch(modelMask.x0, modelMask.y0, r)
-        if minval is not None and minval > 0.:
-            raise NotImplementedError('approximate (minval > 0) MoG rendering is not supported')
+        if minval is None:
+            minval = 0.
+        if minval > 0. or minradius is not None:
+            if radius is not None:
+                rr = radius
+            elif self.radius is not None:
+                rr = self.radius
+            else:
+                # overestimate of the radius where each component falls to minval
+                r = 0.
+                for v in self.mog.var:
+                    vv = v[0, 0] + v[1, 1]
+                    norm = 2. * np.pi * np.linalg.det(v)
+                    r2 = vv * -2. * np.log(minval * norm)
+                    if r2 > 0:
+                        r = max(r, np.sqrt(r2))
+                rr = int(np.ceil(r))
+            x0 = int(np.floor(px - rr))
+            x1 = int(np.ceil(px + rr)) + 1
+            y0 = int(np.floor(py - rr))
+            y1 = int(np.ceil(py + rr)) + 1
+            if clipExtent is not None:
+                xl, xh, yl, yh = clipExtent
+                x0, x1, y0, y1 = max(x0, xl), min(x1, xh), max(y0, yl), min(y1, yh)
+            if x0 >= x1 or y0 >= y1:
+                return None
+            kwa = {} if minradius is None else dict(minradius=minradius)
+            return self.mog.evaluate_grid_approx3(x0, x1, y0, y1, px, py, minval, derivs=derivs,
+                                                  **kwa)
         r = self.getRadius() if radius is None else radius
         x0 = int(np.floor(px - r))
         x1 = int(np.ceil(px + r)) + 1
@@ -257,7 +285,7 @@ class NCircularGaussianPSF(MultiParams):
 
     def getPointSourcePatch(self, px, py, minval=0., radius=None, modelMask=None,
                             clipExtent=None, **kwargs):
-        """tractor/psf.py:898-929 (minval = 0: exact grid)."""
+        """tractor/psf.py:898-929 via mixture_to_patch (exact grid or ring growth)."""
         if modelMask is not None:
             x0, x1, y0, y1 = modelMask.extent
         else:
@@ -267,11 +295,14 @@ class NCircularGaussianPSF(MultiParams):
         if clipExtent is not None:
             xl, xh, yl, yh = clipExtent
             x0, x1, y0, y1 = max(x0, xl), min(x1, xh), max(y0, yl), min(y1, yh)
-        if minval is not None and minval > 0.:
-            raise NotImplementedError('approximate (minval > 0) MoG rendering is not supported')
         mix = self.getMixtureOfGaussians()
-        return Patch(x0, y0, G.mog_render(mix.amp, mix.mean, mix.var, x0, y0, x1 - x0, y1 - y0,
-                                          px, py))
+        # the reference adds (px, py) into the float32 means (psf.py:921-925)
+        mean = mix.mean.copy()
+        mean[:, 0] += px
+        mean[:, 1] += py
+        mix = MixtureOfGaussians(mix.amp, mean, mix.var, quick=True)
+        return mixture_to_patch(mix, x0, x1, y0, y1, minval=minval,
+                                exactExtent=(modelMask is not None))
 
 
 class HybridPixelizedPSF(HybridPSF):
