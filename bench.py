@@ -11,6 +11,11 @@ start point.  value = sources fit per second.
 --workload c2: 4096 sources, one step = one LM iteration (Jacobian renders,
 normal equations, FP64 solve, line search); value = source-iterations/s.
 
+--workload c5: the config-4 sources fit from CNN proposals (the CNN is first
+trained on the GPU on rendered cutouts, outside the timed region); one step =
+cutouts + CNN inference + batched fit + fallback; the plain fit of the same
+data is timed beside it.
+
 Prints one JSON line.
 """
 import argparse
@@ -97,7 +102,10 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', default='c4', choices=['c4', 'c2'])
+    ap.add_argument('--workload', default='c4', choices=['c4', 'c2', 'c5'])
+    ap.add_argument('--cnn-train', type=int, default=1 << 20,
+                    help='c5: number of rendered training cutouts')
+    ap.add_argument('--cnn-epochs', type=int, default=4)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--dchisq', type=float, default=0.,
@@ -110,6 +118,8 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    if args.workload == 'c5':
+        return main_c5(args, torch)
     log('building workload', args.workload)
     wl = build_workload(args.workload, torch)
     f = wl['fitter']
@@ -194,6 +204,115 @@ def main():
             'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
         },
         'cpu_baseline': cpu,
+    }
+    if parity is not None:
+        out['parity'] = parity
+    print(json.dumps(out), flush=True)
+
+
+def roofline_of(prof):
+    """Dominant kernel slot (largest total time) -> roofline dict."""
+    k = int(np.argmax(prof[:, 0]))
+    ms, launches, jobs, nbytes = prof[k]
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = nbytes / max(launches, 1) / avg_s / 1e9
+    return {'bound': 'hbm', 'kernel': SLOT_NAMES[k], 'achieved': round(achieved, 2),
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
+            'traffic': None, 'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches)}
+
+
+def main_c5(args, torch):
+    """Surrogate-initialised fitting of the config-4 workload: train the CNN
+    on rendered cutouts (reported, not timed), then time the surrogate
+    pipeline (cutouts + CNN + batched fit + fallback) and the plain batched
+    fit on the same data, same stopping rule."""
+    from tractor_amd import surrogate as S
+    log('building workload c4 for c5')
+    wl = build_workload('c4', torch)
+    f = wl['fitter']
+    n = wl['n']
+    t0 = time.perf_counter()
+    X, T, L = S.make_training_set(args.cnn_train, chunk=16384, seed=20261203 + 5000)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    net = S.SurrogateCNN(max_batch=1024, seed=5)
+    hist = net.fit(X, T, L, epochs=args.cnn_epochs, batch=1024, lr=3e-3)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log('cnn: %d cutouts rendered in %.2fs, %d epochs in %.2fs, loss %.4f -> %.4f' % (
+        args.cnn_train, t1 - t0, args.cnn_epochs, t2 - t1, hist[0][1], hist[-1][1]))
+    del X, T, L
+    torch.cuda.empty_cache()
+
+    for _ in range(args.warmup):
+        run_step(wl, torch, args.dchisq)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for _ in range(args.steps):
+        run_step(wl, torch, args.dchisq)
+    torch.cuda.synchronize()
+    plain_dt = (time.perf_counter() - ta) / args.steps
+    plain = f.status()
+
+    kw = dict(plain_kind=wl['kind'], dchisq=args.dchisq)
+    for _ in range(args.warmup):
+        S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'], wl['psf'],
+                            wl['start'], **kw)
+    torch.cuda.synchronize()
+    prof = {}
+    ta = time.perf_counter()
+    for i in range(args.steps):
+        res = S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'],
+                                  wl['psf'], wl['start'], profile=prof if i == args.steps - 1 else None,
+                                  **kw)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - ta) / args.steps
+    value = n / dt
+    rel = (res['chisq'] - plain['chisq']) / plain['chisq']
+    acc = float(np.mean(res['proposed_kind'] == wl['kind']))
+    log('surrogate %.0f sources/s vs plain %.0f; timings %s; iterations %.2f vs %.2f' % (
+        value, n / plain_dt, res['timings'], res['iterations'].mean(), plain['iterations'].mean()))
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        m, cdt, rr = cpu_baseline(wl, args.cpu_seconds, dchisq=args.dchisq)
+        cpu = dict(value=round(m / cdt, 4), unit='sources/s', cores=1, kind='port',
+                   sample='first %d sources of the same workload (same data, standard start and '
+                          'type), NumPy FP64 oracle, one process' % m)
+        e = np.array([(res['chisq'][s_] - chi) / chi for (s_, p_, chi, nit) in rr])
+        parity = {'sources': len(rr), 'frac_within_1e-4_of_fp64': float(np.mean(np.abs(e) <= 1e-4)),
+                  'frac_not_worse_than_fp64_by_1e-4': float(np.mean(e <= 1e-4)),
+                  'max_rel_chisq_vs_fp64': float(np.max(np.abs(e)))}
+    out = {
+        'metric': METRIC, 'value': round(value, 3), 'unit': 'sources/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(dt * 1e3, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'bf16 MFMA / f32 acc (CNN); f32 (renders, J^T W J) + f64 (Cholesky, chi2)',
+        'data': 'synthetic',
+        'config': {
+            'workload': 'config5: surrogate-initialised batched fit of the config-4 workload '
+                        '(16384 sources, 30/40/30, P 16..128, 63x63 Moffat PSF, sigma=1); CNN '
+                        '4x conv3x3/s2 32ch + MLP, trained on %d rendered cutouts (Moffat FWHM '
+                        '3-6 px), %d epochs' % (args.cnn_train, args.cnn_epochs),
+            'sources': n, 'dchisq': args.dchisq, 'parallelism': 'single GPU',
+        },
+        'roofline': roofline_of(prof['slots']),
+        'cpu_baseline': cpu,
+        'surrogate': {
+            'plain_sources_per_s': round(n / plain_dt, 3),
+            'speedup_vs_plain': round(plain_dt / dt, 4),
+            'cnn_train_s': round(t2 - t1, 3), 'cnn_render_s': round(t1 - t0, 3),
+            'loss_first_last': [round(hist[0][1], 4), round(hist[-1][1], 4)],
+            'type_accuracy': round(acc, 4),
+            'mean_iterations_plain': round(float(plain['iterations'].mean()), 3),
+            'mean_iterations_surrogate': round(float(res['iterations'].mean()), 3),
+            'timings_s': {k: (round(v, 4) if isinstance(v, float) else v)
+                          for k, v in res['timings'].items()},
+            'frac_not_worse_than_plain_by_1e-4': round(float(np.mean(rel <= 1e-4)), 5),
+            'frac_within_1e-4_of_plain': round(float(np.mean(np.abs(rel) <= 1e-4)), 5),
+        },
     }
     if parity is not None:
         out['parity'] = parity

@@ -223,6 +223,51 @@ int trh_normal_eq_dense(trh_handle *h, const float *Ab, int nrows, int ncols, do
 int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, int scale_columns,
                          double *x, int *info, void *stream);
 
+/*
+ * Surrogate CNN (no reference counterpart: the learned initialiser that
+ * proposes a source's type and starting parameters for the batched fit).
+ * Activations are NHWC bfloat16 stored as uint16; convolutions are 3x3,
+ * stride 2, pad 1, 32 output channels, ReLU, computed with BF16 MFMA and FP32
+ * accumulation.  H and W are even.
+ *   conv_fwd:        x [B][H][W][cin] (cin 1 or 32), wf [32][K] (K = 16 for
+ *                    cin 1, taps 9..15 zero; else 288 = tap*32 + ci), bias[32]
+ *                    -> y [B][H/2][W/2][32] = relu(conv + bias)
+ *   conv_bwd_data:   dy [B][H/2][W/2][32] (gradient at the layer's
+ *                    pre-activation), wt [ci][tap][co], x = the layer's input
+ *                    [B][H][W][32] -> dx = (W^T dy) masked by x > 0 (the
+ *                    previous layer's pre-activation gradient)
+ *   conv_bwd_weight: dw [32][K'] += (K' = 9 for cin 1, 288 otherwise; FP32),
+ *                    db[32] +=
+ *   head:            global average pool of y4 [B][2][2][32], MLP 32 -> 64
+ *                    (ReLU) -> 9 = 6 parameters + 3 type logits; params
+ *                    W1[64][32], b1[64], W2[9][64], b2[9].  With targets
+ *                    [B][6] and labels [B]: loss (mean 0.5 sum rw_i e_i^2,
+ *                    parameters 3..5 only for labels > 0, + tw * cross
+ *                    entropy) added to *loss, dy4 written, head gradients
+ *                    added to dparams; scratch holds B * 169 floats.
+ *   adam:            p -= lr_t m / (sqrt(v) + eps) with m, v updated from
+ *                    g + wd p.
+ *   pack:            FP32 master conv weights [32][9][cin] -> wf (and wt).
+ *   cutouts:         32x32 windows of patches (patch s of side P[s] at
+ *                    pix_off[s]) centred on pixel (P/2, P/2), zero outside,
+ *                    divided by the window maximum (maxv[s]) -> bf16.
+ */
+int trh_cnn_conv_fwd(trh_handle *h, const uint16_t *x, const uint16_t *wf, const float *bias,
+                     uint16_t *y, int B, int H, int W, int cin, void *stream);
+int trh_cnn_conv_bwd_data(trh_handle *h, const uint16_t *dy, const uint16_t *wt, const uint16_t *x,
+                          uint16_t *dx, int B, int H, int W, void *stream);
+int trh_cnn_conv_bwd_weight(trh_handle *h, const uint16_t *dy, const uint16_t *x, float *dw,
+                            float *db, int B, int H, int W, int cin, void *stream);
+int trh_cnn_head(trh_handle *h, const uint16_t *y4, const float *params, int B,
+                 const float *targets, const int32_t *labels, const float *rw, float tw,
+                 float *out, float *loss, uint16_t *dy4, float *scratch, float *dparams,
+                 void *stream);
+int trh_cnn_adam(trh_handle *h, float *p, const float *g, float *m, float *v, int n, float lr_t,
+                 float b1, float b2, float eps, float wd, void *stream);
+int trh_cnn_pack(trh_handle *h, const float *w, int cin, uint16_t *wf, uint16_t *wt, void *stream);
+int trh_cnn_cutouts(trh_handle *h, const float *data, const int64_t *pix_off, const int32_t *P,
+                    int n, uint16_t *out, float *maxv, void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched single-source fitting (config workloads: thousands of sources, one
  * modelMask patch each).  Replaces GpuOptimizer's per-source

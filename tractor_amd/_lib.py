@@ -64,6 +64,14 @@ SIGNATURES = {
     'trh_render_mog_approx3_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     'trh_render_point_batch': (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
+    'trh_cnn_conv_fwd': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    'trh_cnn_conv_bwd_data': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
+    'trh_cnn_conv_bwd_weight': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    'trh_cnn_head': (_I, [_P, _P, _P, _I, _P, _P, _P, ctypes.c_float, _P, _P, _P, _P, _P, _P]),
+    'trh_cnn_adam': (_I, [_P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_float, _P]),
+    'trh_cnn_pack': (_I, [_P, _P, _I, _P, _P, _P]),
+    'trh_cnn_cutouts': (_I, [_P, _P, _P, _P, _I, _P, _P, _P]),
     'trh_fitter_create': (_I, [_P, _PP, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
     'trh_fitter_destroy': (_I, [_P]),
     'trh_fitter_set_params': (_I, [_P, _P, _P]),
