@@ -127,17 +127,19 @@ def main():
         run_step(wl, torch, args.dchisq)
     torch.cuda.synchronize()
 
-    f.profile(True)
-    torch.cuda.synchronize()
     iters = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         iters.append(run_step(wl, torch, args.dchisq))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    dt = (t1 - t0) / args.steps
+    # one more step, outside the timed region, with per-kernel HIP-event
+    # timing (it synchronises every iteration) for the roofline figures
+    f.profile(True)
+    run_step(wl, torch, args.dchisq)
     prof = f.profile_read()
     f.profile(False)
-    dt = (t1 - t0) / args.steps
     value = wl['n'] / dt
     st = f.status()
     log('per-step iterations', iters, 'max per-source iterations', int(st['iterations'].max()))
@@ -145,15 +147,11 @@ def main():
     log('per-source iteration histogram (iterations: sources):',
         {int(i): int(c) for i, c in enumerate(hist) if c})
 
-    # dominant kernel (largest total time over the timed region)
-    k = int(np.argmax(prof[:, 0]))
-    ms, launches, jobs, nbytes = prof[k]
-    avg_s = ms / 1e3 / max(launches, 1)
-    achieved = nbytes / max(launches, 1) / avg_s / 1e9
     slots = {SLOT_NAMES[i]: dict(ms=round(float(prof[i, 0]), 3), launches=int(prof[i, 1]),
-                                 jobs=int(prof[i, 2]), gbytes=round(float(prof[i, 3]) / 1e9, 4))
+                                 jobs=int(prof[i, 2]), gbytes=round(float(prof[i, 3]) / 1e9, 4),
+                                 gbs=round(float(prof[i, 3]) / max(float(prof[i, 0]), 1e-9) / 1e6, 1))
              for i in range(len(SLOT_NAMES)) if prof[i, 1] > 0}
-    log('per-slot totals over the timed region:', json.dumps(slots))
+    log('per-slot totals over one profiled step:', json.dumps(slots))
 
     cpu = None
     parity = None
@@ -197,12 +195,7 @@ def main():
             'dchisq': args.dchisq,
             'parallelism': 'single GPU',
         },
-        'roofline': {
-            'bound': 'hbm', 'kernel': SLOT_NAMES[k],
-            'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': round(achieved / HBM_PEAK_GBS, 5), 'traffic': None,
-            'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
-        },
+        'roofline': roofline_of(prof),
         'cpu_baseline': cpu,
     }
     if parity is not None:
@@ -210,15 +203,37 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+SLOT_SZ = {0: 32, 1: 64, 2: 128, 3: 256, 4: 512, 8: 32, 9: 64, 10: 128, 11: 256, 12: 512}
+MEAN_K = (0.4 * 6 + 0.3 * 9) / 0.7   # MoG components per galaxy render (40% exp, 30% dev)
+FP32_PEAK_TFLOPS = 157.3
+
+
+def render_flops(sz, K=MEAN_K):
+    """Arithmetic of one FFT-path render: MoG Fourier transform (8 flops + 1
+    exp per component and frequency), spectrum product, the 2-D inverse FFT
+    (2.5 sz^2 log2 sz^2) and the two Lanczos-3 passes (28 per pixel)."""
+    nv = sz // 2 + 1
+    return K * sz * nv * 9 + 6 * sz * nv + 2.5 * sz * sz * math.log2(sz * sz) + 28 * sz * sz
+
+
 def roofline_of(prof):
-    """Dominant kernel slot (largest total time) -> roofline dict."""
+    """Dominant kernel slot (largest total time over the profiled step) ->
+    roofline dict: algorithmic HBM bytes per launch / mean launch time, and
+    for an FFT render slot the VALU arithmetic rate as well."""
     k = int(np.argmax(prof[:, 0]))
     ms, launches, jobs, nbytes = prof[k]
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = nbytes / max(launches, 1) / avg_s / 1e9
-    return {'bound': 'hbm', 'kernel': SLOT_NAMES[k], 'achieved': round(achieved, 2),
-            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
-            'traffic': None, 'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches)}
+    out = {'bound': 'hbm', 'kernel': SLOT_NAMES[k], 'achieved': round(achieved, 2),
+           'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
+           'traffic': None, 'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
+           'bytes_per_launch': round(nbytes / max(launches, 1)), 'jobs_per_launch': round(
+               jobs / max(launches, 1), 1)}
+    if k in SLOT_SZ:
+        tf = jobs * render_flops(SLOT_SZ[k]) / (ms / 1e3) / 1e12
+        out['valu'] = {'achieved': round(tf, 2), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                       'frac': round(tf / FP32_PEAK_TFLOPS, 4)}
+    return out
 
 
 def main_c5(args, torch):
@@ -259,14 +274,15 @@ def main_c5(args, torch):
         S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'], wl['psf'],
                             wl['start'], **kw)
     torch.cuda.synchronize()
-    prof = {}
     ta = time.perf_counter()
     for i in range(args.steps):
         res = S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'],
-                                  wl['psf'], wl['start'], profile=prof if i == args.steps - 1 else None,
-                                  **kw)
+                                  wl['psf'], wl['start'], **kw)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - ta) / args.steps
+    prof = {}
+    S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'], wl['psf'],
+                        wl['start'], profile=prof, **kw)
     value = n / dt
     rel = (res['chisq'] - plain['chisq']) / plain['chisq']
     acc = float(np.mean(res['proposed_kind'] == wl['kind']))

@@ -125,8 +125,9 @@ int trh_destroy(trh_handle *h);
 /*
  * PSF spectra (PixelizedPSF.getFourierTransform, tractor/psf.py:249-280, with
  * _padInImage 214-247): n float32 stamps [n][ph][pw] (ph, pw odd) are
- * padded/cropped into sz x sz and transformed to half spectra
- * out[n][sz][sz/2+1] (complex64 as float pairs).  sz in {32,64,128,256,512}.
+ * padded/cropped into sz x sz and transformed to half spectra stored column
+ * by column: out[n][sz/2+1][sz] (complex64 as float pairs; element [i][j] is
+ * frequency (v = i/sz, w = j/sz)).  sz in {32,64,128,256,512}.
  * Writes the centre (cx, cy) of the padded stamp to *cx, *cy (host ints).
  */
 int trh_psf_spectrum_batch(trh_handle *h, const float *stamps, int n, int ph, int pw,
@@ -136,8 +137,8 @@ int trh_psf_spectrum_batch(trh_handle *h, const float *stamps, int n, int ph, in
  * Batched FFT-path renders (replaces mp_fourier.gaussian_fourier_transform_zero_mean
  * + np.fft.irfftn + mp_fourier.lanczos_shift_3f, tractor/galaxy.py:459-472,
  * called once per source from galaxy.py:256).  All jobs use FFT size sz.
- * jobs: device array of njobs trh_fft_job.  psf_spec: [npsf][sz][sz/2+1]
- * complex64.  out: float32 (TRH_STORE / TRH_ADD).  data, inverr: float32
+ * jobs: device array of njobs trh_fft_job.  psf_spec: [npsf][sz/2+1][sz]
+ * complex64 (the trh_psf_spectrum_batch layout).  out: float32 (TRH_STORE / TRH_ADD).  data, inverr: float32
  * (TRH_CHI2).  chi2: double slots (TRH_CHI2).  Unused pointers may be NULL.
  */
 int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njobs,
@@ -146,11 +147,12 @@ int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njo
 
 /*
  * Launch shape of the FFT render kernel for FFT size sz (32..512): threads
- * per block (multiple of 64, <= 1024) and columns per phase-A chunk; 0 keeps
- * the default.  The chunk is reduced automatically to fit LDS.  Tuning knob
- * of the render schedule (no reference counterpart).
+ * per block (multiple of 64, <= 1024; one wavefront per spectral column or
+ * row pair) and the cap on G rows kept in LDS per pass (`rows`); 0 keeps the
+ * default.  Rows are reduced automatically to fit LDS.  Tuning knob of the
+ * render schedule (no reference counterpart).
  */
-int trh_set_fft_config(trh_handle *h, int sz, int threads, int ch);
+int trh_set_fft_config(trh_handle *h, int sz, int threads, int rows);
 
 /*
  * Batched pixel-space MoG evaluation (replaces tractor.mix.c_gauss_2d_grid,

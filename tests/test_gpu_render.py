@@ -42,7 +42,7 @@ def test_psf_spectrum(dev, sz):
     spec, cx, cy = sp.get(sz)
     torch.cuda.synchronize()
     got = spec.cpu().numpy()
-    got = got[..., 0] + 1j * got[..., 1]
+    got = (got[..., 0] + 1j * got[..., 1]).T     # stored [sz/2+1][sz]
     P, (ocx, ocy), _, _ = O.psf_spectrum(stamp, sz / 2.)
     assert (cx, cy) == (ocx, ocy)
     assert rel_err(got, P) < RTOL

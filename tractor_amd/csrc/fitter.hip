@@ -255,13 +255,73 @@ __global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
     lists[b * cap + i] = 4 * s;
 }
 
-// J^T W J, J^T W r and chi^2 for one source per block (grid-stride).
-// Columns (galaxy.py:137-223, pointsource.py:130-241, lsqr_optimizer.py:386-431):
+// Sum over the 64 lanes of a wave, result in every lane (xor butterfly:
+// DPP moves for spans 1..8, ds_swizzle for 16, ds_bpermute for 32).
+__device__ __forceinline__ float wave_allsum(float v) {
+    v += xlane<1>(v);
+    v += xlane<2>(v);
+    v += xlane<4>(v);
+    v += xlane<8>(v);
+    v += xlane<16>(v);
+    v += xlane<32>(v);
+    return v;
+}
+
+__device__ __forceinline__ double wave_allsum_d(double v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One pixel's contribution to the 27 FP32 sums (21 of J^T W J, 6 of J^T W r)
+// and to chi^2 (FP64).  Columns (galaxy.py:137-223, pointsource.py:130-241,
+// lsqr_optimizer.py:386-431):
 //   pos_i = ((dxp*cdi[0][i] + dyp*cdi[1][i]) * counts) * inverr  (central
 //           differences of the unit render, zero 1-px border)
 //   flux  = render * inverr
 //   shape = ((R_k - R_0) * (counts / 0.01)) * inverr
 //   b     = (data - counts * R_0) * inverr
+struct NeqPix {
+    const float *R0, *D, *E;
+    int P, PP, kd;
+    float invP, counts, cstep, c00, c01, c10, c11;
+    __device__ __forceinline__ void add(int idx, float (&acc)[27], double &chi2) const {
+        // exact for PP < 2^24: (idx + 0.5) / P is within 0.5/P of y + 0.5
+        const int y = (int)(((float)idx + 0.5f) * invP);
+        const int x = idx - y * P;
+        const float r0 = R0[idx];
+        float dxp = 0.f, dyp = 0.f;
+        if (x > 0 && x < P - 1 && y > 0 && y < P - 1) {
+            dxp = (R0[idx - 1] - R0[idx + 1]) / 2.f;
+            dyp = (R0[idx - P] - R0[idx + P]) / 2.f;
+        }
+        const float e = E[idx];
+        float col[6];
+        col[0] = ((dxp * c00 + dyp * c10) * counts) * e;
+        col[1] = ((dxp * c01 + dyp * c11) * counts) * e;
+        col[2] = r0 * e;
+        if (kd) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) col[3 + k] = ((R0[(k + 1) * PP + idx] - r0) * cstep) * e;
+        } else {
+            col[3] = col[4] = col[5] = 0.f;
+        }
+        const float b = (D[idx] - r0 * counts) * e;
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = i; j < 6; ++j) acc[q++] += col[i] * col[j];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) acc[21 + i] += col[i] * b;
+        chi2 += (double)b * (double)b;
+    }
+};
+
+// J^T W J, J^T W r (FP32 products and sums) and chi^2 (FP64) for one source
+// per 256-thread block (grid-stride over sources).  Each thread takes four
+// pixels per step so their loads are in flight together; the wave sums use
+// DPP/swizzle exchanges and the four waves combine through LDS.
 __global__ void __launch_bounds__(256)
 k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
                 const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
@@ -270,61 +330,42 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
                 const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
                 double cdi11, double *__restrict__ neq) {
     __shared__ double red[4][28];
-    const float c00 = (float)cdi00, c01 = (float)cdi01, c10 = (float)cdi10, c11 = (float)cdi11;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int s = blockIdx.x; s < S; s += gridDim.x) {
         if (!active[s]) continue;
-        const int kd = kind[s];
-        const int P = Parr[s];
-        const int np = kd ? 6 : 3;
-        const float counts = (float)params[8 * s + 2];
-        const float cstep = (float)(params[8 * s + 2] / 0.01);
-        const float *R0 = renders + roff[s];
-        const float *D = data + pix_off[s];
-        const float *E = inverr + pix_off[s];
-        const int PP = P * P;
+        NeqPix px;
+        px.kd = kind[s];
+        px.P = Parr[s];
+        px.PP = px.P * px.P;
+        px.invP = 1.0f / (float)px.P;
+        px.counts = (float)params[8 * s + 2];
+        px.cstep = (float)(params[8 * s + 2] / 0.01);
+        px.c00 = (float)cdi00;
+        px.c01 = (float)cdi01;
+        px.c10 = (float)cdi10;
+        px.c11 = (float)cdi11;
+        px.R0 = renders + roff[s];
+        px.D = data + pix_off[s];
+        px.E = inverr + pix_off[s];
         float acc[27];
 #pragma unroll
         for (int i = 0; i < 27; ++i) acc[i] = 0.f;
         double chi2 = 0.0;
-        for (int idx = threadIdx.x; idx < PP; idx += blockDim.x) {
-            const int y = idx / P;
-            const int x = idx - y * P;
-            const float r0 = R0[idx];
-            float dxp = 0.f, dyp = 0.f;
-            if (x > 0 && x < P - 1 && y > 0 && y < P - 1) {
-                dxp = (R0[idx - 1] - R0[idx + 1]) / 2.f;
-                dyp = (R0[idx - P] - R0[idx + P]) / 2.f;
-            }
-            const float e = E[idx];
-            float col[6];
-            col[0] = ((dxp * c00 + dyp * c10) * counts) * e;
-            col[1] = ((dxp * c01 + dyp * c11) * counts) * e;
-            col[2] = r0 * e;
-            if (np == 6) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) col[3 + k] = ((R0[(k + 1) * PP + idx] - r0) * cstep) * e;
-            } else {
-                col[3] = col[4] = col[5] = 0.f;
-            }
-            const float b = (D[idx] - r0 * counts) * e;
-            int q = 0;
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-#pragma unroll
-                for (int j = i; j < 6; ++j) acc[q++] += col[i] * col[j];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) acc[21 + i] += col[i] * b;
-            chi2 += (double)b * (double)b;
+        int idx = threadIdx.x;
+        for (; idx + 768 < px.PP; idx += 1024) {
+            px.add(idx, acc, chi2);
+            px.add(idx + 256, acc, chi2);
+            px.add(idx + 512, acc, chi2);
+            px.add(idx + 768, acc, chi2);
         }
-        // block reduction in FP64
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        double v[28];
+        for (; idx < px.PP; idx += 256) px.add(idx, acc, chi2);
 #pragma unroll
-        for (int i = 0; i < 27; ++i) v[i] = wave_sum((double)acc[i]);
-        v[27] = wave_sum(chi2);
+        for (int i = 0; i < 27; ++i) acc[i] = wave_allsum(acc[i]);
+        chi2 = wave_allsum_d(chi2);
         if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < 28; ++i) red[wid][i] = v[i];
+            for (int i = 0; i < 27; ++i) red[wid][i] = (double)acc[i];
+            red[wid][27] = chi2;
         }
         __syncthreads();
         if (threadIdx.x < 28) {

@@ -23,7 +23,7 @@ int tw_offset(int n);
 int pad_center(int sz, int p);
 int ensure_scratch(trh_handle *h, size_t bytes);
 int max_rows_for(int sz, int lds_max);
-void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *ch, size_t *lds,
+void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *rows, size_t *lds,
                       int *grid, int njobs);
 int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,

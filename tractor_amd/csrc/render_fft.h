@@ -9,9 +9,23 @@ namespace trh {
 // -2 pi^2 log2(e): exp(-2 pi^2 q) = exp2(C * q)
 #define TRH_NEG2PI2_LOG2E (-28.47765484177155)
 
-// LDS bytes (dynamic) used by k_render_fft<SZ>: twiddles + work + row buffer.
-__host__ __device__ inline size_t fft_lds_bytes(int sz, int ch, int maxrows) {
-    return sizeof(float2) * ((size_t)sz / 2 + (size_t)ch * sz + (size_t)maxrows * (sz / 2 + 1));
+// Geometry of the wavefront-level FFT: a wave holds NSEQ sequences of length
+// SZ, each lane R elements (element index sl + SEQL * r, sl = lane % SEQL).
+template <int SZ> struct WaveGeom {
+    static constexpr int R = SZ >= 64 ? SZ / 64 : 1;
+    static constexpr int SEQL = SZ >= 64 ? 64 : SZ;
+    static constexpr int NSEQ = 64 / SEQL;
+    static constexpr int NV = SZ / 2 + 1;
+    static constexpr int LBUF = SZ + SZ / 32;   // padded per-sequence line buffer (float2)
+};
+
+// Dynamic LDS of k_render_fft<SZ> with `nwaves` waves per block and `maxrows`
+// rows of the y-filtered half spectrum: per-wave line buffers (2 sequences
+// per wave iteration) + rows.
+__host__ __device__ inline size_t fft_lds_bytes(int sz, int nwaves, int maxrows) {
+    const int nseq = sz >= 64 ? 1 : 64 / sz;
+    const int lbuf = sz + sz / 32;
+    return sizeof(float2) * ((size_t)nwaves * nseq * 2 * lbuf + (size_t)maxrows * (sz / 2 + 1));
 }
 
 struct RenderScratch {
@@ -68,20 +82,138 @@ __device__ __forceinline__ double epilogue(int mode, float val, float scale, int
     return 0.0;
 }
 
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exchange with lane (lane ^ D).  D = 1, 2: one DPP quad permutation;
+// D = 4: row half-mirror then quad reverse; D = 8: row half-mirror then row
+// mirror (DPP moves are VALU ops, no LDS round trip); D = 16: ds_swizzle
+// (xor within 32-lane halves); D = 32: ds_bpermute.
+template <int D>
+__device__ __forceinline__ float xlane(float v) {
+    const int i = __float_as_int(v);
+    if constexpr (D == 1) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, i, 0xB1, 0xF, 0xF, true));
+    } else if constexpr (D == 2) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, i, 0x4E, 0xF, 0xF, true));
+    } else if constexpr (D == 4) {
+        const int t = __builtin_amdgcn_update_dpp(0, i, 0x141, 0xF, 0xF, true);
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, true));
+    } else if constexpr (D == 8) {
+        const int t = __builtin_amdgcn_update_dpp(0, i, 0x141, 0xF, 0xF, true);
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, t, 0x140, 0xF, 0xF, true));
+    } else if constexpr (D == 16) {
+        return __int_as_float(__builtin_amdgcn_ds_swizzle(i, (0x10 << 10) | 0x1F));
+    } else {
+        return __shfl_xor(v, 32, 64);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ float2 xlane2(float2 v) {
+    return make_float2(xlane<D>(v.x), xlane<D>(v.y));
+}
+
+// Per-lane twiddles of the wave FFT, loaded once per block: in-register
+// stages (span d >= 64) need tw[(sl + 64 q) * SZ/(2d)], q < d/64; cross-lane
+// stages tw[(sl & (d-1)) * SZ/(2d)].
+template <int SZ> struct WaveTw {
+    static constexpr int NREG = SZ >= 128 ? SZ / 64 - 1 : 0;
+    static constexpr int NLANE = SZ >= 64 ? 6 : Log2<SZ>::v;
+    float2 reg[NREG > 0 ? NREG : 1];
+    float2 lane[NLANE];
+    __device__ __forceinline__ void load(const float2 *tw, int sl) {
+        int n = 0;
+#pragma unroll
+        for (int d = SZ / 2; d >= 64; d >>= 1)
+#pragma unroll
+            for (int q = 0; q < d / 64; ++q) reg[n++] = tw[(sl + 64 * q) * (SZ / (2 * d))];
+        n = 0;
+#pragma unroll
+        for (int d = (SZ >= 64 ? 32 : SZ / 2); d >= 1; d >>= 1)
+            lane[n++] = tw[(sl & (d - 1)) * (SZ / (2 * d))];
+    }
+};
+
+template <int SZ, int D, int C>
+__device__ __forceinline__ void idif_lane_stage(float2 (&x)[C][WaveGeom<SZ>::R], float2 w,
+                                                bool up) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int r = 0; r < WaveGeom<SZ>::R; ++r) {
+            const float2 y = xlane2<D>(x[c][r]);
+            x[c][r] = up ? cmul(csub(y, x[c][r]), w) : cadd(x[c][r], y);
+        }
+}
+
+// Inverse (exp(+2 pi i jk/SZ)) radix-2 decimation-in-frequency FFT of C
+// independent sequences held by the lanes (interleaved for latency hiding):
+// natural order in, bit-reversed order out.  Stages with span >= 64 pair
+// registers of a lane; spans 32..1 pair lanes sl ^ span.  No LDS arrays, no
+// barriers.
+template <int SZ, int C>
+__device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const WaveTw<SZ> &T,
+                                          int sl) {
+    constexpr int R = WaveGeom<SZ>::R;
+    int n = 0;
+#pragma unroll
+    for (int d = SZ / 2; d >= 64; d >>= 1) {
+        const int rd = d / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r & rd) continue;
+            const float2 w = T.reg[n + (r & (rd - 1))];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float2 a = x[c][r], b = x[c][r + rd];
+                x[c][r] = cadd(a, b);
+                x[c][r + rd] = cmul(csub(a, b), w);
+            }
+        }
+        n += rd;
+    }
+    int s = 0;
+    if constexpr (SZ >= 64) {
+        idif_lane_stage<SZ, 32, C>(x, T.lane[s++], (sl & 32) != 0);
+    }
+    if constexpr (SZ >= 32) idif_lane_stage<SZ, 16, C>(x, T.lane[s++], (sl & 16) != 0);
+    idif_lane_stage<SZ, 8, C>(x, T.lane[s++], (sl & 8) != 0);
+    idif_lane_stage<SZ, 4, C>(x, T.lane[s++], (sl & 4) != 0);
+    idif_lane_stage<SZ, 2, C>(x, T.lane[s++], (sl & 2) != 0);
+    idif_lane_stage<SZ, 1, C>(x, T.lane[s++], (sl & 1) != 0);
+}
+
 // Render one FFT job (see trh_fft_job).  Every thread of the block calls this.
-// tw (LDS) must hold exp(+2 pi i m/SZ), m < SZ/2.  work: ch*SZ float2,
-// hbuf: maxrows*(SZ/2+1) float2.
+// Phase A: each wave takes CW columns v_i of the half spectrum at a time: MoG
+// Fourier transform x PSF spectrum for the SZ frequencies w_j, inverse FFT
+// along w in registers, y-Lanczos onto the G rows this block needs ->
+// hbuf[row][i].  Phase B: each wave takes CW pairs of G rows: the two half
+// spectra packed into one complex sequence, inverse FFT along v, x-Lanczos,
+// crop and epilogue.  Only the hand-off between the phases needs the block.
+// psf_spec: [psf][SZ/2+1][SZ] complex (column-major half spectrum).
+constexpr int CW = 2;
+
 template <int SZ, int MODE>
 __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
-                               const float2 *__restrict__ psf_spec, const float2 *tw,
-                               float2 *work, float2 *hbuf, int ch, int maxrows,
-                               RenderScratch &S, float *__restrict__ out,
-                               const float *__restrict__ data, const float *__restrict__ inverr,
-                               double *__restrict__ chi2) {
-    constexpr int NV = SZ / 2 + 1;
+                               const float2 *__restrict__ psf_spec, const WaveTw<SZ> &T,
+                               float2 *lbuf, float2 *hbuf, int maxrows, RenderScratch &S,
+                               float *__restrict__ out, const float *__restrict__ data,
+                               const float *__restrict__ inverr, double *__restrict__ chi2) {
+    using G = WaveGeom<SZ>;
+    constexpr int NV = G::NV, R = G::R, SEQL = G::SEQL, NSEQ = G::NSEQ;
     constexpr int LOG2 = Log2<SZ>::v;
+    constexpr int PER = NSEQ * CW;   // sequences per wave iteration
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    const int sl = lane & (SEQL - 1), sq = lane / SEQL;
+    float2 *lbw = lbuf + (size_t)wave * PER * G::LBUF;
     const int64_t off = Jp->out_offset;
     const int stride = Jp->out_stride;
     const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy;
@@ -91,10 +223,10 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     const float2 *__restrict__ P = psf_spec + (size_t)Jp->psf * SZ * NV;
 
     if (tid < K) {
-        const double C = TRH_NEG2PI2_LOG2E;
-        S.comp[tid] = make_float4(Jp->amp[tid], (float)(C * Jp->var[tid][0]),
-                                  (float)(2.0 * C * Jp->var[tid][1]),
-                                  (float)(C * Jp->var[tid][2]));
+        const double Cc = TRH_NEG2PI2_LOG2E;
+        S.comp[tid] = make_float4(Jp->amp[tid], (float)(Cc * Jp->var[tid][0]),
+                                  (float)(2.0 * Cc * Jp->var[tid][1]),
+                                  (float)(Cc * Jp->var[tid][2]));
     }
     compute_taps(Jp->mux, Jp->muy, S);
     const float *fx = S.taps;
@@ -119,99 +251,134 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 
     for (int rb0 = ylo; rb0 < yhi; rb0 += maxrows) {
         const int nrb = min(maxrows, yhi - rb0);
-        // ---- phase A: column IFFTs along w, y-Lanczos onto the needed rows
-        for (int c0 = 0; c0 < NV; c0 += ch) {
-            const int nc = min(ch, NV - c0);
-            for (int idx = tid; idx < nc * SZ; idx += nthr) {
-                const int j = idx / nc;
-                const int cc = idx - j * nc;
-                const int i = c0 + cc;
-                const float v = (float)i * (1.0f / SZ);
-                const float w = (float)(j < SZ / 2 ? j : j - SZ) * (1.0f / SZ);
-                const float v2 = v * v, vw = v * w, w2 = w * w;
-                float F = 0.f;
-                for (int k = 0; k < K; ++k) {
-                    const float4 c = S.comp[k];
-                    F += c.x * __builtin_amdgcn_exp2f(c.y * v2 + c.z * vw + c.w * w2);
-                }
-                const float2 p = P[j * NV + i];
-                work[cc * SZ + bitrev<LOG2>(j)] = make_float2(F * p.x, F * p.y);
-            }
-            __syncthreads();
-            fft_dit_r4<SZ, true>(work, nc, tw);
-            for (int idx = tid; idx < nrb * nc; idx += nthr) {
-                const int q = idx / nc;
-                const int cc = idx - q * nc;
-                const int yr = rb0 + q;
-                float2 a = make_float2(0.f, 0.f);
+        // ---- phase A: columns of the half spectrum
+        for (int i0 = wave * PER; i0 < NV; i0 += nw * PER) {
+            float2 x[CW][R];
 #pragma unroll
-                for (int t = 0; t < 7; ++t) {
-                    const int ys = yr - 3 + t;
-                    if (ys >= 0 && ys < SZ) {
-                        const float2 h = work[cc * SZ + ys];
-                        a.x += fy[t] * h.x;
-                        a.y += fy[t] * h.y;
+            for (int c = 0; c < CW; ++c) {
+                const int i = i0 + c * NSEQ + sq;
+                const int ic = i < NV ? i : NV - 1;
+                const float v = (float)ic * (1.0f / SZ);
+                const float v2 = v * v;
+                const float2 *__restrict__ Pc = P + (size_t)ic * SZ;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int j = sl + SEQL * r;
+                    const float w = (float)(j < SZ / 2 ? j : j - SZ) * (1.0f / SZ);
+                    const float vw = v * w, w2 = w * w;
+                    float F = 0.f;
+                    for (int k = 0; k < K; ++k) {
+                        const float4 cp = S.comp[k];
+                        F += cp.x * __builtin_amdgcn_exp2f(cp.y * v2 + cp.z * vw + cp.w * w2);
                     }
+                    const float2 p = Pc[j];
+                    x[c][r] = make_float2(F * p.x, F * p.y);
                 }
-                hbuf[q * NV + c0 + cc] = a;
             }
-            __syncthreads();
-        }
-        // ---- phase B: two real rows per complex IFFT, x-Lanczos, epilogue
-        const int npairs = (nrb + 1) / 2;
-        for (int p0 = 0; p0 < npairs; p0 += ch) {
-            const int npp = min(ch, npairs - p0);
-            for (int idx = tid; idx < npp * SZ; idx += nthr) {
-                const int pp = idx >> LOG2;
-                const int k = idx & (SZ - 1);
-                const int qa = 2 * (p0 + pp);
-                const bool hasb = (qa + 1) < nrb;
-                const float2 *A = hbuf + qa * NV;
-                const float2 *B = A + NV;
-                float2 Z;
-                if (k == 0 || k == SZ / 2) {
-                    // irfft ignores the imaginary parts of DC and Nyquist
-                    Z = make_float2(A[k].x, hasb ? B[k].x : 0.f);
-                } else if (k < SZ / 2) {
-                    const float2 a = A[k];
-                    const float2 b = hasb ? B[k] : make_float2(0.f, 0.f);
-                    Z = make_float2(a.x - b.y, a.y + b.x);
-                } else {
-                    const float2 a = A[SZ - k];
-                    const float2 b = hasb ? B[SZ - k] : make_float2(0.f, 0.f);
-                    Z = make_float2(a.x + b.y, b.x - a.y);
-                }
-                work[pp * SZ + bitrev<LOG2>(k)] = Z;
-            }
-            __syncthreads();
-            fft_dit_r4<SZ, true>(work, npp, tw);
-            const int rowlen = 2 * mw;
-            for (int idx = tid; idx < npp * rowlen; idx += nthr) {
-                const int pp = idx / rowlen;
-                const int rem = idx - pp * rowlen;
-                const int which = rem >= mw ? 1 : 0;
-                const int ox = rem - which * mw;
-                const int q = 2 * (p0 + pp) + which;
-                if (q >= nrb) continue;
-                const int oy = rb0 + q + sy;
-                const int xs = ox - sx;
-                float val = 0.f;
-                if (xs >= 0 && xs < SZ) {
-                    const float2 *row = work + pp * SZ;
+            wave_idif<SZ, CW>(x, T, sl);
 #pragma unroll
-                    for (int u = 0; u < 7; ++u) {
-                        const int xx = xs - 3 + u;
-                        if (xx >= 0 && xx < SZ) {
-                            const float2 g = row[xx];
-                            val += fx[u] * (which ? g.y : g.x);
+            for (int c = 0; c < CW; ++c)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    lbw[(c * NSEQ + sq) * G::LBUF + lpad(bitrev<LOG2>(sl + SEQL * r))] = x[c][r];
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const int i = i0 + c * NSEQ + sq;
+                if (i >= NV) continue;
+                const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
+                for (int q = sl; q < nrb; q += SEQL) {
+                    const int yr = rb0 + q;
+                    float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int t = 0; t < 7; ++t) {
+                        const int ys = yr - 3 + t;
+                        if (ys >= 0 && ys < SZ) {
+                            const float2 h = lb[lpad(ys)];
+                            a.x += fy[t] * h.x;
+                            a.y += fy[t] * h.y;
                         }
                     }
-                    val *= inv_norm;
+                    hbuf[q * NV + i] = a;
                 }
-                acc += epilogue(mode, val, scale, off + (int64_t)oy * stride + ox, out, data, inverr);
             }
-            __syncthreads();
+            wave_sync();
         }
+        __syncthreads();
+        // ---- phase B: two real rows per complex inverse FFT
+        const int npairs = (nrb + 1) / 2;
+        for (int p0 = wave * PER; p0 < npairs; p0 += nw * PER) {
+            float2 x[CW][R];
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const int pp = p0 + c * NSEQ + sq;
+                const bool act = pp < npairs;
+                const int qa = 2 * (act ? pp : 0);
+                const bool hasb = act && (qa + 1) < nrb;
+                const float2 *A = hbuf + qa * NV;
+                const float2 *B = A + NV;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int k = sl + SEQL * r;
+                    float2 Z;
+                    if (!act) {
+                        Z = make_float2(0.f, 0.f);
+                    } else if (k == 0 || k == SZ / 2) {
+                        // irfft ignores the imaginary parts of DC and Nyquist
+                        Z = make_float2(A[k].x, hasb ? B[k].x : 0.f);
+                    } else if (k < SZ / 2) {
+                        const float2 a = A[k];
+                        const float2 b = hasb ? B[k] : make_float2(0.f, 0.f);
+                        Z = make_float2(a.x - b.y, a.y + b.x);
+                    } else {
+                        const float2 a = A[SZ - k];
+                        const float2 b = hasb ? B[SZ - k] : make_float2(0.f, 0.f);
+                        Z = make_float2(a.x + b.y, b.x - a.y);
+                    }
+                    x[c][r] = Z;
+                }
+            }
+            wave_idif<SZ, CW>(x, T, sl);
+#pragma unroll
+            for (int c = 0; c < CW; ++c)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    lbw[(c * NSEQ + sq) * G::LBUF + lpad(bitrev<LOG2>(sl + SEQL * r))] = x[c][r];
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const int pp = p0 + c * NSEQ + sq;
+                if (pp >= npairs) continue;
+                const int qa = 2 * pp;
+                const bool hasb = (qa + 1) < nrb;
+                const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
+                const int oya = rb0 + qa + sy;
+                for (int ox = sl; ox < mw; ox += SEQL) {
+                    const int xs = ox - sx;
+                    float va = 0.f, vb = 0.f;
+                    if (xs >= 0 && xs < SZ) {
+#pragma unroll
+                        for (int u = 0; u < 7; ++u) {
+                            const int xx = xs - 3 + u;
+                            if (xx >= 0 && xx < SZ) {
+                                const float2 g = lb[lpad(xx)];
+                                va += fx[u] * g.x;
+                                vb += fx[u] * g.y;
+                            }
+                        }
+                        va *= inv_norm;
+                        vb *= inv_norm;
+                    }
+                    acc += epilogue(mode, va, scale, off + (int64_t)oya * stride + ox, out, data,
+                                    inverr);
+                    if (hasb)
+                        acc += epilogue(mode, vb, scale, off + (int64_t)(oya + 1) * stride + ox,
+                                        out, data, inverr);
+                }
+            }
+            wave_sync();
+        }
+        __syncthreads();
     }
     if (mode == TRH_CHI2) {
         const double tot = block_sum(acc, S.red);
@@ -273,25 +440,26 @@ static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 
 // Grid-stride render kernel over a job list.  `count` (device, may be NULL:
 // then njobs) bounds the list; `list` (may be NULL: identity) maps list
-// entries to job indices.
+// entries to job indices.  blockDim.x is a multiple of 64; dynamic LDS holds
+// fft_lds_bytes(SZ, blockDim.x / 64, maxrows).
 template <int SZ, int MODE>
 __global__ void __launch_bounds__(1024)
 k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
              const int *__restrict__ count, int njobs, const float2 *__restrict__ psf_spec,
-             const float2 *__restrict__ twg, int ch, int maxrows, float *__restrict__ out,
+             const float2 *__restrict__ twg, int maxrows, float *__restrict__ out,
              const float *__restrict__ data, const float *__restrict__ inverr,
              double *__restrict__ chi2) {
     extern __shared__ float2 smem[];
     __shared__ RenderScratch S;
-    float2 *tw = smem;
-    float2 *work = tw + SZ / 2;
-    float2 *hbuf = work + ch * SZ;
-    for (int i = threadIdx.x; i < SZ / 2; i += blockDim.x) tw[i] = twg[i];
-    __syncthreads();
+    using G = WaveGeom<SZ>;
+    float2 *lbuf = smem;
+    float2 *hbuf = lbuf + (size_t)(blockDim.x >> 6) * G::NSEQ * CW * G::LBUF;
+    WaveTw<SZ> T;
+    T.load(twg, (threadIdx.x & 63) & (G::SEQL - 1));
     const int n = count ? *count : njobs;
     for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
         const int jid = list ? list[jj] : jj;
-        render_fft_one<SZ, MODE>(jobs + jid, psf_spec, tw, work, hbuf, ch, maxrows, S, out, data,
+        render_fft_one<SZ, MODE>(jobs + jid, psf_spec, T, lbuf, hbuf, maxrows, S, out, data,
                                  inverr, chi2);
     }
 }

@@ -171,8 +171,8 @@ k_psf_rows(const float *__restrict__ stamps, int ph, int pw, float2 *__restrict_
         const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
         const float2 B = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
         const int ya = 2 * (pair0 + pp);
-        out[((size_t)n * SZ + ya) * NV + k] = A;
-        out[((size_t)n * SZ + ya + 1) * NV + k] = B;
+        out[((size_t)n * NV + k) * SZ + ya] = A;
+        out[((size_t)n * NV + k) * SZ + ya + 1] = B;
     }
 }
 
@@ -190,16 +190,16 @@ k_psf_cols(float2 *__restrict__ spec, const float2 *__restrict__ twg, int cc) {
     const int nc = min(cc, NV - c0);
     float2 *S = spec + (size_t)n * SZ * NV;
     for (int idx = threadIdx.x; idx < nc * SZ; idx += blockDim.x) {
-        const int j = idx / nc;
-        const int c = idx - j * nc;
-        work[c * SZ + bitrev<LOG2>(j)] = S[j * NV + c0 + c];
+        const int c = idx >> LOG2;
+        const int j = idx & (SZ - 1);
+        work[c * SZ + bitrev<LOG2>(j)] = S[(size_t)(c0 + c) * SZ + j];
     }
     __syncthreads();
     fft_dit<SZ, false>(work, nc, tw);
     for (int idx = threadIdx.x; idx < nc * SZ; idx += blockDim.x) {
-        const int j = idx / nc;
-        const int c = idx - j * nc;
-        S[j * NV + c0 + c] = work[c * SZ + j];
+        const int c = idx >> LOG2;
+        const int j = idx & (SZ - 1);
+        S[(size_t)(c0 + c) * SZ + j] = work[c * SZ + j];
     }
 }
 
@@ -254,25 +254,31 @@ static int sz_index(int sz) {
     return -1;
 }
 
-// Default launch shape per FFT size: threads per block and columns per
-// phase-A chunk (all columns where LDS allows).  trh_set_fft_config
-// overrides them (the autotuner's knob).
-// from a sweep over (threads, ch) on MI355X with the config-2 workload
-static const int kDefThreads[5] = {256, 256, 768, 1024, 1024};
-static const int kDefCh[5] = {17, 17, 16, 8, 8};
+// Default launch shape per FFT size: threads (waves x 64) per block; the
+// second knob caps the G rows kept per block (0: as many as LDS allows).
+// trh_set_fft_config overrides them (the autotuner's knob).
+// (from a sweep over threads x rows on MI355X; sz 128 keeps 120 rows so two
+// blocks share a CU's 160 KB of LDS)
+static const int kDefThreads[5] = {256, 128, 512, 1024, 1024};
+static const int kDefRows[5] = {0, 0, 120, 64, 0};
 
-void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *ch, size_t *lds,
+void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *rows, size_t *lds,
                       int *grid, int njobs) {
     const int si = sz_index(sz);
-    int c = h->fft_ch[si] > 0 ? h->fft_ch[si] : kDefCh[si];
     int nt = h->fft_threads[si] > 0 ? h->fft_threads[si] : kDefThreads[si];
-    if (c > sz / 2 + 1) c = sz / 2 + 1;
-    size_t b = fft_lds_bytes(sz, c, maxrows);
-    while (b + sizeof(RenderScratch) > (size_t)h->lds_max && c > 1) {
-        c /= 2;
-        b = fft_lds_bytes(sz, c, maxrows);
+    int r = h->fft_ch[si] > 0 ? h->fft_ch[si] : (kDefRows[si] > 0 ? kDefRows[si] : sz);
+    if (maxrows > 0 && r > maxrows) r = maxrows;
+    if (r > sz) r = sz;
+    size_t b = fft_lds_bytes(sz, nt / 64, r);
+    // fit LDS: fewer rows first (down to 32), then fewer waves, then rows to 8
+    while (b + sizeof(RenderScratch) > (size_t)h->lds_max) {
+        if (r > 32) r -= 8;
+        else if (nt > 128) nt /= 2;
+        else if (r > 8) r -= 8;
+        else break;
+        b = fft_lds_bytes(sz, nt / 64, r);
     }
-    *ch = c;
+    *rows = r;
     *lds = b;
     *threads = nt;
     const size_t per_cu = 160 * 1024;
@@ -286,33 +292,35 @@ void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *ch,
 }
 
 int max_rows_for(int sz, int lds_max) {
-    // rows of the y-Lanczos output kept in LDS per pass
-    int mr = sz;
-    while (mr > 8 && fft_lds_bytes(sz, 8, mr) + sizeof(RenderScratch) > (size_t)lds_max) mr /= 2;
-    return mr;
+    // upper bound on the G rows per block; fft_launch_shape fits it to LDS
+    (void)lds_max;
+    return sz;
 }
 
 template <int SZ>
 int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list,
                const int *count, int njobs_bound, int maxrows, const float *psf_spec, float *out,
                const float *data, const float *inverr, double *chi2, hipStream_t st) {
-    int nt, ch, grid;
+    int nt, rows, grid;
     size_t lds;
-    fft_launch_shape(h, SZ, maxrows, &nt, &ch, &lds, &grid, njobs_bound);
+    fft_launch_shape(h, SZ, maxrows, &nt, &rows, &lds, &grid, njobs_bound);
+    if (lds + sizeof(RenderScratch) > (size_t)h->lds_max)
+        return set_err(TRH_E_UNSUPPORTED, "k_render_fft<%d>: LDS %zu exceeds %d", SZ, lds,
+                       h->lds_max);
     const float2 *tw = h->tw + tw_offset(SZ);
     const float2 *ps = (const float2 *)psf_spec;
     switch (mode) {
     case TRH_STORE:
         hipLaunchKernelGGL((k_render_fft<SZ, TRH_STORE>), dim3(grid), dim3(nt), lds, st, jobs, list,
-                           count, njobs_bound, ps, tw, ch, maxrows, out, data, inverr, chi2);
+                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2);
         break;
     case TRH_CHI2:
         hipLaunchKernelGGL((k_render_fft<SZ, TRH_CHI2>), dim3(grid), dim3(nt), lds, st, jobs, list,
-                           count, njobs_bound, ps, tw, ch, maxrows, out, data, inverr, chi2);
+                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2);
         break;
     default:
         hipLaunchKernelGGL((k_render_fft<SZ, -1>), dim3(grid), dim3(nt), lds, st, jobs, list, count,
-                           njobs_bound, ps, tw, ch, maxrows, out, data, inverr, chi2);
+                           njobs_bound, ps, tw, rows, out, data, inverr, chi2);
         break;
     }
     return hip_check(hipGetLastError(), "k_render_fft");
@@ -343,15 +351,15 @@ int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, con
 }
 }  // namespace trh
 
-extern "C" int trh_set_fft_config(trh_handle *h, int sz, int threads, int ch) {
+extern "C" int trh_set_fft_config(trh_handle *h, int sz, int threads, int rows) {
     if (!h) return set_err(TRH_E_ARG, "trh_set_fft_config: null handle");
     const int si = sz_index(sz);
     if (si < 0) return set_err(TRH_E_ARG, "trh_set_fft_config: sz %d not in {32..512}", sz);
     if (threads != 0 && (threads % 64 != 0 || threads < 64 || threads > 1024))
         return set_err(TRH_E_ARG, "trh_set_fft_config: threads must be a multiple of 64 in [64, 1024]");
-    if (ch < 0) return set_err(TRH_E_ARG, "trh_set_fft_config: ch must be >= 0");
+    if (rows < 0) return set_err(TRH_E_ARG, "trh_set_fft_config: rows must be >= 0");
     h->fft_threads[si] = threads;
-    h->fft_ch[si] = ch;
+    h->fft_ch[si] = rows;
     return TRH_OK;
 }
 

@@ -130,7 +130,7 @@ class PsfSpectra(object):
             if sz not in SUPPORTED_FFT_SIZES:
                 raise ValueError('FFT size %d outside %s' % (sz, SUPPORTED_FFT_SIZES))
             ph, pw = self.shape
-            out = torch.empty((sz, sz // 2 + 1, 2), dtype=torch.float32, device=self.device)
+            out = torch.empty((sz // 2 + 1, sz, 2), dtype=torch.float32, device=self.device)
             lib = _lib.load()
             h = _lib.handle(torch.device(self.device).index or 0)
             import ctypes
