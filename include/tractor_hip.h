@@ -197,8 +197,8 @@ int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n, i
  * with a modelMask, tractor/psf.py:137-190; one SWIG lanczos_shift_3f call per
  * source in the reference).  Job fields as trh_fft_job with: (sx, sy) = window
  * coordinates of the stamp's pixel (0,0); (mux, muy) = the sub-pixel shift
- * px - round(px); psf, K, amp, var unused.  stamp: float32 [ph][pw];
- * maxw >= every job's mw.
+ * px - round(px); psf selects the stamp at stamp + psf * ph * pw; K, amp,
+ * var unused.  stamp: float32 [nstamp][ph][pw]; maxw >= every job's mw.
  */
 int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, int maxw,
                            const float *stamp, int ph, int pw, float *out, const float *data,
@@ -224,6 +224,50 @@ int trh_normal_eq_dense(trh_handle *h, const float *Ab, int nrows, int ncols, do
  */
 int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, int scale_columns,
                          double *x, int *info, void *stream);
+
+/*
+ * Batched forced photometry (Optimizer.forced_photometry,
+ * tractor/optimize.py:16-145, with LsqrOptimizer's unscaled least-squares
+ * step, tractor/lsqr_optimizer.py:11-251, on one image of many overlapping
+ * sources).  Unit-flux patches u_s live in one float32 buffer: patch s is
+ * win[s] = (x0, y0, w, h) (inside the image, row-major) at poff[s].
+ *   psf_combine:   out[s] = sum_q weights[s][q] * basis[idx[s][q]], q < 4,
+ *                  arrays of `len` floats (bilinear PSF interpolation of
+ *                  stamps or of spectra; replaces PixelizedPsfEx's
+ *                  per-position stamp, tractor/psfex.py:481-520); n <= 65535.
+ *   fp_pair_dots:  A_st = sum u_s u_t inverr^2 over each overlapping pair
+ *                  (s, t) = pairs[p] (FP32 products, FP64 sum), written to
+ *                  the CSR values at pos[p][0] and pos[p][1].
+ *   fp_rhs:        b_s = sum u_s (data - mod0) inverr^2 (mod0 may be NULL).
+ *   fp_block_chol: FP64 Cholesky of A's diagonal blocks, one per tile of
+ *                  consecutive rows [tile_start[T], tile_start[T+1]), dense
+ *                  at blocks + block_off[T]; info[T] = k if pivot k fails.
+ *   fp_precond:    z = blockdiag(L L^T)^-1 r (max_n >= every tile size).
+ *   fp_spmv:       y = A x (CSR, FP64).
+ *   fp_add_patches: img += sum_s flux[s] u_s (model image; Tractor.
+ *                  getModelImage, tractor/engine.py:424-445, for these
+ *                  patches).
+ * (No reference counterpart for fp_block_chol, fp_precond and fp_spmv: the
+ * reference calls scipy's LSQR, lsqr_optimizer.py:586.)
+ */
+int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
+                    const float *weights, const int32_t *idx, float *out, int n, void *stream);
+int trh_fp_pair_dots(trh_handle *h, const int32_t *pairs, const int64_t *pos, int npairs,
+                     const int32_t *win, const int64_t *poff, const float *patches,
+                     const float *inverr, int img_w, double *vals, void *stream);
+int trh_fp_rhs(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
+               const float *patches, const float *data, const float *mod0, const float *inverr,
+               int img_w, double *b, void *stream);
+int trh_fp_block_chol(trh_handle *h, const int64_t *row_ptr, const int32_t *col,
+                      const double *vals, const int32_t *tile_start, const int64_t *block_off,
+                      int ntile, double *blocks, int32_t *info, void *stream);
+int trh_fp_precond(trh_handle *h, const int32_t *tile_start, const int64_t *block_off, int ntile,
+                   int max_n, const double *blocks, const double *r, double *z, void *stream);
+int trh_fp_spmv(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
+                const double *vals, const double *x, double *y, void *stream);
+int trh_fp_add_patches(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
+                       const float *patches, const float *flux, int img_w, float *img,
+                       void *stream);
 
 /*
  * Surrogate CNN (no reference counterpart: the learned initialiser that

@@ -20,7 +20,7 @@ __all__ = [
     'mog_fourier_zero_mean', 'fft_size', 'pad_psf', 'psf_spectrum',
     'lanczos_taps3', 'correlate7f', 'lanczos_shift_image',
     'render_galaxy_fft', 'galaxy_patch_halfsize', 'mog_grid', 'mog_masked',
-    'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch',
+    'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch', 'forced_unit_patch', 'forced_phot_lsq',
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
@@ -989,3 +989,72 @@ def fit_source(prob, params0, dchisq=0., steps=100, dchisq_limited=1e-6, alphas=
         if step_limited and dlnp <= dchisq_limited:
             break
     return params, it + 1, source_chisq(prob, params)
+
+
+# ---------------------------------------------------------------------------
+# Forced photometry (optimize.py:16-222 with LsqrOptimizer's unscaled step,
+# lsqr_optimizer.py:11-251) of many sources with per-source pixelized PSFs.
+# ---------------------------------------------------------------------------
+def forced_unit_patch(kind, px, py, shape, stamp, img_shape):
+    """Unit-flux patch of one source without a modelMask, clipped to the
+    image: point sources via PixelizedPSF.getPointSourcePatch (psf.py:137-207,
+    the Lanczos-shifted stamp), exp (kind 1) / dev (kind 2) galaxies with
+    EllipseESoft shape (log r_e, ee1, ee2) via _getUnitFluxPatchSize
+    (galaxy.py:678-691) and the FFT path (galaxy.py:336-508).  Returns
+    (x0, y0, patch float64) or None."""
+    H, W = img_shape
+    stamp = np.asarray(stamp, np.float32)
+    ph, pw = stamp.shape
+    if kind == 0:
+        x0, y0, p = pixpsf_point_patch(stamp, px, py)
+    else:
+        amp, var = exp_mixture() if kind == 1 else dev_mixture()
+        var = shear_mixture(var, np.eye(2) * 3600., ellipse_esoft_basis(*shape))
+        nre = 4. if kind == 1 else 8.
+        hs = galaxy_patch_halfsize(nre * math.exp(shape[0]), px, py, img_shape,
+                                   math.hypot(ph / 2., pw / 2.))
+        outx, _ = get_overlapping_region(int(np.floor(px - hs)), int(np.ceil(px + hs + 1)), 0, W - 1)
+        outy, _ = get_overlapping_region(int(np.floor(py - hs)), int(np.ceil(py + hs + 1)), 0, H - 1)
+        if outx == [] or outy == []:
+            return None
+        x0, y0, p = render_galaxy_fft(amp, var, px, py, stamp,
+                                      nomask_extent=(outx.start, outx.stop, outy.start,
+                                                     outy.stop, hs))
+    p = np.asarray(p, np.float64)
+    h, w = p.shape
+    X0, Y0 = max(x0, 0), max(y0, 0)
+    X1, Y1 = min(x0 + w, W), min(y0 + h, H)
+    if X1 <= X0 or Y1 <= Y0:
+        return None
+    return X0, Y0, p[Y0 - y0:Y1 - y0, X0 - x0:X1 - x0]
+
+
+def forced_phot_lsq(patches, data, inverr):
+    """Least-squares fluxes min || (data - sum_s f_s u_s) inverr || over the
+    unit-flux patches (x0, y0, patch) (None: flux 0), solved exactly in FP64
+    through the sparse normal equations; returns (flux, iv) with iv =
+    sum (u inverr)^2 (Optimizer._get_iv, optimize.py:378-423)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    H, W = data.shape
+    rows, cols, vals = [], [], []
+    for s, pt in enumerate(patches):
+        if pt is None:
+            continue
+        x0, y0, p = pt
+        h, w = p.shape
+        yy, xx = np.mgrid[y0:y0 + h, x0:x0 + w]
+        rows.append((yy * W + xx).ravel())
+        cols.append(np.full(h * w, s))
+        vals.append((p * inverr[y0:y0 + h, x0:x0 + w]).ravel())
+    n = len(patches)
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(H * W, n))
+    b = (np.asarray(data, np.float64) * inverr).ravel()
+    N = (A.T @ A).tocsc()
+    iv = N.diagonal().copy()
+    empty = iv == 0
+    N = N + sp.diags(empty.astype(np.float64))
+    f = spl.spsolve(N, A.T @ b)
+    f[empty] = 0.
+    return f, iv

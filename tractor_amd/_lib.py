@@ -64,6 +64,13 @@ SIGNATURES = {
     'trh_render_mog_approx3_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     'trh_render_point_batch': (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
+    'trh_psf_combine': (_I, [_P, _P, _I, ctypes.c_int64, _P, _P, _P, _I, _P]),
+    'trh_fp_pair_dots': (_I, [_P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _P]),
+    'trh_fp_rhs': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    'trh_fp_block_chol': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
+    'trh_fp_precond': (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P]),
+    'trh_fp_spmv': (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    'trh_fp_add_patches': (_I, [_P, _I, _P, _P, _P, _P, _I, _P, _P]),
     'trh_cnn_conv_fwd': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
     'trh_cnn_conv_bwd_data': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
     'trh_cnn_conv_bwd_weight': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),

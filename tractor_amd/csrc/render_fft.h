@@ -479,7 +479,9 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
     for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
         const int jid = list ? list[jj] : jj;
         if (jobs[jid].mw > maxw) continue;  // host guarantees maxw >= every mw
-        render_point_one(jobs + jid, stamp, ph, pw, lstamp, tmp, S, out, data, inverr, chi2);
+        // the job's psf field selects one of the stamps at stamp + psf * ph * pw
+        render_point_one(jobs + jid, stamp + (size_t)jobs[jid].psf * ph * pw, ph, pw, lstamp, tmp,
+                         S, out, data, inverr, chi2);
     }
 }
 

@@ -103,3 +103,60 @@ def patch_offsets(P):
     off = np.zeros(len(P), np.int64)
     off[1:] = np.cumsum(P.astype(np.int64) ** 2)[:-1]
     return off, int(np.sum(P.astype(np.int64) ** 2))
+
+
+def psf_grid_stamps(size=33, fwhm_lo=3.0, fwhm_hi=5.0, ngrid=3):
+    """ngrid x ngrid grid of PSF stamps for a spatially varying PSF: two
+    concentric Gaussians (sigma, 2 sigma; weights 0.85, 0.15), FWHM rising
+    linearly from fwhm_lo at node (0, 0) to fwhm_hi at the far corner.
+    Returns (ngrid*ngrid, size, size) float32, node (gx, gy) at gy*ngrid+gx."""
+    out = []
+    for gy in range(ngrid):
+        for gx in range(ngrid):
+            f = fwhm_lo + (fwhm_hi - fwhm_lo) * (gx + gy) / (2. * (ngrid - 1))
+            s = f / (2. * math.sqrt(2. * math.log(2.)))
+            out.append(gaussian_mixture_stamp(size, [s, 2. * s], [0.85, 0.15]))
+    return np.array(out, np.float32)
+
+
+def bilinear_psf_weights(x, y, W, H, ngrid=3):
+    """Bilinear interpolation weights of the PSF grid (nodes at
+    i (W-1)/(ngrid-1), j (H-1)/(ngrid-1)) at pixel positions (x, y):
+    returns (idx (n, 4) int32, w (n, 4) float32)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    u = x / ((W - 1.) / (ngrid - 1))
+    v = y / ((H - 1.) / (ngrid - 1))
+    i0 = np.clip(np.floor(u).astype(int), 0, ngrid - 2)
+    j0 = np.clip(np.floor(v).astype(int), 0, ngrid - 2)
+    tx = np.clip(u - i0, 0., 1.)
+    ty = np.clip(v - j0, 0., 1.)
+    idx = np.stack([j0 * ngrid + i0, j0 * ngrid + i0 + 1, (j0 + 1) * ngrid + i0,
+                    (j0 + 1) * ngrid + i0 + 1], 1).astype(np.int32)
+    w = np.stack([(1 - tx) * (1 - ty), tx * (1 - ty), (1 - tx) * ty, tx * ty], 1).astype(np.float32)
+    return idx, w
+
+
+def draw_forced_scene(n=20000, size=2048, frac_point=0.6, rng=None):
+    """Forced-photometry scene: n sources at Poisson (uniform) positions on a
+    size x size image; 60% point sources, the rest exp / dev galaxies (4:3)
+    with fixed shapes drawn as the batched benchmark (r_e log-uniform
+    0.5..10 px, |e| ~ U(0, 0.6), angle ~ U(0, pi)); fluxes log-uniform
+    10..1e4.  Returns kind (n,), pos (n, 2), flux (n,), shape (n, 3) =
+    EllipseESoft (log r_e, ee1, ee2)."""
+    if rng is None:
+        rng = np.random.default_rng(seed(3, 0))
+    npt = int(round(frac_point * n))
+    ngal = n - npt
+    nexp = int(round(ngal * 4. / 7.))
+    kind = np.array([KIND_POINT] * npt + [KIND_EXP] * nexp + [KIND_DEV] * (ngal - nexp), np.int32)
+    rng.shuffle(kind)
+    pos = rng.uniform(0., size, (n, 2))
+    flux = np.exp(rng.uniform(np.log(10.), np.log(1e4), n))
+    re = np.exp(rng.uniform(np.log(0.5), np.log(10.), n))
+    e = rng.uniform(0., 0.6, n)
+    th = rng.uniform(0., np.pi, n)
+    ee = -np.log(1. - e)
+    shape = np.stack([np.log(re), ee * np.cos(2 * th), ee * np.sin(2 * th)], 1)
+    shape[kind == KIND_POINT] = 0.
+    return kind, pos, flux, shape
