@@ -64,11 +64,11 @@ def build_workload(name, torch):
                 inverr=inverr, pix_off=pix_off, start=start)
 
 
-def run_step(wl, torch, dchisq=0.):
+def run_step(wl, torch, dchisq=0., graph=True):
     f = wl['fitter']
     if wl['cfg'] == 4:
         f.set_params(wl['start'])
-        return f.fit(max_iter=100, dchisq=dchisq)
+        return f.fit(max_iter=100, dchisq=dchisq, graph=graph)
     f.set_params(wl['start'])
     f.iterate(0., 0.)
     return 1
@@ -143,7 +143,7 @@ def main():
     # one more step, outside the timed region, with per-kernel HIP-event
     # timing (it synchronises every iteration) for the roofline figures
     f.profile(True)
-    run_step(wl, torch, args.dchisq)
+    run_step(wl, torch, args.dchisq, graph=False)
     prof = f.profile_read()
     f.profile(False)
     value = wl['n'] / dt

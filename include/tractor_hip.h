@@ -356,6 +356,17 @@ int trh_fitter_get_params(trh_fitter *f, double *params, void *stream);
 int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream);
 
 /*
+ * `n_iter` LM iterations replayed from a hipGraph that captures one
+ * iteration's launches (captured on first use, and again when lambda or
+ * dchisq change), on a library-owned stream ordered after and before
+ * `stream`.  Same results as calling trh_fitter_iterate n_iter times; not
+ * available while per-launch profiling is on.  No reference counterpart
+ * (launch scheduling).
+ */
+int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int n_iter,
+                             void *stream);
+
+/*
  * Render every source's model (counts x unit-flux patch at the current
  * parameters, as getModelPatch, tractor/pointsource.py:36-60) into `out`,
  * laid out like `data` (patch s at pix_off[s], P*P).  Enqueued on `stream`.

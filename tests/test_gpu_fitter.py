@@ -134,3 +134,38 @@ def test_full_fit_vs_oracle(problem):
                 'sig', sig, 'chi', st['chisq'][s], chi, 'iters', st['iterations'][s], nref)
     say('sources within 1e-4 rel or 0.01 sigma: %d / %d' % (tight, S))
     assert nbad == 0
+
+
+def test_graph_replay_matches_eager(problem):
+    """Iterations replayed from the captured hipGraph give bit-identical
+    parameters, chi^2 and iteration counts to eager launches, across several
+    replays with eager work and synchronisation between them, and after a
+    change of lambda (which recaptures)."""
+    say('graph replay vs eager')
+    f = problem['fitter']
+    start = problem['start']
+    f.set_params(start)
+    for _ in range(12):
+        f.iterate(0., 0.)
+    for _ in range(3):
+        f.iterate(1e-3, 0.)
+    eager_p = f.get_params()
+    eager_s = f.status()
+    f.set_params(start)
+    for _ in range(3):
+        f.iterate_graph(4, 0., 0.)
+        torch.cuda.synchronize()
+        f.status()          # eager work between replays
+    f.iterate_graph(3, 1e-3, 0.)
+    graph_p = f.get_params()
+    graph_s = f.status()
+    assert np.array_equal(eager_p, graph_p)
+    assert np.array_equal(eager_s['chisq'], graph_s['chisq'])
+    assert np.array_equal(eager_s['iterations'], graph_s['iterations'])
+    # fit() with graph replays reaches the same state as eager fit()
+    f.set_params(start)
+    f.fit(max_iter=100, graph=False)
+    a = f.get_params()
+    f.set_params(start)
+    f.fit(max_iter=100, graph=True)
+    assert np.array_equal(a, f.get_params())

@@ -79,6 +79,11 @@ class BatchFitter(object):
         check(self.lib.trh_fitter_iterate(self.f, float(lam), float(dchisq), self.stream),
               'iterate')
 
+    def iterate_graph(self, n, lam=0., dchisq=0.):
+        """n iterations replayed from a captured hipGraph of one iteration."""
+        check(self.lib.trh_fitter_iterate_graph(self.f, float(lam), float(dchisq), int(n),
+                                                self.stream), 'iterate_graph')
+
     def status(self):
         active = np.zeros(self.S, np.int32)
         iters = np.zeros(self.S, np.int32)
@@ -108,13 +113,23 @@ class BatchFitter(object):
         check(self.lib.trh_fitter_profile_read(self.f, ptr(out)), 'profile_read')
         return out
 
-    def fit(self, max_iter=100, dchisq=0., lam=0., check_every=1):
-        """Iterate until every source meets the optimize_loop stopping rule."""
+    def fit(self, max_iter=100, dchisq=0., lam=0., check_every=None, graph=True):
+        """Iterate until every source meets the optimize_loop stopping rule
+        (sources that have stopped are skipped by every kernel, so extra
+        iterations between checks change nothing).  graph=True replays a
+        captured hipGraph of one iteration, check_every (default 4) at a
+        time; graph=False launches eagerly (default check every iteration)."""
+        if check_every is None:
+            check_every = 4 if graph else 1
         it = 0
         while it < max_iter:
-            for _ in range(check_every):
-                self.iterate(lam, dchisq)
-                it += 1
+            k = min(check_every, max_iter - it)
+            if graph:
+                self.iterate_graph(k, lam, dchisq)
+            else:
+                for _ in range(k):
+                    self.iterate(lam, dchisq)
+            it += k
             if self.n_active() == 0:
                 break
         return it

@@ -75,6 +75,12 @@ struct trh_fitter {
     double tot_ms[TRH_NSLOT];
     double tot_launch[TRH_NSLOT];
     double tot_extra_bytes[TRH_NSLOT];   // PSF spectrum reads, counted per launch
+    // one LM iteration captured as a hipGraph, replayed on gstream
+    hipStream_t gstream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t gev[2];
+    bool have_gev = false;
+    double g_lambda = 0.0, g_dchisq = 0.0;
     std::vector<void *> allocs;
 };
 
@@ -637,6 +643,10 @@ extern "C" int trh_fitter_destroy(trh_fitter *f) {
         for (int i = 0; i < TRH_NSLOT; ++i)
             for (int j = 0; j < 2; ++j) (void)hipEventDestroy(f->pev[i][j]);
     }
+    if (f->gexec) (void)hipGraphExecDestroy(f->gexec);
+    if (f->have_gev)
+        for (int i = 0; i < 2; ++i) (void)hipEventDestroy(f->gev[i]);
+    if (f->gstream) (void)hipStreamDestroy(f->gstream);
     delete f;
     return TRH_OK;
 }
@@ -798,15 +808,17 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
 #define SLOT_BEGIN(k) do { if (f->profiling) (void)hipEventRecord(f->pev[k][0], st); } while (0)
 #define SLOT_END(k) do { if (f->profiling) (void)hipEventRecord(f->pev[k][1], st); } while (0)
 
-extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream) {
-    if (!f) return set_err(TRH_E_ARG, "trh_fitter_iterate: null fitter");
-    hipStream_t st = (hipStream_t)stream;
+// The launches of one LM iteration on `st`.  `capture`: the stream is being
+// captured into a graph, so no events and no per-launch profiling.
+static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t st,
+                        bool capture) {
     const int S = f->S;
     const int cap_jac = 4 * S, cap_ls = NALPHA * S;
     const int tb = 256, nb = (S + tb - 1) / tb;
-    unsigned long long *acc = f->profiling ? f->acc : nullptr;
+    const bool prof = f->profiling && !capture;
+    unsigned long long *acc = prof ? f->acc : nullptr;
     int rc;
-    (void)hipEventRecord(f->ev[0], st);
+    if (!capture) (void)hipEventRecord(f->ev[0], st);
     SLOT_BEGIN(SL_PREP);
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
     hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->render_off,
@@ -827,7 +839,7 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
                            f->renders, nullptr, st)))
         return rc;
     SLOT_END(SL_JPT);
-    (void)hipEventRecord(f->ev[1], st);
+    if (!capture) (void)hipEventRecord(f->ev[1], st);
     SLOT_BEGIN(SL_NEQ);
     {
         int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
@@ -837,13 +849,13 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
         if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq"))) return rc;
     }
     SLOT_END(SL_NEQ);
-    (void)hipEventRecord(f->ev[2], st);
+    if (!capture) (void)hipEventRecord(f->ev[2], st);
     SLOT_BEGIN(SL_SOLVE);
     hipLaunchKernelGGL(k_fit_solve, dim3((S + 63) / 64), dim3(64), 0, st, S, f->kind, f->neq,
                        f->params, f->active, lambda, f->X, f->trial, f->nsteps, f->steplim);
     if ((rc = hip_check(hipGetLastError(), "k_fit_solve"))) return rc;
     SLOT_END(SL_SOLVE);
-    (void)hipEventRecord(f->ev[3], st);
+    if (!capture) (void)hipEventRecord(f->ev[3], st);
     hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->pix_off,
                        f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw, f->jobs_ls,
                        f->list_ls, cap_ls, f->cnt + LS_CNT, f->err, acc);
@@ -861,13 +873,13 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
                            NALPHA * f->n_pt, nullptr, f->chi2ls, st)))
         return rc;
     SLOT_END(SL_LPT);
-    (void)hipEventRecord(f->ev[4], st);
+    if (!capture) (void)hipEventRecord(f->ev[4], st);
     hipLaunchKernelGGL(k_fit_accept, dim3(nb), dim3(tb), 0, st, S, f->neq, f->trial, f->nsteps,
                        f->steplim, f->chi2ls, f->params, f->active, f->iters, f->chisq, f->dlnp,
                        dchisq, 100);
     if ((rc = hip_check(hipGetLastError(), "k_fit_accept"))) return rc;
-    (void)hipEventRecord(f->ev[5], st);
-    if (f->profiling) {
+    if (!capture) (void)hipEventRecord(f->ev[5], st);
+    if (prof) {
         // accumulate this iteration's per-slot times (synchronises the stream)
         if ((rc = hip_check(hipEventSynchronize(f->ev[5]), "profile sync"))) return rc;
         int32_t cnt[16];
@@ -889,6 +901,62 @@ extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, v
             }
         }
     }
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream) {
+    if (!f) return set_err(TRH_E_ARG, "trh_fitter_iterate: null fitter");
+    return iterate_body(f, lambda, dchisq, (hipStream_t)stream, false);
+}
+
+extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int n_iter,
+                                        void *stream) {
+    if (!f || n_iter < 0) return set_err(TRH_E_ARG, "trh_fitter_iterate_graph: bad arguments");
+    if (f->profiling)
+        return set_err(TRH_E_ARG, "trh_fitter_iterate_graph: per-launch profiling needs "
+                                  "trh_fitter_iterate");
+    int rc;
+    if (!f->gstream &&
+        (rc = hip_check(hipStreamCreateWithFlags(&f->gstream, hipStreamNonBlocking), "graph stream")))
+        return rc;
+    if (!f->have_gev) {
+        for (int i = 0; i < 2; ++i)
+            if ((rc = hip_check(hipEventCreateWithFlags(&f->gev[i], hipEventDisableTiming),
+                                "graph event")))
+                return rc;
+        f->have_gev = true;
+    }
+    if (!f->gexec || lambda != f->g_lambda || dchisq != f->g_dchisq) {
+        if (f->gexec) {
+            (void)hipGraphExecDestroy(f->gexec);
+            f->gexec = nullptr;
+        }
+        if ((rc = hip_check(hipStreamBeginCapture(f->gstream, hipStreamCaptureModeThreadLocal),
+                            "begin capture")))
+            return rc;
+        const int brc = iterate_body(f, lambda, dchisq, f->gstream, true);
+        hipGraph_t g = nullptr;
+        rc = hip_check(hipStreamEndCapture(f->gstream, &g), "end capture");
+        if (brc) {
+            if (g) (void)hipGraphDestroy(g);
+            return brc;
+        }
+        if (rc) return rc;
+        rc = hip_check(hipGraphInstantiate(&f->gexec, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        f->g_lambda = lambda;
+        f->g_dchisq = dchisq;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = hip_check(hipEventRecord(f->gev[0], st), "graph order in")) ||
+        (rc = hip_check(hipStreamWaitEvent(f->gstream, f->gev[0], 0), "graph wait in")))
+        return rc;
+    for (int i = 0; i < n_iter; ++i)
+        if ((rc = hip_check(hipGraphLaunch(f->gexec, f->gstream), "graph launch"))) return rc;
+    if ((rc = hip_check(hipEventRecord(f->gev[1], f->gstream), "graph order out")) ||
+        (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "graph wait out")))
+        return rc;
     return TRH_OK;
 }
 

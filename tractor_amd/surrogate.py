@@ -349,7 +349,7 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
     f.set_params(start_all)
     if profile is not None:
         f.profile(True)
-    it = f.fit(max_iter=max_iter, dchisq=dchisq)
+    it = f.fit(max_iter=max_iter, dchisq=dchisq, graph=profile is None)
     if profile is not None:
         profile['slots'] = f.profile_read()
         f.profile(False)
