@@ -110,6 +110,9 @@ def main():
     ap.add_argument('--cnn-train', type=int, default=1 << 20,
                     help='c5: number of rendered training cutouts')
     ap.add_argument('--cnn-epochs', type=int, default=4)
+    ap.add_argument('--hbm-copy-probe', action='store_true',
+                    help='only run three 256 MiB device-to-device copies (HBM counter '
+                         'calibration under rocprofv3 --pmc)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--dchisq', type=float, default=0.,
@@ -122,6 +125,14 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    if args.hbm_copy_probe:
+        a = torch.ones(64 << 20, dtype=torch.float32, device='cuda')
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_probe_bytes_per_copy': a.numel() * 4, 'copies': 3}), flush=True)
+        return
     if args.workload == 'c5':
         return main_c5(args, torch)
     if args.workload == 'c3':
@@ -235,6 +246,16 @@ def roofline_of(prof):
            'traffic': None, 'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
            'bytes_per_launch': round(nbytes / max(launches, 1)), 'jobs_per_launch': round(
                jobs / max(launches, 1), 1)}
+    tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_render_fft128_chi2.json')
+    if os.path.exists(tfile):
+        # measured HBM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated)
+        # per render of this kernel, scaled to this launch's mean job count
+        with open(tfile) as fh:
+            tr = json.load(fh)
+        if tr['kernel'] == SLOT_NAMES[k]:
+            per_job = (tr['read_bytes'] + tr['write_bytes']) / tr['jobs']
+            out['traffic'] = round(per_job * jobs / max(launches, 1))
+            out['traffic_source'] = 'rocprofv3 PMC, %s, %.0f B per render' % (tr['workload'], per_job)
     if k in SLOT_SZ:
         tf = jobs * render_flops(SLOT_SZ[k]) / (ms / 1e3) / 1e12
         out['valu'] = {'achieved': round(tf, 2), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',

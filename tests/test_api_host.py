@@ -128,3 +128,13 @@ def test_no_cpu_fallback_without_device():
     tr = T.Tractor([tim], [T.PointSource(T.PixPos(10., 10.), T.Flux(1.))])
     with pytest.raises(Exception):
         tr.getModelImage(0)
+
+
+def test_autotune_candidates_and_buckets():
+    """The autotuner's search space and its bucket rule (no device)."""
+    from tractor_amd import autotune as A
+    for sz in A.SIZES:
+        c = A.candidates(sz)
+        assert c and all(t % 64 == 0 and 64 <= t <= 1024 and r >= 0 for t, r in c)
+    # 63x63 PSF: P <= 56 -> 64, 64..120 -> 128, 128 -> 256 (the benchmark buckets)
+    assert [A.bucket_of_P(P, (63, 63)) for P in (16, 56, 64, 120, 128)] == [64, 64, 128, 128, 256]

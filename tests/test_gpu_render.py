@@ -172,3 +172,45 @@ def test_chi2_epilogue(dev):
     torch.cuda.synchronize()
     ref = float((((data - (G * np.float32(1000.)).astype(np.float32)) * ie).astype(np.float64) ** 2).sum())
     assert abs(chi2[1].item() - ref) / ref < 1e-5
+
+
+def test_autotuner_installs_shapes_and_keeps_results(dev):
+    """tune_fft times the candidate shapes and installs the fastest.  The
+    launch shape moves no per-pixel arithmetic: STORE renders are identical
+    bit for bit under every shape, and the FP64 chi^2 sums differ only by
+    summation order (1e-12 relative)."""
+    from tractor_amd import autotune as A, _lib
+    from tractor_amd._lib import TRH_STORE
+    say('autotune')
+    stamp = W.moffat_stamp(63, 4.0, 3.5)
+    sp = R.PsfSpectra(stamp, dev)
+    rng = np.random.default_rng(4)
+    J, npix = A._jobs(128, np.array([72, 96, 120]), 256, stamp.shape, rng)
+    Js = J.copy()
+    Js['mode'] = TRH_STORE
+    dj = R.jobs_to_device(J, dev)
+    djs = R.jobs_to_device(Js, dev)
+    data = torch.randn(npix, device=dev)
+    inv = torch.ones_like(data)
+    lib, h = _lib.load(), _lib.handle(0)
+    spec, _, _ = sp.get(128)
+
+    def render():
+        c = torch.zeros(256, dtype=torch.float64, device=dev)
+        out = torch.zeros(npix, device=dev)
+        _lib.check(lib.trh_render_fft_batch(h.value, 128, _lib.ptr(dj), 256, _lib.ptr(spec), None,
+                                            _lib.ptr(data), _lib.ptr(inv), _lib.ptr(c), None), 'r')
+        _lib.check(lib.trh_render_fft_batch(h.value, 128, _lib.ptr(djs), 256, _lib.ptr(spec),
+                                            _lib.ptr(out), None, None, None, None), 'r')
+        torch.cuda.synchronize()
+        return c.cpu().numpy(), out.cpu().numpy()
+    c0, p0 = render()
+    best = A.tune_fft(stamp, [72, 96, 120], njobs=2048, reps=1)
+    assert set(best) == {128} and best[128][2] > 0
+    for t, r in [(None, None), (256, 64), (1024, 0), (512, 120)]:
+        if t is not None:
+            _lib.check(lib.trh_set_fft_config(h.value, 128, t, r), 'cfg')
+        c1, p1 = render()
+        assert np.array_equal(p1, p0), (t, r)
+        np.testing.assert_allclose(c1, c0, rtol=1e-12, atol=0)
+    _lib.check(lib.trh_set_fft_config(h.value, 128, 0, 0), 'cfg')
