@@ -34,7 +34,7 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-@pytest.mark.parametrize('sz', [32, 64, 128, 256])
+@pytest.mark.parametrize('sz', [32, 64, 128, 256, 512])
 def test_psf_spectrum(dev, sz):
     say('psf spectrum sz', sz)
     stamp = W.moffat_stamp(63, 4.0, 3.5)

@@ -46,3 +46,43 @@ def test_error_path_without_gpu():
     # argument validation happens before any device call
     assert lib.trh_render_fft_batch(None, 64, None, 1, None, None, None, None, None, None) == -1
     assert b'null' in lib.trh_last_error()
+
+
+def test_every_entry_point_rejects_bad_arguments_without_a_device():
+    """Argument validation runs on the host before any device call: a null
+    handle (or empty/invalid sizes) gives TRH_E_ARG and a message, never a
+    device access."""
+    lib = _lib.load()
+    N = None
+    calls = {
+        'trh_psf_spectrum_batch': (N, N, 1, 63, 63, 64, N, N, N, N),
+        'trh_render_fft_batch': (N, 64, N, 1, N, N, N, N, N, N),
+        'trh_render_mog_batch': (N, N, 1, 1) + (N,) * 11,
+        'trh_render_mog_approx3_batch': (N, N, 1, 10) + (N,) * 8,
+        'trh_lanczos_shift_batch': (N, N, N, 1, 4, 4, N, N, N),
+        'trh_render_point_batch': (N, N, 1, 4, N, 3, 3, N, N, N, N, N),
+        'trh_normal_eq_dense': (N, N, 10, 3, N, N),
+        'trh_solve_normal_f64': (N, N, 3, 0., 0, N, N, N),
+        'trh_set_fft_config': (N, 64, 0, 0),
+        'trh_psf_combine': (N, N, 9, 100, N, N, N, 1, N),
+        'trh_fp_pair_dots': (N, N, N, 1, N, N, N, N, 10, N, N),
+        'trh_fp_rhs': (N, 1, N, N, N, N, N, N, 10, N, N),
+        'trh_fp_block_chol': (N, N, N, N, N, N, 1, N, N, N),
+        'trh_fp_precond': (N, N, N, 1, 8, N, N, N, N),
+        'trh_fp_spmv': (N, 4, N, N, N, N, N, N),
+        'trh_fp_add_patches': (N, 1, N, N, N, N, 10, N, N),
+        'trh_cnn_conv_fwd': (N, N, N, N, N, 1, 32, 32, 1, N),
+        'trh_cnn_conv_bwd_data': (N, N, N, N, N, 1, 32, 32, N),
+        'trh_cnn_conv_bwd_weight': (N, N, N, N, N, 1, 32, 32, 1, N),
+        'trh_cnn_head': (N, N, N, 1, N, N, N, 1.0, N, N, N, N, N, N),
+        'trh_cnn_adam': (N, N, N, N, N, 10, 1e-3, 0.9, 0.999, 1e-8, 0., N),
+        'trh_cnn_pack': (N, N, 1, N, N, N),
+        'trh_cnn_cutouts': (N, N, N, N, 1, N, N, N),
+        'trh_fitter_iterate': (N, 0., 0., N),
+        'trh_fitter_iterate_graph': (N, 0., 0., 1, N),
+    }
+    assert set(calls) <= set(_lib.SIGNATURES)
+    for name, args in calls.items():
+        rc = getattr(lib, name)(*args)
+        assert rc == -1, (name, rc)
+        assert lib.trh_last_error(), name
