@@ -26,7 +26,7 @@ def candidates(sz):
     many as LDS allows)."""
     th = [128, 256, 512, 1024] if sz <= 64 else [256, 512, 1024]
     rows = [0]
-    if sz >= 128:
+    if sz == 128:   # from 256 on the rows live in global memory, one pass
         rows += [sz // 2, sz - sz // 16]
     return [(t, r) for t in th for r in rows]
 

@@ -931,6 +931,10 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
             (void)hipGraphExecDestroy(f->gexec);
             f->gexec = nullptr;
         }
+        // every galaxy bucket is launched (device-side counts), so the global
+        // row slots of the SZ >= 256 renders must exist before capture
+        for (int b = 0; b < NFFT && f->n_gal; ++b)
+            if ((rc = reserve_fft_hbuf(f->h, kSizes[b]))) return rc;
         if ((rc = hip_check(hipStreamBeginCapture(f->gstream, hipStreamCaptureModeThreadLocal),
                             "begin capture")))
             return rc;

@@ -17,11 +17,23 @@ template <int SZ> struct WaveGeom {
     static constexpr int NSEQ = 64 / SEQL;
     static constexpr int NV = SZ / 2 + 1;
     static constexpr int LBUF = SZ + SZ / 32;   // padded per-sequence line buffer (float2)
+    // The y-filtered half spectrum between the two passes holds up to SZ rows
+    // of NV columns: in LDS for SZ <= 128; from SZ = 256 on (264 KB and more)
+    // it lives in a per-block slot of global memory, row stride HS = NV + 1
+    // so that rows start 16-byte aligned and phase A stores column pairs.
+    static constexpr bool GH = SZ >= 256;
+    static constexpr int HS = GH ? NV + 1 : NV;
 };
+
+// Floats of one block's global half-spectrum slot (SZ rows of HS columns).
+__host__ __device__ inline size_t fft_hbuf_slot_floats(int sz) {
+    return (size_t)2 * sz * (sz / 2 + 2);
+}
 
 // Dynamic LDS of k_render_fft<SZ> with `nwaves` waves per block and `maxrows`
 // rows of the y-filtered half spectrum: per-wave line buffers (2 sequences
-// per wave iteration) + rows.
+// per wave iteration) + rows (maxrows = 0 when the rows live in global
+// memory, SZ >= 256).
 __host__ __device__ inline size_t fft_lds_bytes(int sz, int nwaves, int maxrows) {
     const int nseq = sz >= 64 ? 1 : 64 / sz;
     const int lbuf = sz + sz / 32;
@@ -120,13 +132,18 @@ __device__ __forceinline__ float2 xlane2(float2 v) {
 }
 
 // Per-lane twiddles of the wave FFT, loaded once per block: in-register
-// stages (span d >= 64) need tw[(sl + 64 q) * SZ/(2d)], q < d/64; cross-lane
-// stages tw[(sl & (d-1)) * SZ/(2d)].
+// stages (span d >= 64) need tw[(sl + 64 q) * SZ/(2d)], q < d/64.  A
+// cross-lane stage of span d computes t = x + s * x[lane ^ d] with s = +1 on
+// the lower lane of each pair and -1 on the upper, then t * w' with w' = 1 on
+// the lower lane and -tw[(sl & (d-1)) * SZ/(2d)] on the upper: the upper lane
+// gets (x_hi - x_lo)(-w) = (x_lo - x_hi) w, the DIF difference times the
+// twiddle, with no divergent branch.  sgn[] and lane[] hold s and w' per stage.
 template <int SZ> struct WaveTw {
     static constexpr int NREG = SZ >= 128 ? SZ / 64 - 1 : 0;
     static constexpr int NLANE = SZ >= 64 ? 6 : Log2<SZ>::v;
     float2 reg[NREG > 0 ? NREG : 1];
     float2 lane[NLANE];
+    float sgn[NLANE];
     __device__ __forceinline__ void load(const float2 *tw, int sl) {
         int n = 0;
 #pragma unroll
@@ -135,20 +152,29 @@ template <int SZ> struct WaveTw {
             for (int q = 0; q < d / 64; ++q) reg[n++] = tw[(sl + 64 * q) * (SZ / (2 * d))];
         n = 0;
 #pragma unroll
-        for (int d = (SZ >= 64 ? 32 : SZ / 2); d >= 1; d >>= 1)
-            lane[n++] = tw[(sl & (d - 1)) * (SZ / (2 * d))];
+        for (int d = (SZ >= 64 ? 32 : SZ / 2); d >= 1; d >>= 1) {
+            const bool up = (sl & d) != 0;
+            const float2 w = tw[(sl & (d - 1)) * (SZ / (2 * d))];
+            lane[n] = up ? make_float2(-w.x, -w.y) : make_float2(1.f, 0.f);
+            sgn[n++] = up ? -1.f : 1.f;
+        }
     }
 };
 
+// One cross-lane DIF stage of span D on every register of every sequence:
+// lower lane x_lo + x_hi, upper lane (x_lo - x_hi) w.  The partner's value
+// comes through a DPP move that the compiler folds into the v_fmac_f32 that
+// applies the sign (spans 1, 2; spans 4, 8 keep one separate DPP move).
 template <int SZ, int D, int C>
 __device__ __forceinline__ void idif_lane_stage(float2 (&x)[C][WaveGeom<SZ>::R], float2 w,
-                                                bool up) {
+                                                float s) {
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int r = 0; r < WaveGeom<SZ>::R; ++r) {
-            const float2 y = xlane2<D>(x[c][r]);
-            x[c][r] = up ? cmul(csub(y, x[c][r]), w) : cadd(x[c][r], y);
+            const float px = xlane<D>(x[c][r].x), py = xlane<D>(x[c][r].y);
+            const float tx = fmaf(s, px, x[c][r].x), ty = fmaf(s, py, x[c][r].y);
+            x[c][r] = make_float2(tx * w.x - ty * w.y, tx * w.y + ty * w.x);
         }
 }
 
@@ -180,13 +206,17 @@ __device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const
     }
     int s = 0;
     if constexpr (SZ >= 64) {
-        idif_lane_stage<SZ, 32, C>(x, T.lane[s++], (sl & 32) != 0);
+        idif_lane_stage<SZ, 32, C>(x, T.lane[s], T.sgn[s]);
+        ++s;
     }
-    if constexpr (SZ >= 32) idif_lane_stage<SZ, 16, C>(x, T.lane[s++], (sl & 16) != 0);
-    idif_lane_stage<SZ, 8, C>(x, T.lane[s++], (sl & 8) != 0);
-    idif_lane_stage<SZ, 4, C>(x, T.lane[s++], (sl & 4) != 0);
-    idif_lane_stage<SZ, 2, C>(x, T.lane[s++], (sl & 2) != 0);
-    idif_lane_stage<SZ, 1, C>(x, T.lane[s++], (sl & 1) != 0);
+    if constexpr (SZ >= 32) {
+        idif_lane_stage<SZ, 16, C>(x, T.lane[s], T.sgn[s]);
+        ++s;
+    }
+    idif_lane_stage<SZ, 8, C>(x, T.lane[s], T.sgn[s]);
+    idif_lane_stage<SZ, 4, C>(x, T.lane[s + 1], T.sgn[s + 1]);
+    idif_lane_stage<SZ, 2, C>(x, T.lane[s + 2], T.sgn[s + 2]);
+    idif_lane_stage<SZ, 1, C>(x, T.lane[s + 3], T.sgn[s + 3]);
 }
 
 // Render one FFT job (see trh_fft_job).  Every thread of the block calls this.
@@ -206,9 +236,10 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                                float *__restrict__ out, const float *__restrict__ data,
                                const float *__restrict__ inverr, double *__restrict__ chi2) {
     using G = WaveGeom<SZ>;
-    constexpr int NV = G::NV, R = G::R, SEQL = G::SEQL, NSEQ = G::NSEQ;
+    constexpr int NV = G::NV, R = G::R, SEQL = G::SEQL, NSEQ = G::NSEQ, HS = G::HS;
     constexpr int LOG2 = Log2<SZ>::v;
     constexpr int PER = NSEQ * CW;   // sequences per wave iteration
+    static_assert(!G::GH || (NSEQ == 1 && CW == 2), "global rows store column pairs");
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
@@ -254,25 +285,44 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
         // ---- phase A: columns of the half spectrum
         for (int i0 = wave * PER; i0 < NV; i0 += nw * PER) {
             float2 x[CW][R];
+            // MoG Fourier transform: components outermost, so each one is read
+            // from LDS once for all CW x R frequencies of the lane
+            float v2[CW], vw[CW][R], F[CW][R], w[R], w2[R];
+            int ic[CW];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = sl + SEQL * r;
+                w[r] = (float)(j < SZ / 2 ? j : j - SZ) * (1.0f / SZ);
+                w2[r] = w[r] * w[r];
+            }
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
                 const int i = i0 + c * NSEQ + sq;
-                const int ic = i < NV ? i : NV - 1;
-                const float v = (float)ic * (1.0f / SZ);
-                const float v2 = v * v;
-                const float2 *__restrict__ Pc = P + (size_t)ic * SZ;
+                ic[c] = i < NV ? i : NV - 1;
+                const float v = (float)ic[c] * (1.0f / SZ);
+                v2[c] = v * v;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int j = sl + SEQL * r;
-                    const float w = (float)(j < SZ / 2 ? j : j - SZ) * (1.0f / SZ);
-                    const float vw = v * w, w2 = w * w;
-                    float F = 0.f;
-                    for (int k = 0; k < K; ++k) {
-                        const float4 cp = S.comp[k];
-                        F += cp.x * __builtin_amdgcn_exp2f(cp.y * v2 + cp.z * vw + cp.w * w2);
-                    }
-                    const float2 p = Pc[j];
-                    x[c][r] = make_float2(F * p.x, F * p.y);
+                    vw[c][r] = v * w[r];
+                    F[c][r] = 0.f;
+                }
+            }
+            for (int k = 0; k < K; ++k) {
+                const float4 cp = S.comp[k];
+#pragma unroll
+                for (int c = 0; c < CW; ++c)
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        F[c][r] += cp.x * __builtin_amdgcn_exp2f(cp.y * v2[c] + cp.z * vw[c][r] +
+                                                                 cp.w * w2[r]);
+            }
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const float2 *__restrict__ Pc = P + (size_t)ic[c] * SZ;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float2 p = Pc[sl + SEQL * r];
+                    x[c][r] = make_float2(F[c][r] * p.x, F[c][r] * p.y);
                 }
             }
             wave_idif<SZ, CW>(x, T, sl);
@@ -282,24 +332,49 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 for (int r = 0; r < R; ++r)
                     lbw[(c * NSEQ + sq) * G::LBUF + lpad(bitrev<LOG2>(sl + SEQL * r))] = x[c][r];
             wave_sync();
-#pragma unroll
-            for (int c = 0; c < CW; ++c) {
-                const int i = i0 + c * NSEQ + sq;
-                if (i >= NV) continue;
-                const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
+            if constexpr (G::GH) {
+                // columns i0, i0 + 1 of each row as one 16-byte store (the
+                // pad column HS - 1 takes the clamped duplicate when i0 = NV - 1)
                 for (int q = sl; q < nrb; q += SEQL) {
                     const int yr = rb0 + q;
-                    float2 a = make_float2(0.f, 0.f);
+                    float2 a[CW];
 #pragma unroll
-                    for (int t = 0; t < 7; ++t) {
-                        const int ys = yr - 3 + t;
-                        if (ys >= 0 && ys < SZ) {
-                            const float2 h = lb[lpad(ys)];
-                            a.x += fy[t] * h.x;
-                            a.y += fy[t] * h.y;
+                    for (int c = 0; c < CW; ++c) {
+                        const float2 *lb = lbw + c * G::LBUF;
+                        a[c] = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int t = 0; t < 7; ++t) {
+                            const int ys = yr - 3 + t;
+                            if (ys >= 0 && ys < SZ) {
+                                const float2 h = lb[lpad(ys)];
+                                a[c].x += fy[t] * h.x;
+                                a[c].y += fy[t] * h.y;
+                            }
                         }
                     }
-                    hbuf[q * NV + i] = a;
+                    *reinterpret_cast<float4 *>(hbuf + (size_t)q * HS + i0) =
+                        make_float4(a[0].x, a[0].y, a[1].x, a[1].y);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CW; ++c) {
+                    const int i = i0 + c * NSEQ + sq;
+                    if (i >= NV) continue;
+                    const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
+                    for (int q = sl; q < nrb; q += SEQL) {
+                        const int yr = rb0 + q;
+                        float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int t = 0; t < 7; ++t) {
+                            const int ys = yr - 3 + t;
+                            if (ys >= 0 && ys < SZ) {
+                                const float2 h = lb[lpad(ys)];
+                                a.x += fy[t] * h.x;
+                                a.y += fy[t] * h.y;
+                            }
+                        }
+                        hbuf[q * HS + i] = a;
+                    }
                 }
             }
             wave_sync();
@@ -315,8 +390,8 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 const bool act = pp < npairs;
                 const int qa = 2 * (act ? pp : 0);
                 const bool hasb = act && (qa + 1) < nrb;
-                const float2 *A = hbuf + qa * NV;
-                const float2 *B = A + NV;
+                const float2 *A = hbuf + (size_t)qa * HS;
+                const float2 *B = A + HS;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int k = sl + SEQL * r;
@@ -441,19 +516,27 @@ static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 // Grid-stride render kernel over a job list.  `count` (device, may be NULL:
 // then njobs) bounds the list; `list` (may be NULL: identity) maps list
 // entries to job indices.  blockDim.x is a multiple of 64; dynamic LDS holds
-// fft_lds_bytes(SZ, blockDim.x / 64, maxrows).
+// fft_lds_bytes(SZ, blockDim.x / 64, maxrows).  For SZ >= 256 the rows live
+// in hbuf_g, one slot of fft_hbuf_slot_floats(SZ) floats per block (gridDim.x
+// slots), and every pass takes all SZ rows.
 template <int SZ, int MODE>
 __global__ void __launch_bounds__(1024)
 k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
              const int *__restrict__ count, int njobs, const float2 *__restrict__ psf_spec,
              const float2 *__restrict__ twg, int maxrows, float *__restrict__ out,
              const float *__restrict__ data, const float *__restrict__ inverr,
-             double *__restrict__ chi2) {
+             double *__restrict__ chi2, float2 *__restrict__ hbuf_g) {
     extern __shared__ float2 smem[];
     __shared__ RenderScratch S;
     using G = WaveGeom<SZ>;
     float2 *lbuf = smem;
-    float2 *hbuf = lbuf + (size_t)(blockDim.x >> 6) * G::NSEQ * CW * G::LBUF;
+    float2 *hbuf;
+    if constexpr (G::GH) {
+        hbuf = hbuf_g + (size_t)blockIdx.x * (fft_hbuf_slot_floats(SZ) / 2);
+        maxrows = SZ;
+    } else {
+        hbuf = lbuf + (size_t)(blockDim.x >> 6) * G::NSEQ * CW * G::LBUF;
+    }
     WaveTw<SZ> T;
     T.load(twg, (threadIdx.x & 63) & (G::SEQL - 1));
     const int n = count ? *count : njobs;

@@ -114,6 +114,8 @@ extern "C" int trh_destroy(trh_handle *h) {
     (void)hipSetDevice(h->device);
     if (h->tw) (void)hipFree(h->tw);
     if (h->scratch) (void)hipFree(h->scratch);
+    for (int i = 0; i < 5; ++i)
+        if (h->hbuf[i]) (void)hipFree(h->hbuf[i]);
     delete h;
     return TRH_OK;
 }
@@ -258,9 +260,14 @@ static int sz_index(int sz) {
 // second knob caps the G rows kept per block (0: as many as LDS allows).
 // trh_set_fft_config overrides them (the autotuner's knob).
 // (from a sweep over threads x rows on MI355X; sz 128 keeps 120 rows so two
-// blocks share a CU's 160 KB of LDS)
-static const int kDefThreads[5] = {256, 128, 512, 1024, 1024};
-static const int kDefRows[5] = {0, 0, 120, 64, 0};
+// blocks share a CU's 160 KB of LDS; from sz 256 on the rows are in global
+// slots and the row knob is unused)
+static const int kDefThreads[5] = {256, 128, 512, 512, 1024};
+static const int kDefRows[5] = {0, 0, 120, 0, 0};
+
+// Global row slots of the SZ >= 256 renders (one per block of the grid):
+// 4 per CU at SZ = 256 (272 MB), 2 per CU at SZ = 512 (541 MB).
+int fft_hbuf_slots(const trh_handle *h, int sz) { return h->n_cu * (sz >= 512 ? 2 : 4); }
 
 void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *rows, size_t *lds,
                       int *grid, int njobs) {
@@ -269,14 +276,16 @@ void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *row
     int r = h->fft_ch[si] > 0 ? h->fft_ch[si] : (kDefRows[si] > 0 ? kDefRows[si] : sz);
     if (maxrows > 0 && r > maxrows) r = maxrows;
     if (r > sz) r = sz;
-    size_t b = fft_lds_bytes(sz, nt / 64, r);
+    const bool gh = sz >= 256;   // rows in global slots, all SZ of them per pass
+    if (gh) r = sz;
+    size_t b = fft_lds_bytes(sz, nt / 64, gh ? 0 : r);
     // fit LDS: fewer rows first (down to 32), then fewer waves, then rows to 8
     while (b + sizeof(RenderScratch) > (size_t)h->lds_max) {
-        if (r > 32) r -= 8;
+        if (!gh && r > 32) r -= 8;
         else if (nt > 128) nt /= 2;
-        else if (r > 8) r -= 8;
+        else if (!gh && r > 8) r -= 8;
         else break;
-        b = fft_lds_bytes(sz, nt / 64, r);
+        b = fft_lds_bytes(sz, nt / 64, gh ? 0 : r);
     }
     *rows = r;
     *lds = b;
@@ -287,8 +296,22 @@ void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *row
     if (wg_per_cu > by_threads) wg_per_cu = by_threads;
     if (wg_per_cu < 1) wg_per_cu = 1;
     int g = h->n_cu * wg_per_cu;
+    if (gh && g > fft_hbuf_slots(h, sz)) g = fft_hbuf_slots(h, sz);
     if (njobs > 0 && njobs < g) g = njobs;
     *grid = g < 1 ? 1 : g;
+}
+
+int reserve_fft_hbuf(trh_handle *h, int sz) {
+    if (sz < 256) return TRH_OK;
+    const int si = sz_index(sz);
+    if (si < 0) return set_err(TRH_E_ARG, "reserve_fft_hbuf: sz %d", sz);
+    if (h->hbuf[si]) return TRH_OK;
+    const size_t bytes = sizeof(float) * fft_hbuf_slot_floats(sz) * (size_t)fft_hbuf_slots(h, sz);
+    if (hipMalloc(&h->hbuf[si], bytes) != hipSuccess) {
+        h->hbuf[si] = nullptr;
+        return set_err(TRH_E_NOMEM, "k_render_fft<%d>: row slots of %zu bytes", sz, bytes);
+    }
+    return TRH_OK;
 }
 
 int max_rows_for(int sz, int lds_max) {
@@ -307,20 +330,34 @@ int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list
     if (lds + sizeof(RenderScratch) > (size_t)h->lds_max)
         return set_err(TRH_E_UNSUPPORTED, "k_render_fft<%d>: LDS %zu exceeds %d", SZ, lds,
                        h->lds_max);
+    float2 *hb = nullptr;
+    if constexpr (WaveGeom<SZ>::GH) {
+        const int si = sz_index(SZ);
+        if (!h->hbuf[si]) {
+            // the slots are allocated once, outside any stream capture
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+                return set_err(TRH_E_UNSUPPORTED,
+                               "k_render_fft<%d>: row slots not reserved before capture", SZ);
+            const int rc = reserve_fft_hbuf(h, SZ);
+            if (rc) return rc;
+        }
+        hb = h->hbuf[si];
+    }
     const float2 *tw = h->tw + tw_offset(SZ);
     const float2 *ps = (const float2 *)psf_spec;
     switch (mode) {
     case TRH_STORE:
         hipLaunchKernelGGL((k_render_fft<SZ, TRH_STORE>), dim3(grid), dim3(nt), lds, st, jobs, list,
-                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2);
+                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb);
         break;
     case TRH_CHI2:
         hipLaunchKernelGGL((k_render_fft<SZ, TRH_CHI2>), dim3(grid), dim3(nt), lds, st, jobs, list,
-                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2);
+                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb);
         break;
     default:
         hipLaunchKernelGGL((k_render_fft<SZ, -1>), dim3(grid), dim3(nt), lds, st, jobs, list, count,
-                           njobs_bound, ps, tw, rows, out, data, inverr, chi2);
+                           njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb);
         break;
     }
     return hip_check(hipGetLastError(), "k_render_fft");
