@@ -168,6 +168,25 @@ __global__ void k_fit_reset(int32_t *cnt, int n) {
     if ((int)threadIdx.x < n) cnt[threadIdx.x] = 0;
 }
 
+// Append `slot` to render list b (b < 0: nothing).  Every lane of the wave
+// calls this together: one atomic per bucket present in the wave.
+__device__ __forceinline__ void wave_list_append(int b, int slot, int32_t *__restrict__ lists,
+                                                 int cap, int32_t *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int bb = 0; bb < NBUCKET; ++bb) {
+        const unsigned long long m = __ballot(b == bb);
+        if (m == 0ull) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[bb], __popcll(m));
+        base = __shfl(base, leader);
+        if (b == bb) lists[bb * cap + base + __popcll(m & ((1ull << lane) - 1ull))] = slot;
+    }
+}
+
+// One thread per (source, render): render 0 of every active source and
+// renders 1..3 (EllipseE steps) of the galaxies.
 __global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
                                const int32_t *__restrict__ Parr, const int64_t *__restrict__ roff,
                                const double *__restrict__ params,
@@ -175,55 +194,44 @@ __global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
                                int ph, int pw, trh_fft_job *__restrict__ jobs,
                                int32_t *__restrict__ lists, int cap, int32_t *__restrict__ cnt,
                                int32_t *__restrict__ err, unsigned long long *__restrict__ acc) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S || !active[s]) return;
-    const int kd = kind[s];
-    const int P = Parr[s];
-    const double *p = params + 8 * s;
-    const unsigned long long PP = (unsigned long long)P * P;
-    if (acc) {
-        // normal equations read N_r renders + data + inverr, write 28 doubles
-        atomicAdd(&acc[2 * SL_NEQ], 1ull);
-        atomicAdd(&acc[2 * SL_NEQ + 1], (kd ? 4ull * 4 + 8 : 4ull + 8) * PP + 28 * 8);
-        atomicAdd(&acc[2 * SL_SOLVE], 1ull);
-        atomicAdd(&acc[2 * SL_SOLVE + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
-    }
-    if (kd == 0) {
-        trh_fft_job &J = jobs[4 * s];
-        build_point_job(J, p[0], p[1], P, ph, pw);
-        J.out_offset = roff[s];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = t >> 2, k = t & 3;
+    int b = -1;
+    if (s < S && active[s] && (k == 0 || kind[s] != 0)) {
+        const int kd = kind[s];
+        const int P = Parr[s];
+        const double *p = params + 8 * s;
+        const unsigned long long PP = (unsigned long long)P * P;
+        if (acc && k == 0) {
+            // normal equations read N_r renders + data + inverr, write 28 doubles
+            atomicAdd(&acc[2 * SL_NEQ], 1ull);
+            atomicAdd(&acc[2 * SL_NEQ + 1], (kd ? 4ull * 4 + 8 : 4ull + 8) * PP + 28 * 8);
+            atomicAdd(&acc[2 * SL_SOLVE], 1ull);
+            atomicAdd(&acc[2 * SL_SOLVE + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
+        }
+        trh_fft_job &J = jobs[t];
+        if (kd == 0) {
+            build_point_job(J, p[0], p[1], P, ph, pw);
+            J.out_offset = roff[s];
+            b = PTB;
+        } else {
+            double sh[3] = {p[3], p[4], p[5]};
+            if (k > 0) sh[k - 1] += 0.01;  // EllipseE stepsizes (ellipses.py:44)
+            const int sz = build_gal_job(J, *pr, kd, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
+            J.out_offset = roff[s] + (int64_t)k * P * P;
+            b = sz_bucket(sz);
+            if (b < 0) atomicExch(err, sz);
+        }
         J.mode = TRH_STORE;
         J.scale = 1.f;
         J.slot = 0;
-        const int i = atomicAdd(&cnt[PTB], 1);
-        lists[PTB * cap + i] = 4 * s;
-        if (acc) {
-            atomicAdd(&acc[2 * SL_JPT], 1ull);
-            atomicAdd(&acc[2 * SL_JPT + 1], 4ull * PP);
-        }
-        return;
-    }
-    for (int k = 0; k < 4; ++k) {
-        double sh[3] = {p[3], p[4], p[5]};
-        if (k > 0) sh[k - 1] += 0.01;  // EllipseE stepsizes (ellipses.py:44)
-        trh_fft_job &J = jobs[4 * s + k];
-        const int sz = build_gal_job(J, *pr, kd, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
-        J.out_offset = roff[s] + (int64_t)k * P * P;
-        J.mode = TRH_STORE;
-        J.scale = 1.f;
-        J.slot = 0;
-        const int b = sz_bucket(sz);
-        if (b < 0) {
-            atomicExch(err, sz);
-            continue;
-        }
-        const int i = atomicAdd(&cnt[b], 1);
-        lists[b * cap + i] = 4 * s + k;
-        if (acc) {
-            atomicAdd(&acc[2 * b], 1ull);
-            atomicAdd(&acc[2 * b + 1], 4ull * PP);  // render written
+        if (acc && b >= 0) {
+            const int sl = b == PTB ? SL_JPT : b;
+            atomicAdd(&acc[2 * sl], 1ull);
+            atomicAdd(&acc[2 * sl + 1], 4ull * PP);  // render written
         }
     }
+    wave_list_append(b, t, lists, cap, cnt);
 }
 
 // Model renders (counts x unit-flux patch at the current parameters) into an
@@ -496,6 +504,7 @@ __global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
     steplim[s] = anylim;
 }
 
+// One thread per (source, trial step) of the line search.
 __global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
                               const int32_t *__restrict__ Parr,
                               const int64_t *__restrict__ pix_off,
@@ -506,39 +515,33 @@ __global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
                               trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
                               int cap, int32_t *__restrict__ cnt, int32_t *__restrict__ err,
                               unsigned long long *__restrict__ acc) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S || !active[s]) return;
-    const int kd = kind[s];
-    const int P = Parr[s];
-    const int n = nsteps[s];
-    for (int a = 0; a < n; ++a) {
-        const double *p = trial + ((size_t)s * NALPHA + a) * 8;
-        const int slot = s * NALPHA + a;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = slot / NALPHA, a = slot - s * NALPHA;
+    int b = -1;
+    if (s < S && active[s] && a < nsteps[s]) {
+        const int kd = kind[s];
+        const int P = Parr[s];
+        const double *p = trial + (size_t)slot * 8;
         trh_fft_job &J = jobs[slot];
-        int b;
         if (kd == 0) {
             build_point_job(J, p[0], p[1], P, ph, pw);
             b = PTB;
         } else {
             const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
             b = sz_bucket(sz);
-            if (b < 0) {
-                atomicExch(err, sz);
-                continue;
-            }
+            if (b < 0) atomicExch(err, sz);
         }
         J.out_offset = pix_off[s];
         J.mode = TRH_CHI2;
         J.scale = (float)p[2];
         J.slot = slot;
-        const int i = atomicAdd(&cnt[b], 1);
-        lists[b * cap + i] = slot;
-        if (acc) {
+        if (acc && b >= 0) {
             const int sl = b == PTB ? SL_LPT : SL_LS + b;
             atomicAdd(&acc[2 * sl], 1ull);
             atomicAdd(&acc[2 * sl + 1], 8ull * P * P);  // data + inverr read
         }
     }
+    wave_list_append(b, slot, lists, cap, cnt);
 }
 
 // tryUpdates acceptance (constrained_optimizer.py:36-82) and the optimize_loop
@@ -821,7 +824,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if (!capture) (void)hipEventRecord(f->ev[0], st);
     SLOT_BEGIN(SL_PREP);
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
-    hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->render_off,
+    hipLaunchKernelGGL(k_fit_prep_jac, dim3((4 * S + tb - 1) / tb), dim3(tb), 0, st, S, f->kind, f->P, f->render_off,
                        f->params, f->active, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
                        cap_jac, f->cnt, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
@@ -856,7 +859,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if ((rc = hip_check(hipGetLastError(), "k_fit_solve"))) return rc;
     SLOT_END(SL_SOLVE);
     if (!capture) (void)hipEventRecord(f->ev[3], st);
-    hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb), dim3(tb), 0, st, S, f->kind, f->P, f->pix_off,
+    hipLaunchKernelGGL(k_fit_prep_ls, dim3((NALPHA * S + tb - 1) / tb), dim3(tb), 0, st, S, f->kind, f->P, f->pix_off,
                        f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw, f->jobs_ls,
                        f->list_ls, cap_ls, f->cnt + LS_CNT, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;

@@ -467,7 +467,7 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 // window coordinates (sx, sy), Lanczos-3 shifted by (mux, muy); stamp, tmp
 // in LDS: tmp holds ph x mw floats.
 static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
-                                 const float *__restrict__ stamp, int ph, int pw,
+                                 const float *__restrict__ stamp, bool load_stamp, int ph, int pw,
                                  float *lstamp, float *tmp, RenderScratch &S,
                                  float *__restrict__ out, const float *__restrict__ data,
                                  const float *__restrict__ inverr, double *__restrict__ chi2) {
@@ -479,10 +479,14 @@ static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     compute_taps(Jp->mux, Jp->muy, S);
     const float *fx = S.taps;
     const float *fy = S.taps + 7;
-    for (int i = tid; i < ph * pw; i += nthr) lstamp[i] = stamp[i];
+    // the stamp stays in LDS while consecutive jobs of the block share it
+    if (load_stamp)
+        for (int i = tid; i < ph * pw; i += nthr) lstamp[i] = stamp[i];
     __syncthreads();
-    // x pass over stamp rows: tmp[r][ox] = sum_u fx[u] stamp[r][ox-sx-3+u]
-    for (int idx = tid; idx < ph * mw; idx += nthr) {
+    // x pass over the stamp rows the y taps reach, [rlo, rhi):
+    // tmp[r][ox] = sum_u fx[u] stamp[r][ox-sx-3+u]
+    const int rlo = max(0, -sy - 3), rhi = min(ph, mh - sy + 3);
+    for (int idx = tid + rlo * mw; idx < rhi * mw; idx += nthr) {
         const int r = idx / mw;
         const int ox = idx - r * mw;
         float a = 0.f;
@@ -559,12 +563,15 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
     float *lstamp = reinterpret_cast<float *>(smem);
     float *tmp = lstamp + ((ph * pw + 3) & ~3);
     const int n = count ? *count : njobs;
+    int cur = -1;   // stamp index held in LDS
     for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
         const int jid = list ? list[jj] : jj;
         if (jobs[jid].mw > maxw) continue;  // host guarantees maxw >= every mw
         // the job's psf field selects one of the stamps at stamp + psf * ph * pw
-        render_point_one(jobs + jid, stamp + (size_t)jobs[jid].psf * ph * pw, ph, pw, lstamp, tmp,
-                         S, out, data, inverr, chi2);
+        const int psf = jobs[jid].psf;
+        render_point_one(jobs + jid, stamp + (size_t)psf * ph * pw, psf != cur, ph, pw, lstamp,
+                         tmp, S, out, data, inverr, chi2);
+        cur = psf;
     }
 }
 
