@@ -33,11 +33,13 @@ __host__ __device__ inline size_t fft_hbuf_slot_floats(int sz) {
 // Dynamic LDS of k_render_fft<SZ> with `nwaves` waves per block and `maxrows`
 // rows of the y-filtered half spectrum: per-wave line buffers (2 sequences
 // per wave iteration) + rows (maxrows = 0 when the rows live in global
-// memory, SZ >= 256).
+// memory, SZ >= 256, which keep the 2 SZ Lanczos responses there instead).
 __host__ __device__ inline size_t fft_lds_bytes(int sz, int nwaves, int maxrows) {
     const int nseq = sz >= 64 ? 1 : 64 / sz;
     const int lbuf = sz + sz / 32;
-    return sizeof(float2) * ((size_t)nwaves * nseq * 2 * lbuf + (size_t)maxrows * (sz / 2 + 1));
+    const size_t ltab = sz >= 256 ? 2 * (size_t)sz : 0;   // Lanczos responses (global rows)
+    return sizeof(float2) * ((size_t)nwaves * nseq * 2 * lbuf + (size_t)maxrows * (sz / 2 + 1) +
+                             ltab);
 }
 
 struct RenderScratch {
@@ -126,24 +128,48 @@ __device__ __forceinline__ float xlane(float v) {
     }
 }
 
-template <int D>
-__device__ __forceinline__ float2 xlane2(float2 v) {
-    return make_float2(xlane<D>(v.x), xlane<D>(v.y));
+// Transpose lane bit B with a register bit: for the register pair (a, b),
+// a' = a on lanes with bit B clear and b[lane - 2^B] on lanes with it set;
+// b' = a[lane + 2^B] on lanes with bit B clear and b on lanes with it set.
+// Afterwards a holds the elements whose index bit moved from the lane to the
+// register was 0 and b those where it was 1, and lane bit B carries what the
+// register bit carried.  B = 5, 4: gfx950's v_permlane32_swap /
+// v_permlane16_swap (one instruction for both registers); B = 3, 2: two DPP
+// row shifts, each writing only the 4-lane banks of one half.
+template <int B>
+__device__ __forceinline__ void lane_transpose(float &a, float &b) {
+    if constexpr (B == 5 || B == 4) {
+        const auto r = B == 5 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a),
+                                                                  __float_as_uint(b), false, false)
+                              : __builtin_amdgcn_permlane16_swap(__float_as_uint(a),
+                                                                  __float_as_uint(b), false, false);
+        a = __uint_as_float(r[0]);
+        b = __uint_as_float(r[1]);
+    } else {
+        static_assert(B == 3 || B == 2, "lane bits 2..5");
+        constexpr int D = 1 << B;
+        constexpr int hi = B == 3 ? 0xC : 0xA, lo = B == 3 ? 0x3 : 0x5;   // bank masks
+        const int ia = __float_as_int(a), ib = __float_as_int(b);
+        a = __int_as_float(__builtin_amdgcn_update_dpp(ia, ib, 0x110 + D, 0xF, hi, false));
+        b = __int_as_float(__builtin_amdgcn_update_dpp(ib, ia, 0x100 + D, 0xF, lo, false));
+    }
 }
 
-// Per-lane twiddles of the wave FFT, loaded once per block: in-register
-// stages (span d >= 64) need tw[(sl + 64 q) * SZ/(2d)], q < d/64.  A
-// cross-lane stage of span d computes t = x + s * x[lane ^ d] with s = +1 on
-// the lower lane of each pair and -1 on the upper, then t * w' with w' = 1 on
-// the lower lane and -tw[(sl & (d-1)) * SZ/(2d)] on the upper: the upper lane
-// gets (x_hi - x_lo)(-w) = (x_lo - x_hi) w, the DIF difference times the
-// twiddle, with no divergent branch.  sgn[] and lane[] hold s and w' per stage.
+// Per-lane twiddles of the wave FFT, loaded once per block.  In-register
+// stages (span d >= 64) need tw[(sl + 64 q) * SZ/(2d)], q < d/64; the stages
+// of span d = 32..4, done in registers after a lane transpose, need
+// tw[(sl & (d-1)) * SZ/(2d)] (the lane bits below d are still the element
+// bits below d).  The stages of span 2 and 1 stay across lanes: t = x + s *
+// x[lane ^ d] with s = +1 on the lower lane of each pair and -1 on the upper,
+// then t * w' with w' = 1 on the lower lane and -tw[(sl & (d-1)) * SZ/(2d)] on
+// the upper, so the upper lane gets (x_hi - x_lo)(-w) = (x_lo - x_hi) w with
+// no divergent branch.
 template <int SZ> struct WaveTw {
     static constexpr int NREG = SZ >= 128 ? SZ / 64 - 1 : 0;
     static constexpr int NLANE = SZ >= 64 ? 6 : Log2<SZ>::v;
     float2 reg[NREG > 0 ? NREG : 1];
-    float2 lane[NLANE];
-    float sgn[NLANE];
+    float2 lane[NLANE];   // spans 32 (or 16) .. 1
+    float sgn[2];         // spans 2, 1
     __device__ __forceinline__ void load(const float2 *tw, int sl) {
         int n = 0;
 #pragma unroll
@@ -153,18 +179,49 @@ template <int SZ> struct WaveTw {
         n = 0;
 #pragma unroll
         for (int d = (SZ >= 64 ? 32 : SZ / 2); d >= 1; d >>= 1) {
-            const bool up = (sl & d) != 0;
             const float2 w = tw[(sl & (d - 1)) * (SZ / (2 * d))];
-            lane[n] = up ? make_float2(-w.x, -w.y) : make_float2(1.f, 0.f);
-            sgn[n++] = up ? -1.f : 1.f;
+            if (d >= 4) {
+                lane[n++] = w;
+            } else {
+                const bool up = (sl & d) != 0;
+                lane[n++] = up ? make_float2(-w.x, -w.y) : make_float2(1.f, 0.f);
+                sgn[d == 2 ? 0 : 1] = up ? -1.f : 1.f;
+            }
         }
     }
 };
 
-// One cross-lane DIF stage of span D on every register of every sequence:
-// lower lane x_lo + x_hi, upper lane (x_lo - x_hi) w.  The partner's value
-// comes through a DPP move that the compiler folds into the v_fmac_f32 that
-// applies the sign (spans 1, 2; spans 4, 8 keep one separate DPP move).
+// DIF butterfly between the register pair (a: element bit 0, b: bit 1) after
+// the transpose of lane bit B: a + b, (a - b) w.
+template <int B>
+__device__ __forceinline__ void transposed_butterfly(float2 &a, float2 &b, float2 w) {
+    lane_transpose<B>(a.x, b.x);
+    lane_transpose<B>(a.y, b.y);
+    const float2 s = cadd(a, b), d = csub(a, b);
+    a = s;
+    b = cmul(d, w);
+}
+
+// The stage of span 2^B (B = 2..5).  The register bit it uses: bit 0 of the
+// register index for SZ >= 128 (it held element bit 6, finished by then),
+// the sequence bit c for SZ <= 64.
+template <int SZ, int B, int C>
+__device__ __forceinline__ void idif_transposed_stage(float2 (&x)[C][WaveGeom<SZ>::R], float2 w) {
+    constexpr int R = WaveGeom<SZ>::R;
+    if constexpr (SZ >= 128) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int m = 0; m < R / 2; ++m) transposed_butterfly<B>(x[c][2 * m], x[c][2 * m + 1], w);
+    } else {
+        static_assert(C == 2, "the sequence bit pairs two sequences");
+#pragma unroll
+        for (int r = 0; r < R; ++r) transposed_butterfly<B>(x[0][r], x[1][r], w);
+    }
+}
+
+// One cross-lane DIF stage of span D = 2 or 1 on every register: lower lane
+// x_lo + x_hi, upper lane (x_lo - x_hi) w (see WaveTw).
 template <int SZ, int D, int C>
 __device__ __forceinline__ void idif_lane_stage(float2 (&x)[C][WaveGeom<SZ>::R], float2 w,
                                                 float s) {
@@ -179,13 +236,15 @@ __device__ __forceinline__ void idif_lane_stage(float2 (&x)[C][WaveGeom<SZ>::R],
 }
 
 // Inverse (exp(+2 pi i jk/SZ)) radix-2 decimation-in-frequency FFT of C
-// independent sequences held by the lanes (interleaved for latency hiding):
-// natural order in, bit-reversed order out.  Stages with span >= 64 pair
-// registers of a lane; spans 32..1 pair lanes sl ^ span.  No LDS arrays, no
+// independent sequences held by the lanes (interleaved for latency hiding).
+// In: register (c, r) of lane l holds element sl + SEQL r of sequence
+// c NSEQ + l / SEQL (sl = l % SEQL).  Stages of span >= 64 pair registers of
+// a lane; spans 32..4 first transpose their lane bit into a register bit
+// (lane_transpose) and then pair registers; spans 2 and 1 pair lanes.  Out:
+// see wave_fft_out; the elements are in bit-reversed order.  No LDS, no
 // barriers.
 template <int SZ, int C>
-__device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const WaveTw<SZ> &T,
-                                          int sl) {
+__device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const WaveTw<SZ> &T) {
     constexpr int R = WaveGeom<SZ>::R;
     int n = 0;
 #pragma unroll
@@ -204,19 +263,53 @@ __device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const
         }
         n += rd;
     }
-    int s = 0;
-    if constexpr (SZ >= 64) {
-        idif_lane_stage<SZ, 32, C>(x, T.lane[s], T.sgn[s]);
-        ++s;
+    constexpr int TOP = SZ >= 64 ? 5 : Log2<SZ>::v - 1;   // highest lane element bit
+    if constexpr (TOP >= 5) idif_transposed_stage<SZ, 5, C>(x, T.lane[TOP - 5]);
+    idif_transposed_stage<SZ, 4, C>(x, T.lane[TOP - 4]);
+    idif_transposed_stage<SZ, 3, C>(x, T.lane[TOP - 3]);
+    idif_transposed_stage<SZ, 2, C>(x, T.lane[TOP - 2]);
+    idif_lane_stage<SZ, 2, C>(x, T.lane[TOP - 1], T.sgn[0]);
+    idif_lane_stage<SZ, 1, C>(x, T.lane[TOP], T.sgn[1]);
+}
+
+// Where wave_idif leaves register (c, r) of lane l: the sequence slot
+// (c NSEQ + sq of the input numbering) and the element index e (the output
+// index is bitrev(e)).  Element bits 0, 1 stay on lane bits 0, 1; element bit
+// 2 ends on the register bit of the transposes; element bits 3..5 (3..4 for
+// SZ = 32) on lane bits 2..4 (2..3); what the register bit held first
+// (element bit 6, or the sequence bit c) on the next lane bit.
+template <int SZ>
+__device__ __forceinline__ void wave_fft_out(int l, int c, int r, int &slot, int &e) {
+    if constexpr (SZ >= 128) {
+        slot = c;
+        e = (l & 3) | ((r & 1) << 2) | (((l >> 2) & 15) << 3) | ((r >> 1) << 7);
+    } else if constexpr (SZ == 64) {
+        slot = l >> 5;
+        e = (l & 3) | (c << 2) | (((l >> 2) & 7) << 3);
+    } else {
+        static_assert(SZ == 32, "SZ 32..512");
+        slot = ((l >> 4) & 1) * 2 + (l >> 5);
+        e = (l & 3) | (c << 2) | (((l >> 2) & 3) << 3);
     }
-    if constexpr (SZ >= 32) {
-        idif_lane_stage<SZ, 16, C>(x, T.lane[s], T.sgn[s]);
-        ++s;
+}
+
+// Frequency response of the 7-tap Lanczos filter f at DFT index k of an SZ
+// point transform: sum_u f[u] exp(+2 pi i k (u - 3) / SZ).  Filtering the
+// spectrum by it is the circular convolution h[x] = sum_u f[u] g[x - 3 + u],
+// equal to the reference's truncated one wherever all 7 taps fall inside the
+// box.  tw: exp(+2 pi i m / SZ), m < SZ/2.
+template <int SZ>
+__device__ __forceinline__ float2 lanczos_response(const float *f, int k,
+                                                   const float2 *__restrict__ tw) {
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < 7; ++u) {
+        const int m = (k * (u - 3)) & (SZ - 1);
+        const float2 e = m < SZ / 2 ? tw[m] : make_float2(-tw[m - SZ / 2].x, -tw[m - SZ / 2].y);
+        acc.x += f[u] * e.x;
+        acc.y += f[u] * e.y;
     }
-    idif_lane_stage<SZ, 8, C>(x, T.lane[s], T.sgn[s]);
-    idif_lane_stage<SZ, 4, C>(x, T.lane[s + 1], T.sgn[s + 1]);
-    idif_lane_stage<SZ, 2, C>(x, T.lane[s + 2], T.sgn[s + 2]);
-    idif_lane_stage<SZ, 1, C>(x, T.lane[s + 3], T.sgn[s + 3]);
+    return acc;
 }
 
 // Render one FFT job (see trh_fft_job).  Every thread of the block calls this.
@@ -229,9 +322,10 @@ __device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const
 // psf_spec: [psf][SZ/2+1][SZ] complex (column-major half spectrum).
 constexpr int CW = 2;
 
-template <int SZ, int MODE>
+template <int SZ, int MODE, bool FD>
 __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                                const float2 *__restrict__ psf_spec, const WaveTw<SZ> &T,
+                               const float2 *__restrict__ twg, float2 *ltab,
                                float2 *lbuf, float2 *hbuf, int maxrows, RenderScratch &S,
                                float *__restrict__ out, const float *__restrict__ data,
                                const float *__restrict__ inverr, double *__restrict__ chi2) {
@@ -263,6 +357,24 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     const float *fx = S.taps;
     const float *fy = S.taps + 7;
     const float inv_norm = 1.0f / (float)(SZ * SZ);
+    // FD: the Lanczos shift as a spectrum filter (the window keeps every tap
+    // inside the box): y response at the lane's row frequencies (phase A),
+    // x response at its column frequencies (phase B), both at sl + SEQL r.
+    // Held in registers up to SZ = 128; from 256 on (R >= 4) in the LDS table
+    // ltab (y at [0, SZ), x at [SZ, 2 SZ)), filled by the block per job.
+    constexpr bool LREG = FD && !G::GH;
+    float2 Ly[LREG ? R : 1], Lx[LREG ? R : 1];
+    if constexpr (LREG) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Ly[r] = lanczos_response<SZ>(fy, sl + SEQL * r, twg);
+            Lx[r] = lanczos_response<SZ>(fx, sl + SEQL * r, twg);
+        }
+    } else if constexpr (FD) {
+        for (int k = tid; k < 2 * SZ; k += nthr)
+            ltab[k] = lanczos_response<SZ>(k < SZ ? fy : fx, k & (SZ - 1), twg);
+        __syncthreads();
+    }
 
     double acc = 0.0;
     // G rows [ylo, yhi) land in the window; other window rows are empty.
@@ -321,16 +433,21 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 const float2 *__restrict__ Pc = P + (size_t)ic[c] * SZ;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const float2 p = Pc[sl + SEQL * r];
+                    float2 p = Pc[sl + SEQL * r];
+                    if constexpr (LREG) p = cmul(p, Ly[r]);
+                    else if constexpr (FD) p = cmul(p, ltab[sl + SEQL * r]);
                     x[c][r] = make_float2(F[c][r] * p.x, F[c][r] * p.y);
                 }
             }
-            wave_idif<SZ, CW>(x, T, sl);
+            wave_idif<SZ, CW>(x, T);
 #pragma unroll
             for (int c = 0; c < CW; ++c)
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    lbw[(c * NSEQ + sq) * G::LBUF + lpad(bitrev<LOG2>(sl + SEQL * r))] = x[c][r];
+                for (int r = 0; r < R; ++r) {
+                    int slot, e;
+                    wave_fft_out<SZ>(lane, c, r, slot, e);
+                    lbw[slot * G::LBUF + lpad(bitrev<LOG2>(e))] = x[c][r];
+                }
             wave_sync();
             if constexpr (G::GH) {
                 // columns i0, i0 + 1 of each row as one 16-byte store (the
@@ -341,14 +458,18 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
                         const float2 *lb = lbw + c * G::LBUF;
-                        a[c] = make_float2(0.f, 0.f);
+                        if constexpr (FD) {
+                            a[c] = lb[lpad(yr)];
+                        } else {
+                            a[c] = make_float2(0.f, 0.f);
 #pragma unroll
-                        for (int t = 0; t < 7; ++t) {
-                            const int ys = yr - 3 + t;
-                            if (ys >= 0 && ys < SZ) {
-                                const float2 h = lb[lpad(ys)];
-                                a[c].x += fy[t] * h.x;
-                                a[c].y += fy[t] * h.y;
+                            for (int t = 0; t < 7; ++t) {
+                                const int ys = yr - 3 + t;
+                                if (ys >= 0 && ys < SZ) {
+                                    const float2 h = lb[lpad(ys)];
+                                    a[c].x += fy[t] * h.x;
+                                    a[c].y += fy[t] * h.y;
+                                }
                             }
                         }
                     }
@@ -363,14 +484,19 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                     const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
                     for (int q = sl; q < nrb; q += SEQL) {
                         const int yr = rb0 + q;
-                        float2 a = make_float2(0.f, 0.f);
+                        float2 a;
+                        if constexpr (FD) {
+                            a = lb[lpad(yr)];
+                        } else {
+                            a = make_float2(0.f, 0.f);
 #pragma unroll
-                        for (int t = 0; t < 7; ++t) {
-                            const int ys = yr - 3 + t;
-                            if (ys >= 0 && ys < SZ) {
-                                const float2 h = lb[lpad(ys)];
-                                a.x += fy[t] * h.x;
-                                a.y += fy[t] * h.y;
+                            for (int t = 0; t < 7; ++t) {
+                                const int ys = yr - 3 + t;
+                                if (ys >= 0 && ys < SZ) {
+                                    const float2 h = lb[lpad(ys)];
+                                    a.x += fy[t] * h.x;
+                                    a.y += fy[t] * h.y;
+                                }
                             }
                         }
                         hbuf[q * HS + i] = a;
@@ -410,15 +536,20 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                         const float2 b = hasb ? B[SZ - k] : make_float2(0.f, 0.f);
                         Z = make_float2(a.x + b.y, b.x - a.y);
                     }
+                    if constexpr (LREG) Z = cmul(Z, Lx[r]);
+                    else if constexpr (FD) Z = cmul(Z, ltab[SZ + k]);
                     x[c][r] = Z;
                 }
             }
-            wave_idif<SZ, CW>(x, T, sl);
+            wave_idif<SZ, CW>(x, T);
 #pragma unroll
             for (int c = 0; c < CW; ++c)
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    lbw[(c * NSEQ + sq) * G::LBUF + lpad(bitrev<LOG2>(sl + SEQL * r))] = x[c][r];
+                for (int r = 0; r < R; ++r) {
+                    int slot, e;
+                    wave_fft_out<SZ>(lane, c, r, slot, e);
+                    lbw[slot * G::LBUF + lpad(bitrev<LOG2>(e))] = x[c][r];
+                }
             wave_sync();
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
@@ -431,7 +562,12 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 for (int ox = sl; ox < mw; ox += SEQL) {
                     const int xs = ox - sx;
                     float va = 0.f, vb = 0.f;
-                    if (xs >= 0 && xs < SZ) {
+                    if constexpr (FD) {
+                        // every xs is inside [3, SZ - 4] on this path
+                        const float2 g = lb[lpad(xs)];
+                        va = g.x * inv_norm;
+                        vb = g.y * inv_norm;
+                    } else if (xs >= 0 && xs < SZ) {
 #pragma unroll
                         for (int u = 0; u < 7; ++u) {
                             const int xx = xs - 3 + u;
@@ -534,9 +670,10 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
     __shared__ RenderScratch S;
     using G = WaveGeom<SZ>;
     float2 *lbuf = smem;
-    float2 *hbuf;
+    float2 *hbuf, *ltab = nullptr;
     if constexpr (G::GH) {
         hbuf = hbuf_g + (size_t)blockIdx.x * (fft_hbuf_slot_floats(SZ) / 2);
+        ltab = lbuf + (size_t)(blockDim.x >> 6) * G::NSEQ * CW * G::LBUF;
         maxrows = SZ;
     } else {
         hbuf = lbuf + (size_t)(blockDim.x >> 6) * G::NSEQ * CW * G::LBUF;
@@ -546,8 +683,18 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
     const int n = count ? *count : njobs;
     for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
         const int jid = list ? list[jj] : jj;
-        render_fft_one<SZ, MODE>(jobs + jid, psf_spec, T, lbuf, hbuf, maxrows, S, out, data,
-                                 inverr, chi2);
+        // the spectrum-filter path when every Lanczos tap of every output
+        // pixel stays inside the box (rows [3, SZ-3), columns likewise)
+        const trh_fft_job *Jp = jobs + jid;
+        const int sy = Jp->sy, sx = Jp->sx;
+        const int ylo = max(0, -sy), yhi = min(SZ, Jp->mh - sy);
+        const bool fd = ylo >= 3 && yhi <= SZ - 3 && -sx >= 3 && Jp->mw - sx <= SZ - 3;
+        if (fd)
+            render_fft_one<SZ, MODE, true>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, S,
+                                           out, data, inverr, chi2);
+        else
+            render_fft_one<SZ, MODE, false>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, S,
+                                            out, data, inverr, chi2);
     }
 }
 
