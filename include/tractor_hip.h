@@ -140,6 +140,8 @@ int trh_psf_spectrum_batch(trh_handle *h, const float *stamps, int n, int ph, in
  * jobs: device array of njobs trh_fft_job.  psf_spec: [npsf][sz/2+1][sz]
  * complex64 (the trh_psf_spectrum_batch layout).  out: float32 (TRH_STORE / TRH_ADD).  data, inverr: float32
  * (TRH_CHI2).  chi2: double slots (TRH_CHI2).  Unused pointers may be NULL.
+ * With out NULL and chi2, data, inverr given, every job is a TRH_CHI2 job
+ * (its mode field is not read) and a chi^2-specialised kernel runs.
  */
 int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jobs, int njobs,
                          const float *psf_spec, float *out, const float *data,

@@ -115,3 +115,44 @@ def test_cnn_single_cutout(dev):
     assert out.shape == (1, 9) and torch.all(torch.isfinite(out))
     with pytest.raises(ValueError):
         net.forward(torch.rand((2, 32, 32, 1), device=dev).to(torch.bfloat16), 2)
+
+
+def _interior(sz, J):
+    """The render kernel filters the spectrum (Lanczos as a frequency
+    response) when every tap of every output pixel stays inside the box."""
+    sy, sx, mh, mw = int(J['sy']), int(J['sx']), int(J['mh']), int(J['mw'])
+    ylo, yhi = max(0, -sy), min(sz, mh - sy)
+    return ylo >= 3 and yhi <= sz - 3 and -sx >= 3 and mw - sx <= sz - 3
+
+
+@pytest.mark.parametrize('sz_target', [64, 128, 256])
+def test_spectrum_filter_and_spatial_paths(dev, sz_target):
+    """Windows on both sides of the interior rule, for one FFT size: the
+    spectrum-filter path and the spatial Lanczos path agree with the oracle."""
+    import torch
+    import oracle as O
+    from tractor_amd import render as R, workloads as W
+    from tractor_amd._lib import FFT_JOB, TRH_STORE
+    psf = W.moffat_stamp(31, 3.0, 3.5)
+    sp = R.PsfSpectra(psf, dev)
+    amp, var = _sheared(2, O.esoft_from_e(3., 0.3, -0.2))
+    rng = np.random.default_rng(sz_target)
+    seen = set()
+    Ps = list(range(sz_target // 2 - 1, sz_target - 1, max(1, sz_target // 16))) + \
+        [sz_target - 3, sz_target - 2]
+    for P in Ps:
+        for _ in range(2):
+            px, py = P / 2. + rng.uniform(-0.5, 0.5), P / 2. + rng.uniform(-0.5, 0.5)
+            sz, J, win = R.galaxy_fft_job(amp, var, px, py, psf.shape, mm=(0, 0, P, P))
+            if sz != sz_target:
+                continue
+            path = _interior(sz, J)
+            seen.add(path)
+            J['out_offset'], J['out_stride'], J['mode'] = 0, P, TRH_STORE
+            out = torch.zeros(P * P, device=dev)
+            R.render_fft_jobs(sz, np.array([J], FFT_JOB), sp, out=out, device=dev)
+            torch.cuda.synchronize()
+            _, _, G = O.render_galaxy_fft(amp, var, px, py, psf, mm=(0, 0, P, P))
+            got = out.cpu().numpy().reshape(P, P)
+            assert np.max(np.abs(got - G)) <= 1e-5 * np.max(np.abs(G)), (sz, P, px, py, path)
+    assert seen == {True, False}, seen

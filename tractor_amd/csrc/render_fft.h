@@ -379,7 +379,7 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     double acc = 0.0;
     // G rows [ylo, yhi) land in the window; other window rows are empty.
     const int ylo = max(0, -sy);
-    const int yhi = min(SZ, mh - sy);
+    const int yhi = mw > 0 ? min(SZ, mh - sy) : ylo;   // a window of width 0 has no pixels
     if (mode != TRH_ADD) {
         const int top = min(mh, max(0, sy));               // rows [0, top)
         const int bot = max(top, min(mh, SZ + sy));        // rows [bot, mh)
@@ -520,22 +520,17 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 const float2 *B = A + HS;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
+                    // Z_k = a_k + i b_k from the half spectra of rows a, b:
+                    // k > SZ/2 takes the conjugates at SZ - k, and irfft
+                    // ignores the imaginary parts at DC and Nyquist (factor
+                    // f on the imaginary parts; no divergent branch)
                     const int k = sl + SEQL * r;
-                    float2 Z;
-                    if (!act) {
-                        Z = make_float2(0.f, 0.f);
-                    } else if (k == 0 || k == SZ / 2) {
-                        // irfft ignores the imaginary parts of DC and Nyquist
-                        Z = make_float2(A[k].x, hasb ? B[k].x : 0.f);
-                    } else if (k < SZ / 2) {
-                        const float2 a = A[k];
-                        const float2 b = hasb ? B[k] : make_float2(0.f, 0.f);
-                        Z = make_float2(a.x - b.y, a.y + b.x);
-                    } else {
-                        const float2 a = A[SZ - k];
-                        const float2 b = hasb ? B[SZ - k] : make_float2(0.f, 0.f);
-                        Z = make_float2(a.x + b.y, b.x - a.y);
-                    }
+                    const int kk = k <= SZ / 2 ? k : SZ - k;
+                    const float f = (k == 0 || k == SZ / 2) ? 0.f : (k < SZ / 2 ? 1.f : -1.f);
+                    float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+                    if (act) a = A[kk];
+                    if (hasb) b = B[kk];
+                    float2 Z = make_float2(a.x - f * b.y, f * a.y + b.x);
                     if constexpr (LREG) Z = cmul(Z, Lx[r]);
                     else if constexpr (FD) Z = cmul(Z, ltab[SZ + k]);
                     x[c][r] = Z;
@@ -551,40 +546,92 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                     lbw[slot * G::LBUF + lpad(bitrev<LOG2>(e))] = x[c][r];
                 }
             wave_sync();
+            // G rows a, b (real, imaginary part) at column xs of the box
+            auto pair_value = [&](const float2 *lb, int xs) -> float2 {
+                float va = 0.f, vb = 0.f;
+                if constexpr (FD) {
+                    // every xs of the window is inside [3, SZ - 4] on this path
+                    const float2 g = lb[lpad(xs)];
+                    va = g.x;
+                    vb = g.y;
+                } else if (xs >= 0 && xs < SZ) {
 #pragma unroll
-            for (int c = 0; c < CW; ++c) {
-                const int pp = p0 + c * NSEQ + sq;
-                if (pp >= npairs) continue;
-                const int qa = 2 * pp;
-                const bool hasb = (qa + 1) < nrb;
-                const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
-                const int oya = rb0 + qa + sy;
-                for (int ox = sl; ox < mw; ox += SEQL) {
-                    const int xs = ox - sx;
-                    float va = 0.f, vb = 0.f;
-                    if constexpr (FD) {
-                        // every xs is inside [3, SZ - 4] on this path
-                        const float2 g = lb[lpad(xs)];
-                        va = g.x * inv_norm;
-                        vb = g.y * inv_norm;
-                    } else if (xs >= 0 && xs < SZ) {
-#pragma unroll
-                        for (int u = 0; u < 7; ++u) {
-                            const int xx = xs - 3 + u;
-                            if (xx >= 0 && xx < SZ) {
-                                const float2 g = lb[lpad(xx)];
-                                va += fx[u] * g.x;
-                                vb += fx[u] * g.y;
-                            }
+                    for (int u = 0; u < 7; ++u) {
+                        const int xx = xs - 3 + u;
+                        if (xx >= 0 && xx < SZ) {
+                            const float2 g = lb[lpad(xx)];
+                            va += fx[u] * g.x;
+                            vb += fx[u] * g.y;
                         }
-                        va *= inv_norm;
-                        vb *= inv_norm;
                     }
-                    acc += epilogue(mode, va, scale, off + (int64_t)oya * stride + ox, out, data,
-                                    inverr);
-                    if (hasb)
-                        acc += epilogue(mode, vb, scale, off + (int64_t)(oya + 1) * stride + ox,
-                                        out, data, inverr);
+                }
+                return make_float2(va * inv_norm, vb * inv_norm);
+            };
+            if (MODE == TRH_CHI2 && mw <= SZ) {
+                // (compile-time chi^2 kernels only: in the kernel that reads
+                // the mode from the job it would raise every mode's register
+                // pressure) the data / inverr loads of TB columns per lane
+                // (both sequences, both rows) are issued before the arithmetic:
+                // indices clamped into the window (mw <= SZ: SZ / SEQL columns
+                // per lane at most; true of every modelMask window, whose
+                // halfsize covers it), the clamped pixels' terms dropped
+                constexpr int NT = SZ / SEQL, TB = SZ >= 256 ? 1 : (NT < 2 ? NT : 2);
+                int64_t rowa[CW], rowb[CW];
+                bool act[CW], hasb[CW];
+#pragma unroll
+                for (int c = 0; c < CW; ++c) {
+                    const int pp = p0 + c * NSEQ + sq;
+                    act[c] = pp < npairs;
+                    const int qa = act[c] ? 2 * pp : 0;
+                    hasb[c] = act[c] && (qa + 1) < nrb;
+                    rowa[c] = off + (int64_t)(rb0 + qa + sy) * stride;
+                    rowb[c] = hasb[c] ? rowa[c] + stride : rowa[c];
+                }
+#pragma unroll
+                for (int t0 = 0; t0 < NT; t0 += TB) {
+                    float dv[CW][TB][2], iv[CW][TB][2];
+#pragma unroll
+                    for (int c = 0; c < CW; ++c)
+#pragma unroll
+                        for (int tt = 0; tt < TB; ++tt) {
+                            const int oxc = min(sl + SEQL * (t0 + tt), mw - 1);
+                            dv[c][tt][0] = data[rowa[c] + oxc];
+                            iv[c][tt][0] = inverr[rowa[c] + oxc];
+                            dv[c][tt][1] = data[rowb[c] + oxc];
+                            iv[c][tt][1] = inverr[rowb[c] + oxc];
+                        }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
+#pragma unroll
+                        for (int tt = 0; tt < TB; ++tt) {
+                            const int ox = sl + SEQL * (t0 + tt);
+                            const float2 v = pair_value(lb, min(ox, mw - 1) - sx);
+                            const float ca = (dv[c][tt][0] - v.x * scale) * iv[c][tt][0];
+                            const float cb = (dv[c][tt][1] - v.y * scale) * iv[c][tt][1];
+                            if (act[c] && ox < mw) acc += (double)ca * (double)ca;
+                            if (hasb[c] && ox < mw) acc += (double)cb * (double)cb;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < CW; ++c) {
+                    const int pp = p0 + c * NSEQ + sq;
+                    if (pp >= npairs) continue;
+                    const int qa = 2 * pp;
+                    const bool hasb = (qa + 1) < nrb;
+                    const float2 *lb = lbw + (c * NSEQ + sq) * G::LBUF;
+                    const int oya = rb0 + qa + sy;
+                    for (int ox = sl; ox < mw; ox += SEQL) {
+                        const float2 v = pair_value(lb, ox - sx);
+                        acc += epilogue(mode, v.x, scale, off + (int64_t)oya * stride + ox, out,
+                                        data, inverr);
+                        if (hasb)
+                            acc += epilogue(mode, v.y, scale,
+                                            off + (int64_t)(oya + 1) * stride + ox, out, data,
+                                            inverr);
+                    }
                 }
             }
             wave_sync();

@@ -407,7 +407,10 @@ extern "C" int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jo
         return set_err(TRH_E_ARG, "trh_render_fft_batch: null argument");
     if (njobs <= 0) return TRH_OK;
     const int maxrows = max_rows_for(sz, h->lds_max);
-    return launch_fft_any(h, sz, -1, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out, data,
+    // no output image but chi^2 slots: every job is a chi^2 job (a STORE or
+    // ADD job would need `out`), so the chi^2-specialised kernel runs
+    const int mode = (!out && chi2 && data && inverr) ? TRH_CHI2 : -1;
+    return launch_fft_any(h, sz, mode, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out, data,
                           inverr, chi2, (hipStream_t)stream);
 }
 
