@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/tractor_hip.h"
@@ -26,6 +27,7 @@ using namespace trh;
 #define NFFT 5     // FFT sizes 32, 64, 128, 256, 512
 #define PTB NFFT   // bucket index of point sources
 #define NBUCKET (NFFT + 1)
+#define PREP_TB 128   // threads per block of the job-building kernels
 #define LS_CNT 8   // line-search counters start at cnt[8]
 // launch slots: 0-4 jacobian FFT renders (sz 32..512), 5 jacobian point
 // renders, 6 normal equations, 7 solve, 8-12 line-search FFT renders,
@@ -65,6 +67,12 @@ struct trh_fitter {
     FitProfiles *prof = nullptr;
     trh_fft_job *jobs_jac = nullptr, *jobs_ls = nullptr;
     int32_t *list_jac = nullptr, *list_ls = nullptr, *cnt = nullptr;  // cnt[0..4] jac, [8..12] ls
+    int32_t *neq_order = nullptr;   // sources by decreasing P (normal equations)
+    // render lists without atomics: the job-building kernels write each job's
+    // bucket (bk_*) and per-block bucket counts (bcnt_*), k_fit_lists turns
+    // them into lists and counts
+    int8_t *bk_jac = nullptr, *bk_ls = nullptr;
+    int32_t *bcnt_jac = nullptr, *bcnt_ls = nullptr;
     hipEvent_t ev[6];
     bool have_ev = false;
     // per-launch profiling (trh_fitter_profile): events around every launch
@@ -168,32 +176,100 @@ __global__ void k_fit_reset(int32_t *cnt, int n) {
     if ((int)threadIdx.x < n) cnt[threadIdx.x] = 0;
 }
 
-// Append `slot` to render list b (b < 0: nothing).  Every lane of the wave
-// calls this together: one atomic per bucket present in the wave.
-__device__ __forceinline__ void wave_list_append(int b, int slot, int32_t *__restrict__ lists,
-                                                 int cap, int32_t *__restrict__ cnt) {
-    const int lane = threadIdx.x & 63;
+// Bucket of this thread's job (b < 0: none) into bk[slot], and the block's
+// job count per bucket into bcnt[blockIdx.x][.] (no global atomics: on this
+// GPU, device-scope atomics on one address from thousands of waves
+// serialise).  Every thread of the block calls this.
+__device__ __forceinline__ void block_bucket_counts(int b, int slot, int njobs,
+                                                    int8_t *__restrict__ bk,
+                                                    int32_t *__restrict__ bcnt) {
+    __shared__ int32_t wc[PREP_TB / 64][NBUCKET];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (slot < njobs) bk[slot] = (int8_t)b;
 #pragma unroll
     for (int bb = 0; bb < NBUCKET; ++bb) {
         const unsigned long long m = __ballot(b == bb);
-        if (m == 0ull) continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&cnt[bb], __popcll(m));
-        base = __shfl(base, leader);
-        if (b == bb) lists[bb * cap + base + __popcll(m & ((1ull << lane) - 1ull))] = slot;
+        if (lane == 0) wc[w][bb] = __popcll(m);
     }
+    __syncthreads();
+    if ((int)threadIdx.x < NBUCKET) {
+        int t = 0;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) t += wc[ww][threadIdx.x];
+        bcnt[blockIdx.x * NBUCKET + threadIdx.x] = t;
+    }
+}
+
+// Render lists from the per-job buckets: block k (the job-building kernel's
+// block k) adds up the counts of the blocks before it, which gives its
+// offset in every list, then writes its jobs' slots in order; the last block
+// also writes the list lengths.  Launched with the job-building kernel's grid.
+__global__ void __launch_bounds__(PREP_TB)
+k_fit_lists(int njobs, const int8_t *__restrict__ bk, const int32_t *__restrict__ bcnt,
+            int32_t *__restrict__ lists, int cap, int32_t *__restrict__ cnt) {
+    __shared__ int32_t part[PREP_TB / 64][NBUCKET];
+    __shared__ int32_t base[NBUCKET];
+    const int blk = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int sum[NBUCKET];
+#pragma unroll
+    for (int bb = 0; bb < NBUCKET; ++bb) sum[bb] = 0;
+    for (int i = threadIdx.x; i < blk; i += blockDim.x)
+#pragma unroll
+        for (int bb = 0; bb < NBUCKET; ++bb) sum[bb] += bcnt[i * NBUCKET + bb];
+#pragma unroll
+    for (int bb = 0; bb < NBUCKET; ++bb) {
+        int v = sum[bb];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (lane == 0) part[w][bb] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NBUCKET) {
+        int t = 0;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) t += part[ww][threadIdx.x];
+        base[threadIdx.x] = t;
+        if (blk == (int)gridDim.x - 1) cnt[threadIdx.x] = t + bcnt[blk * NBUCKET + threadIdx.x];
+    }
+    __syncthreads();
+    const int slot = blk * blockDim.x + threadIdx.x;
+    const int b = slot < njobs ? (int)bk[slot] : -1;
+    int rank = 0;
+#pragma unroll
+    for (int bb = 0; bb < NBUCKET; ++bb) {
+        const unsigned long long m = __ballot(b == bb);
+        if (lane == 0) part[w][bb] = __popcll(m);
+        if (b == bb) rank = __popcll(m & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    if (b >= 0) {
+        int off = base[b] + rank;
+        for (int ww = 0; ww < w; ++ww) off += part[ww][b];
+        lists[b * cap + off] = slot;
+    }
+}
+
+// The job-building kernels run PREP_TB threads per block, one job record per
+// thread: the records are assembled in LDS and copied out with coalesced
+// 8-byte stores (a record is 328 bytes; written field by field from each
+// thread, every store instruction would touch 64 cache lines).
+
+__device__ __forceinline__ void copy_jobs_out(const trh_fft_job *sj, trh_fft_job *dst, int count) {
+    static_assert(sizeof(trh_fft_job) % 8 == 0, "8-byte words");
+    constexpr int W = sizeof(trh_fft_job) / 8;
+    const uint2 *s = reinterpret_cast<const uint2 *>(sj);
+    uint2 *d = reinterpret_cast<uint2 *>(dst);
+    for (int i = threadIdx.x; i < count * W; i += blockDim.x) d[i] = s[i];
 }
 
 // One thread per (source, render): render 0 of every active source and
 // renders 1..3 (EllipseE steps) of the galaxies.
-__global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
+__global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
                                const int32_t *__restrict__ Parr, const int64_t *__restrict__ roff,
                                const double *__restrict__ params,
                                const int32_t *__restrict__ active, const FitProfiles *__restrict__ pr,
                                int ph, int pw, trh_fft_job *__restrict__ jobs,
-                               int32_t *__restrict__ lists, int cap, int32_t *__restrict__ cnt,
+                               int8_t *__restrict__ bk, int32_t *__restrict__ bcnt,
                                int32_t *__restrict__ err, unsigned long long *__restrict__ acc) {
+    __shared__ trh_fft_job sj[PREP_TB];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = t >> 2, k = t & 3;
     int b = -1;
@@ -209,7 +285,7 @@ __global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
             atomicAdd(&acc[2 * SL_SOLVE], 1ull);
             atomicAdd(&acc[2 * SL_SOLVE + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
         }
-        trh_fft_job &J = jobs[t];
+        trh_fft_job &J = sj[threadIdx.x];
         if (kd == 0) {
             build_point_job(J, p[0], p[1], P, ph, pw);
             J.out_offset = roff[s];
@@ -231,7 +307,11 @@ __global__ void k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
             atomicAdd(&acc[2 * sl + 1], 4ull * PP);  // render written
         }
     }
-    wave_list_append(b, t, lists, cap, cnt);
+    block_bucket_counts(b, t, 4 * S, bk, bcnt);
+    __syncthreads();
+    // records of threads without a job are copied too (never listed, never read)
+    const int first = blockIdx.x * blockDim.x;
+    copy_jobs_out(sj, jobs + first, min((int)blockDim.x, 4 * S - first));
 }
 
 // Model renders (counts x unit-flux patch at the current parameters) into an
@@ -333,19 +413,22 @@ struct NeqPix {
 };
 
 // J^T W J, J^T W r (FP32 products and sums) and chi^2 (FP64) for one source
-// per 256-thread block (grid-stride over sources).  Each thread takes four
-// pixels per step so their loads are in flight together; the wave sums use
-// DPP/swizzle exchanges and the four waves combine through LDS.
+// per 256-thread block, grid-stride over the sources in order of decreasing
+// P (sources span 64x in pixels; big ones first keeps the tail short).  Each
+// thread takes four pixels per step so their loads are in flight together;
+// the wave sums use DPP/swizzle exchanges and the four waves combine through
+// LDS.
 __global__ void __launch_bounds__(256)
 k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
                 const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
                 const float *__restrict__ data, const float *__restrict__ inverr,
                 const float *__restrict__ renders, const double *__restrict__ params,
                 const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
-                double cdi11, double *__restrict__ neq) {
+                double cdi11, double *__restrict__ neq, const int32_t *__restrict__ order) {
     __shared__ double red[4][28];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+    for (int i = blockIdx.x; i < S; i += gridDim.x) {
+        const int s = order[i];
         if (!active[s]) continue;
         NeqPix px;
         px.kd = kind[s];
@@ -363,7 +446,7 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         px.E = inverr + pix_off[s];
         float acc[27];
 #pragma unroll
-        for (int i = 0; i < 27; ++i) acc[i] = 0.f;
+        for (int k = 0; k < 27; ++k) acc[k] = 0.f;
         double chi2 = 0.0;
         int idx = threadIdx.x;
         for (; idx + 768 < px.PP; idx += 1024) {
@@ -374,11 +457,11 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         }
         for (; idx < px.PP; idx += 256) px.add(idx, acc, chi2);
 #pragma unroll
-        for (int i = 0; i < 27; ++i) acc[i] = wave_allsum(acc[i]);
+        for (int k = 0; k < 27; ++k) acc[k] = wave_allsum(acc[k]);
         chi2 = wave_allsum_d(chi2);
         if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < 27; ++i) red[wid][i] = (double)acc[i];
+            for (int k = 0; k < 27; ++k) red[wid][k] = (double)acc[k];
             red[wid][27] = chi2;
         }
         __syncthreads();
@@ -505,16 +588,17 @@ __global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
 }
 
 // One thread per (source, trial step) of the line search.
-__global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
+__global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
                               const int32_t *__restrict__ Parr,
                               const int64_t *__restrict__ pix_off,
                               const double *__restrict__ trial,
                               const int32_t *__restrict__ nsteps,
                               const int32_t *__restrict__ active,
                               const FitProfiles *__restrict__ pr, int ph, int pw,
-                              trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
-                              int cap, int32_t *__restrict__ cnt, int32_t *__restrict__ err,
+                              trh_fft_job *__restrict__ jobs, int8_t *__restrict__ bk,
+                              int32_t *__restrict__ bcnt, int32_t *__restrict__ err,
                               unsigned long long *__restrict__ acc) {
+    __shared__ trh_fft_job sj[PREP_TB];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slot / NALPHA, a = slot - s * NALPHA;
     int b = -1;
@@ -522,7 +606,7 @@ __global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
         const int kd = kind[s];
         const int P = Parr[s];
         const double *p = trial + (size_t)slot * 8;
-        trh_fft_job &J = jobs[slot];
+        trh_fft_job &J = sj[threadIdx.x];
         if (kd == 0) {
             build_point_job(J, p[0], p[1], P, ph, pw);
             b = PTB;
@@ -541,7 +625,10 @@ __global__ void k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
             atomicAdd(&acc[2 * sl + 1], 8ull * P * P);  // data + inverr read
         }
     }
-    wave_list_append(b, slot, lists, cap, cnt);
+    block_bucket_counts(b, slot, NALPHA * S, bk, bcnt);
+    __syncthreads();
+    const int first = blockIdx.x * blockDim.x;
+    copy_jobs_out(sj, jobs + first, min((int)blockDim.x, NALPHA * S - first));
 }
 
 // tryUpdates acceptance (constrained_optimizer.py:36-82) and the optimize_loop
@@ -716,7 +803,11 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         (rc = falloc(f, &f->err, 1)) || (rc = falloc(f, &f->prof, 1)) ||
         (rc = falloc(f, &f->jobs_jac, (size_t)cap_jac)) || (rc = falloc(f, &f->jobs_ls, (size_t)cap_ls)) ||
         (rc = falloc(f, &f->list_jac, (size_t)NBUCKET * cap_jac)) ||
-        (rc = falloc(f, &f->list_ls, (size_t)NBUCKET * cap_ls)) || (rc = falloc(f, &f->cnt, 16))) {
+        (rc = falloc(f, &f->list_ls, (size_t)NBUCKET * cap_ls)) || (rc = falloc(f, &f->cnt, 16)) ||
+        (rc = falloc(f, &f->neq_order, S)) || (rc = falloc(f, &f->bk_jac, (size_t)cap_jac)) ||
+        (rc = falloc(f, &f->bk_ls, (size_t)cap_ls)) ||
+        (rc = falloc(f, &f->bcnt_jac, (size_t)NBUCKET * ((cap_jac + PREP_TB - 1) / PREP_TB))) ||
+        (rc = falloc(f, &f->bcnt_ls, (size_t)NBUCKET * ((cap_ls + PREP_TB - 1) / PREP_TB)))) {
         trh_fitter_destroy(f);
         return rc;
     }
@@ -729,10 +820,14 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
     }
     FitProfiles pr;
     host_profiles(pr);
-    std::vector<int32_t> ones(S, 1), zeros(S, 0);
+    std::vector<int32_t> ones(S, 1), zeros(S, 0), order(S);
+    for (int s = 0; s < S; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P[a] > P[b]; });
     const int32_t zero = 0;
     if ((rc = hip_check(hipMemcpy(f->kind, kind, 4 * (size_t)S, hipMemcpyHostToDevice), "copy kind")) ||
         (rc = hip_check(hipMemcpy(f->P, P, 4 * (size_t)S, hipMemcpyHostToDevice), "copy P")) ||
+        (rc = hip_check(hipMemcpy(f->neq_order, order.data(), 4 * (size_t)S, hipMemcpyHostToDevice),
+                        "copy order")) ||
         (rc = hip_check(hipMemcpy(f->pix_off, pix_off, 8 * (size_t)S, hipMemcpyHostToDevice), "copy pix_off")) ||
         (rc = hip_check(hipMemcpy(f->render_off, roff.data(), 8 * (size_t)S, hipMemcpyHostToDevice), "copy roff")) ||
         (rc = hip_check(hipMemcpy(f->stamp, psf_stamp, 4 * (size_t)ph * pw, hipMemcpyHostToDevice), "copy stamp")) ||
@@ -824,10 +919,14 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if (!capture) (void)hipEventRecord(f->ev[0], st);
     SLOT_BEGIN(SL_PREP);
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
-    hipLaunchKernelGGL(k_fit_prep_jac, dim3((4 * S + tb - 1) / tb), dim3(tb), 0, st, S, f->kind, f->P, f->render_off,
-                       f->params, f->active, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
-                       cap_jac, f->cnt, f->err, acc);
+    const int nb_jac = (cap_jac + PREP_TB - 1) / PREP_TB, nb_ls = (cap_ls + PREP_TB - 1) / PREP_TB;
+    hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb_jac), dim3(PREP_TB), 0, st, S, f->kind, f->P,
+                       f->render_off, f->params, f->active, f->prof, f->ph, f->pw, f->jobs_jac,
+                       f->bk_jac, f->bcnt_jac, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
+    hipLaunchKernelGGL(k_fit_lists, dim3(nb_jac), dim3(PREP_TB), 0, st, cap_jac, f->bk_jac,
+                       f->bcnt_jac, f->list_jac, cap_jac, f->cnt);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
     SLOT_END(SL_PREP);
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         SLOT_BEGIN(b);
@@ -845,10 +944,11 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if (!capture) (void)hipEventRecord(f->ev[1], st);
     SLOT_BEGIN(SL_NEQ);
     {
-        int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
+        const int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
         hipLaunchKernelGGL(k_fit_normal_eq, dim3(grid), dim3(256), 0, st, S, f->kind, f->P,
                            f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
-                           f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq);
+                           f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq,
+                           f->neq_order);
         if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq"))) return rc;
     }
     SLOT_END(SL_NEQ);
@@ -859,10 +959,13 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if ((rc = hip_check(hipGetLastError(), "k_fit_solve"))) return rc;
     SLOT_END(SL_SOLVE);
     if (!capture) (void)hipEventRecord(f->ev[3], st);
-    hipLaunchKernelGGL(k_fit_prep_ls, dim3((NALPHA * S + tb - 1) / tb), dim3(tb), 0, st, S, f->kind, f->P, f->pix_off,
-                       f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw, f->jobs_ls,
-                       f->list_ls, cap_ls, f->cnt + LS_CNT, f->err, acc);
+    hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb_ls), dim3(PREP_TB), 0, st, S, f->kind, f->P,
+                       f->pix_off, f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw,
+                       f->jobs_ls, f->bk_ls, f->bcnt_ls, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
+    hipLaunchKernelGGL(k_fit_lists, dim3(nb_ls), dim3(PREP_TB), 0, st, cap_ls, f->bk_ls,
+                       f->bcnt_ls, f->list_ls, cap_ls, f->cnt + LS_CNT);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         SLOT_BEGIN(SL_LS + b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
