@@ -201,6 +201,7 @@ int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n, i
  * coordinates of the stamp's pixel (0,0); (mux, muy) = the sub-pixel shift
  * px - round(px); psf selects the stamp at stamp + psf * ph * pw; K, amp,
  * var unused.  stamp: float32 [nstamp][ph][pw]; maxw >= every job's mw.
+ * With out NULL and chi2, data, inverr given, every job is a TRH_CHI2 job.
  */
 int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, int maxw,
                            const float *stamp, int ph, int pw, float *out, const float *data,

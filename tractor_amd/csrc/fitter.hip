@@ -252,12 +252,16 @@ k_fit_lists(int njobs, const int8_t *__restrict__ bk, const int32_t *__restrict_
 // 8-byte stores (a record is 328 bytes; written field by field from each
 // thread, every store instruction would touch 64 cache lines).
 
-__device__ __forceinline__ void copy_jobs_out(const trh_fft_job *sj, trh_fft_job *dst, int count) {
+// Copy the staged records whose flag is set (jobs that were built) to
+// dst[0 .. count): consecutive threads write consecutive 8-byte words.
+__device__ __forceinline__ void copy_jobs_out(const trh_fft_job *sj, const int8_t *built,
+                                              trh_fft_job *dst, int count) {
     static_assert(sizeof(trh_fft_job) % 8 == 0, "8-byte words");
     constexpr int W = sizeof(trh_fft_job) / 8;
     const uint2 *s = reinterpret_cast<const uint2 *>(sj);
     uint2 *d = reinterpret_cast<uint2 *>(dst);
-    for (int i = threadIdx.x; i < count * W; i += blockDim.x) d[i] = s[i];
+    for (int i = threadIdx.x; i < count * W; i += blockDim.x)
+        if (built[i / W]) d[i] = s[i];
 }
 
 // One thread per (source, render): render 0 of every active source and
@@ -307,11 +311,12 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, const int32_t *
             atomicAdd(&acc[2 * sl + 1], 4ull * PP);  // render written
         }
     }
+    __shared__ int8_t built[PREP_TB];
+    built[threadIdx.x] = b >= 0;
     block_bucket_counts(b, t, 4 * S, bk, bcnt);
     __syncthreads();
-    // records of threads without a job are copied too (never listed, never read)
     const int first = blockIdx.x * blockDim.x;
-    copy_jobs_out(sj, jobs + first, min((int)blockDim.x, 4 * S - first));
+    copy_jobs_out(sj, built, jobs + first, min((int)blockDim.x, 4 * S - first));
 }
 
 // Model renders (counts x unit-flux patch at the current parameters) into an
@@ -413,19 +418,20 @@ struct NeqPix {
 };
 
 // J^T W J, J^T W r (FP32 products and sums) and chi^2 (FP64) for one source
-// per 256-thread block, grid-stride over the sources in order of decreasing
+// per NEQ_TB-thread block, grid-stride over the sources in order of decreasing
 // P (sources span 64x in pixels; big ones first keeps the tail short).  Each
 // thread takes four pixels per step so their loads are in flight together;
-// the wave sums use DPP/swizzle exchanges and the four waves combine through
-// LDS.
-__global__ void __launch_bounds__(256)
+// the wave sums use DPP/swizzle exchanges and the waves combine through LDS.
+#define NEQ_TB 512   // threads per source in k_fit_normal_eq
+
+__global__ void __launch_bounds__(NEQ_TB)
 k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
                 const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
                 const float *__restrict__ data, const float *__restrict__ inverr,
                 const float *__restrict__ renders, const double *__restrict__ params,
                 const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
                 double cdi11, double *__restrict__ neq, const int32_t *__restrict__ order) {
-    __shared__ double red[4][28];
+    __shared__ double red[NEQ_TB / 64][28];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = blockIdx.x; i < S; i += gridDim.x) {
         const int s = order[i];
@@ -449,13 +455,13 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         for (int k = 0; k < 27; ++k) acc[k] = 0.f;
         double chi2 = 0.0;
         int idx = threadIdx.x;
-        for (; idx + 768 < px.PP; idx += 1024) {
+        for (; idx + 3 * NEQ_TB < px.PP; idx += 4 * NEQ_TB) {
             px.add(idx, acc, chi2);
-            px.add(idx + 256, acc, chi2);
-            px.add(idx + 512, acc, chi2);
-            px.add(idx + 768, acc, chi2);
+            px.add(idx + NEQ_TB, acc, chi2);
+            px.add(idx + 2 * NEQ_TB, acc, chi2);
+            px.add(idx + 3 * NEQ_TB, acc, chi2);
         }
-        for (; idx < px.PP; idx += 256) px.add(idx, acc, chi2);
+        for (; idx < px.PP; idx += NEQ_TB) px.add(idx, acc, chi2);
 #pragma unroll
         for (int k = 0; k < 27; ++k) acc[k] = wave_allsum(acc[k]);
         chi2 = wave_allsum_d(chi2);
@@ -476,34 +482,37 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
 
 // Column-scaled damped Cholesky solve (FP64) + getParameterSteps
 // (constrained_optimizer.py:84-159).  One thread per source.
-__global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
-                            const double *__restrict__ neq, const double *__restrict__ params,
-                            const int32_t *__restrict__ active, double lambda,
-                            double *__restrict__ X, double *__restrict__ trial,
-                            int32_t *__restrict__ nsteps, int32_t *__restrict__ steplim) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    if (!active[s]) {
-        nsteps[s] = 0;
-        return;
-    }
-    const int np = kind[s] ? 6 : 3;
-    const double *q = neq + 32 * s;
-    double A[6][6], g[6], D[6];
-    int t = 0;
-    for (int i = 0; i < 6; ++i)
-        for (int j = i; j < 6; ++j) {
-            A[i][j] = q[t];
-            A[j][i] = q[t];
-            ++t;
+// Column-scaled, damped FP64 Cholesky solve of one source's NP x NP normal
+// equations (q: the 21 upper-triangle entries of the 6 x 6 matrix, then the
+// 6 gradient entries), then the line-search steps (getParameterSteps).  NP is
+// a compile-time constant so every array stays in registers.
+template <int NP>
+__device__ __forceinline__ void solve_source(int s, const double *__restrict__ q,
+                                             const double *__restrict__ p0, double lambda,
+                                             double *__restrict__ X, double *__restrict__ trial,
+                                             int32_t *__restrict__ nsteps,
+                                             int32_t *__restrict__ steplim) {
+    double A[NP][NP], g[NP], D[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+        for (int j = i; j < NP; ++j) {
+            // upper-triangle index of (i, j) in the 6 x 6 packing
+            const double v = q[i * 6 - i * (i - 1) / 2 + (j - i)];
+            A[i][j] = v;
+            A[j][i] = v;
         }
-    for (int i = 0; i < 6; ++i) g[i] = q[21 + i];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) g[i] = q[21 + i];
     bool any = false;
-    for (int i = 0; i < np; ++i) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
         D[i] = sqrt(A[i][i]);
         if (D[i] > 0.0) any = true;
     }
-    double x[6] = {0, 0, 0, 0, 0, 0};
+    double x[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) x[i] = 0.0;
     if (!any) {
         // all derivatives zero: no update (gpu_optimizer.py:762-767)
         nsteps[s] = 0;
@@ -513,10 +522,12 @@ __global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
     }
     double lam = lambda;
     for (int attempt = 0; attempt < 12; ++attempt) {
-        double L[6][6], rhs[6];
+        double L[NP][NP], rhs[NP];
         bool ok = true;
-        for (int i = 0; i < np; ++i) {
-            for (int j = 0; j < np; ++j) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
                 if (D[i] > 0.0 && D[j] > 0.0)
                     L[i][j] = A[i][j] / (D[i] * D[j]);
                 else
@@ -525,50 +536,60 @@ __global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
             if (D[i] > 0.0) L[i][i] += lam;
             rhs[i] = D[i] > 0.0 ? g[i] / D[i] : 0.0;
         }
-        for (int j = 0; j < np && ok; ++j) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
             double d = L[j][j];
+#pragma unroll
             for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
-            if (!(d > 1e-300)) {
-                ok = false;
-                break;
-            }
-            d = sqrt(d);
+            if (!(d > 1e-300)) ok = false;
+            d = sqrt(fmax(d, 1e-300));
             L[j][j] = d;
-            for (int i = j + 1; i < np; ++i) {
+#pragma unroll
+            for (int i = j + 1; i < NP; ++i) {
                 double v = L[i][j];
+#pragma unroll
                 for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
                 L[i][j] = v / d;
             }
         }
         if (ok) {
-            double y[6];
-            for (int i = 0; i < np; ++i) {
+            double y[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
                 double v = rhs[i];
+#pragma unroll
                 for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
                 y[i] = v / L[i][i];
             }
-            for (int i = np - 1; i >= 0; --i) {
+#pragma unroll
+            for (int i = NP - 1; i >= 0; --i) {
                 double v = y[i];
-                for (int k = i + 1; k < np; ++k) v -= L[k][i] * x[k];
+#pragma unroll
+                for (int k = i + 1; k < NP; ++k) v -= L[k][i] * x[k];
                 x[i] = v / L[i][i];
             }
-            for (int i = 0; i < np; ++i) x[i] = D[i] > 0.0 ? x[i] / D[i] : 0.0;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) x[i] = D[i] > 0.0 ? x[i] / D[i] : 0.0;
             break;
         }
         lam = lam > 0.0 ? lam * 10.0 : 1e-12;
     }
-    for (int i = 0; i < 8; ++i) X[8 * s + i] = i < np ? x[i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) X[8 * s + i] = i < NP ? x[i < NP ? i : 0] : 0.0;
 
     // getParameterSteps: maxstep PixPos [1,1] (wcs.py:384), Flux none,
     // EllipseESoft [1, 0.1, 0.1] (ellipses.py:48); no bounds for ESoft.
     const double maxstep[6] = {1.0, 1.0, -1.0, 1.0, 0.1, 0.1};
-    const double *p0 = params + 8 * s;
+    double p[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) p[i] = p0[i];
     int n = 0;
     int anylim = 0;
     for (int a = 0; a < NALPHA; ++a) {
         double alpha = a <= 10 ? ldexp(1.0, a - 10) : (a == 11 ? sqrt(2.0) : 2.0);
         bool do_break = false;
-        for (int i = 0; i < np; ++i) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
             if (maxstep[i] < 0.0) continue;
             const double step = alpha * fabs(x[i]);
             if (step > maxstep[i]) {
@@ -579,12 +600,30 @@ __global__ void k_fit_solve(int S, const int32_t *__restrict__ kind,
         }
         if (alpha < 1e-8) break;
         double *tp = trial + ((size_t)s * NALPHA + n) * 8;
-        for (int i = 0; i < 8; ++i) tp[i] = i < np ? p0[i] + alpha * x[i] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) tp[i] = i < NP ? p[i < NP ? i : 0] + alpha * x[i < NP ? i : 0] : 0.0;
         ++n;
         if (do_break) break;
     }
     nsteps[s] = n;
     steplim[s] = anylim;
+}
+
+__global__ void __launch_bounds__(64) k_fit_solve(int S, const int32_t *__restrict__ kind,
+                            const double *__restrict__ neq, const double *__restrict__ params,
+                            const int32_t *__restrict__ active, double lambda,
+                            double *__restrict__ X, double *__restrict__ trial,
+                            int32_t *__restrict__ nsteps, int32_t *__restrict__ steplim) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    if (!active[s]) {
+        nsteps[s] = 0;
+        return;
+    }
+    if (kind[s])
+        solve_source<6>(s, neq + 32 * s, params + 8 * s, lambda, X, trial, nsteps, steplim);
+    else
+        solve_source<3>(s, neq + 32 * s, params + 8 * s, lambda, X, trial, nsteps, steplim);
 }
 
 // One thread per (source, trial step) of the line search.
@@ -625,10 +664,12 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
             atomicAdd(&acc[2 * sl + 1], 8ull * P * P);  // data + inverr read
         }
     }
+    __shared__ int8_t built[PREP_TB];
+    built[threadIdx.x] = b >= 0;
     block_bucket_counts(b, slot, NALPHA * S, bk, bcnt);
     __syncthreads();
     const int first = blockIdx.x * blockDim.x;
-    copy_jobs_out(sj, jobs + first, min((int)blockDim.x, NALPHA * S - first));
+    copy_jobs_out(sj, built, jobs + first, min((int)blockDim.x, NALPHA * S - first));
 }
 
 // tryUpdates acceptance (constrained_optimizer.py:36-82) and the optimize_loop
@@ -897,8 +938,14 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
     if (wg > 8) wg = 8;
     int grid = f->h->n_cu * wg;
     if (bound < grid) grid = bound;
-    hipLaunchKernelGGL(k_render_point<0>, dim3(grid), dim3(256), lds, st, jobs, list, count, bound,
-                       f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data, f->inverr, chi2);
+    if (chi2)
+        hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds, st, jobs, list,
+                           count, bound, f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data,
+                           f->inverr, chi2);
+    else
+        hipLaunchKernelGGL(k_render_point<TRH_STORE>, dim3(grid), dim3(256), lds, st, jobs, list,
+                           count, bound, f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data,
+                           f->inverr, chi2);
     return hip_check(hipGetLastError(), "k_render_point");
 }
 
@@ -945,7 +992,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     SLOT_BEGIN(SL_NEQ);
     {
         const int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
-        hipLaunchKernelGGL(k_fit_normal_eq, dim3(grid), dim3(256), 0, st, S, f->kind, f->P,
+        hipLaunchKernelGGL(k_fit_normal_eq, dim3(grid), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
                            f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
                            f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq,
                            f->neq_order);

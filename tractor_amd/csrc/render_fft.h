@@ -649,7 +649,8 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 // modelMask, tractor/psf.py:137-190): the stamp, its (0,0) pixel placed at
 // window coordinates (sx, sy), Lanczos-3 shifted by (mux, muy); stamp, tmp
 // in LDS: tmp holds ph x mw floats.
-static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
+template <int MODE>
+__device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
                                  const float *__restrict__ stamp, bool load_stamp, int ph, int pw,
                                  float *lstamp, float *tmp, RenderScratch &S,
                                  float *__restrict__ out, const float *__restrict__ data,
@@ -657,7 +658,8 @@ static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int64_t off = Jp->out_offset;
     const int stride = Jp->out_stride;
-    const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy, mode = Jp->mode;
+    const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy;
+    const int mode = MODE >= 0 ? MODE : Jp->mode;
     const float scale = Jp->scale;
     compute_taps(Jp->mux, Jp->muy, S);
     const float *fx = S.taps;
@@ -682,16 +684,44 @@ static __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     }
     __syncthreads();
     double acc = 0.0;
-    for (int idx = tid; idx < mh * mw; idx += nthr) {
-        const int oy = idx / mw;
-        const int ox = idx - oy * mw;
+    const int n = mh * mw;
+    auto yval = [&](int oy, int ox) -> float {
         float a = 0.f;
 #pragma unroll
         for (int t = 0; t < 7; ++t) {
             const int r = oy - sy - 3 + t;
             if (r >= 0 && r < ph) a += fy[t] * tmp[r * mw + ox];
         }
-        acc += epilogue(mode, a, scale, off + (int64_t)oy * stride + ox, out, data, inverr);
+        return a;
+    };
+    if (MODE == TRH_CHI2) {
+        // four pixels per thread per step, their data / inverr loads issued
+        // together (indices clamped to the last pixel, extra terms dropped)
+        for (int base = tid; base < n; base += 4 * nthr) {
+            float dv[4], iv[4];
+            int oyv[4], oxv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idc = min(base + u * nthr, n - 1);
+                oyv[u] = idc / mw;
+                oxv[u] = idc - oyv[u] * mw;
+                const int64_t g = off + (int64_t)oyv[u] * stride + oxv[u];
+                dv[u] = data[g];
+                iv[u] = inverr[g];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float chi = (dv[u] - yval(oyv[u], oxv[u]) * scale) * iv[u];
+                if (base + u * nthr < n) acc += (double)chi * (double)chi;
+            }
+        }
+    } else {
+        for (int idx = tid; idx < n; idx += nthr) {
+            const int oy = idx / mw;
+            const int ox = idx - oy * mw;
+            acc += epilogue(mode, yval(oy, ox), scale, off + (int64_t)oy * stride + ox, out, data,
+                            inverr);
+        }
     }
     if (mode == TRH_CHI2) {
         const double tot = block_sum(acc, S.red);
@@ -745,7 +775,8 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
     }
 }
 
-template <int UNUSED = 0>
+// MODE: TRH_STORE / TRH_CHI2 fixed at compile time, or -1 (read from each job).
+template <int MODE>
 __global__ void __launch_bounds__(256)
 k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
                const int *__restrict__ count, int njobs, const float *__restrict__ stamp,
@@ -763,8 +794,8 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
         if (jobs[jid].mw > maxw) continue;  // host guarantees maxw >= every mw
         // the job's psf field selects one of the stamps at stamp + psf * ph * pw
         const int psf = jobs[jid].psf;
-        render_point_one(jobs + jid, stamp + (size_t)psf * ph * pw, psf != cur, ph, pw, lstamp,
-                         tmp, S, out, data, inverr, chi2);
+        render_point_one<MODE>(jobs + jid, stamp + (size_t)psf * ph * pw, psf != cur, ph, pw,
+                               lstamp, tmp, S, out, data, inverr, chi2);
         cur = psf;
     }
 }

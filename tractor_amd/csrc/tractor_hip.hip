@@ -586,8 +586,15 @@ extern "C" int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, in
         return set_err(TRH_E_UNSUPPORTED, "point render: stamp %dx%d with window width %d exceeds LDS",
                        ph, pw, maxw);
     int grid = njobs < h->n_cu * 4 ? njobs : h->n_cu * 4;
-    hipLaunchKernelGGL(k_render_point<0>, dim3(grid), dim3(256), lds, (hipStream_t)stream, jobs,
-                       nullptr, nullptr, njobs, stamp, ph, pw, maxw, out, data, inverr, chi2);
+    // no output image but chi^2 slots: every job is a chi^2 job
+    if (!out && chi2 && data && inverr)
+        hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds,
+                           (hipStream_t)stream, jobs, nullptr, nullptr, njobs, stamp, ph, pw, maxw,
+                           out, data, inverr, chi2);
+    else
+        hipLaunchKernelGGL(k_render_point<-1>, dim3(grid), dim3(256), lds, (hipStream_t)stream,
+                           jobs, nullptr, nullptr, njobs, stamp, ph, pw, maxw, out, data, inverr,
+                           chi2);
     return hip_check(hipGetLastError(), "k_render_point");
 }
 
