@@ -114,6 +114,10 @@ def main():
                     help='only run three 256 MiB device-to-device copies (HBM counter '
                          'calibration under rocprofv3 --pmc)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-slot-profile', action='store_true',
+                    help='skip the extra per-launch-timed step after the timed region (its '
+                         'event timing and byte counters slow that step down; use this under '
+                         'rocprofv3 so the trace holds the timed fit only); roofline is then null')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--dchisq', type=float, default=0.,
                     help='optimize_loop stopping tolerance on dlnp (reference default 0)')
@@ -153,10 +157,12 @@ def main():
     dt = (t1 - t0) / args.steps
     # one more step, outside the timed region, with per-kernel HIP-event
     # timing (it synchronises every iteration) for the roofline figures
-    f.profile(True)
-    run_step(wl, torch, args.dchisq, graph=False)
-    prof = f.profile_read()
-    f.profile(False)
+    prof = None
+    if not args.no_slot_profile:
+        f.profile(True)
+        run_step(wl, torch, args.dchisq, graph=False)
+        prof = f.profile_read()
+        f.profile(False)
     value = wl['n'] / dt
     st = f.status()
     log('per-step iterations', iters, 'max per-source iterations', int(st['iterations'].max()))
@@ -164,11 +170,13 @@ def main():
     log('per-source iteration histogram (iterations: sources):',
         {int(i): int(c) for i, c in enumerate(hist) if c})
 
-    slots = {SLOT_NAMES[i]: dict(ms=round(float(prof[i, 0]), 3), launches=int(prof[i, 1]),
-                                 jobs=int(prof[i, 2]), gbytes=round(float(prof[i, 3]) / 1e9, 4),
-                                 gbs=round(float(prof[i, 3]) / max(float(prof[i, 0]), 1e-9) / 1e6, 1))
-             for i in range(len(SLOT_NAMES)) if prof[i, 1] > 0}
-    log('per-slot totals over one profiled step:', json.dumps(slots))
+    if prof is not None:
+        slots = {SLOT_NAMES[i]: dict(ms=round(float(prof[i, 0]), 3), launches=int(prof[i, 1]),
+                                     jobs=int(prof[i, 2]), gbytes=round(float(prof[i, 3]) / 1e9, 4),
+                                     gbs=round(float(prof[i, 3]) / max(float(prof[i, 0]), 1e-9) / 1e6,
+                                               1))
+                 for i in range(len(SLOT_NAMES)) if prof[i, 1] > 0}
+        log('per-slot totals over one profiled step:', json.dumps(slots))
 
     cpu = None
     parity = None
@@ -212,7 +220,7 @@ def main():
             'dchisq': args.dchisq,
             'parallelism': 'single GPU',
         },
-        'roofline': roofline_of(prof),
+        'roofline': roofline_of(prof) if prof is not None else None,
         'cpu_baseline': cpu,
     }
     if parity is not None:
