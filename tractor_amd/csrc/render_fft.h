@@ -404,9 +404,10 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
         for (int i0 = wave * PER; i0 < NV; i0 += nw * PER) {
             float2 x[CW][R];
             // MoG Fourier transform: components outermost, so each one is read
-            // from LDS once for all CW x R frequencies of the lane
+            // from LDS once for all CW x R frequencies of the lane.  The PSF
+            // spectrum loads are issued first; the exp work covers their latency.
             float v2[CW], vw[CW][R], F[CW][R], w[R], w2[R];
-            int ic[CW];
+            float2 pv[CW][R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int j = sl + SEQL * r;
@@ -416,8 +417,11 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
                 const int i = i0 + c * NSEQ + sq;
-                ic[c] = i < NV ? i : NV - 1;
-                const float v = (float)ic[c] * (1.0f / SZ);
+                const int ic = i < NV ? i : NV - 1;
+                const float2 *__restrict__ Pc = P + (size_t)ic * SZ;
+#pragma unroll
+                for (int r = 0; r < R; ++r) pv[c][r] = Pc[sl + SEQL * r];
+                const float v = (float)ic * (1.0f / SZ);
                 v2[c] = v * v;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -436,10 +440,9 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
             }
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
-                const float2 *__restrict__ Pc = P + (size_t)ic[c] * SZ;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    float2 p = Pc[sl + SEQL * r];
+                    float2 p = pv[c][r];
                     if constexpr (LREG) p = cmul(p, Ly[r]);
                     else if constexpr (FD) p = cmul(p, ltab[sl + SEQL * r]);
                     x[c][r] = make_float2(F[c][r] * p.x, F[c][r] * p.y);
