@@ -354,16 +354,30 @@ __global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
     lists[b * cap + i] = 4 * s;
 }
 
-// Sum over the 64 lanes of a wave, result in every lane (xor butterfly:
-// DPP moves for spans 1..8, ds_swizzle for 16, ds_bpermute for 32).
-__device__ __forceinline__ float wave_allsum(float v) {
-    v += xlane<1>(v);
-    v += xlane<2>(v);
-    v += xlane<4>(v);
-    v += xlane<8>(v);
-    v += xlane<16>(v);
-    v += xlane<32>(v);
-    return v;
+// Sums over the 64 lanes of 32 per-lane values, scattered: afterwards lane l
+// holds the sum of value q(l) = 16 b5 + 8 b4 + 4 b3 + 2 b2 + b1 (b_k = bit k of
+// l; lanes l and l ^ 1 hold the same sum).  Each step keeps the half of the
+// values selected by one lane bit and adds the partner's copy of it: 31
+// exchanges instead of 32 x 6 for 32 separate all-sums.
+__device__ __forceinline__ float wave_reduce_scatter32(const float (&v)[32]) {
+    const int lane = threadIdx.x & 63;
+    float a[16], b[8], c[4], d[2];
+    bool up = (lane & 32) != 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        a[i] = (up ? v[16 + i] : v[i]) + xlane<32>(up ? v[i] : v[16 + i]);
+    up = (lane & 16) != 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = (up ? a[8 + i] : a[i]) + xlane<16>(up ? a[i] : a[8 + i]);
+    up = (lane & 8) != 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = (up ? b[4 + i] : b[i]) + xlane<8>(up ? b[i] : b[4 + i]);
+    up = (lane & 4) != 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) d[i] = (up ? c[2 + i] : c[i]) + xlane<4>(up ? c[i] : c[2 + i]);
+    up = (lane & 2) != 0;
+    float e = (up ? d[1] : d[0]) + xlane<2>(up ? d[0] : d[1]);
+    return e + xlane<1>(e);
 }
 
 __device__ __forceinline__ double wave_allsum_d(double v) {
@@ -421,7 +435,8 @@ struct NeqPix {
 // per NEQ_TB-thread block, grid-stride over the sources in order of decreasing
 // P (sources span 64x in pixels; big ones first keeps the tail short).  Each
 // thread takes four pixels per step so their loads are in flight together;
-// the wave sums use DPP/swizzle exchanges and the waves combine through LDS.
+// each wave reduce-scatters its 27 sums (wave_reduce_scatter32) and the waves
+// combine through LDS.
 #define NEQ_TB 512   // threads per source in k_fit_normal_eq
 
 __global__ void __launch_bounds__(NEQ_TB)
@@ -462,14 +477,15 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
             px.add(idx + 3 * NEQ_TB, acc, chi2);
         }
         for (; idx < px.PP; idx += NEQ_TB) px.add(idx, acc, chi2);
+        float v32[32];
 #pragma unroll
-        for (int k = 0; k < 27; ++k) acc[k] = wave_allsum(acc[k]);
+        for (int k = 0; k < 32; ++k) v32[k] = k < 27 ? acc[k] : 0.f;
+        const float sum = wave_reduce_scatter32(v32);
+        const int q = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 +
+                      ((lane >> 2) & 1) * 2 + ((lane >> 1) & 1);
         chi2 = wave_allsum_d(chi2);
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 27; ++k) red[wid][k] = (double)acc[k];
-            red[wid][27] = chi2;
-        }
+        if (!(lane & 1) && q < 27) red[wid][q] = (double)sum;
+        if (lane == 0) red[wid][27] = chi2;
         __syncthreads();
         if (threadIdx.x < 28) {
             double t = 0.0;

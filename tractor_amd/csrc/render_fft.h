@@ -677,25 +677,32 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     if (load_stamp)
         for (int i = tid; i < ph * pw; i += nthr) lstamp[i] = stamp[i];
     __syncthreads();
-    // x pass over the stamp rows the y taps reach, [rlo, rhi):
-    // tmp[r][ox] = sum_u fx[u] stamp[r][ox-sx-3+u]
+    // The shifted stamp covers window columns [cx0, cx1) and rows [ry0, ry1)
+    // (taps reach 3 pixels past it); the model is exactly 0 elsewhere.
+    const int cx0 = max(0, sx - 3), cx1 = min(mw, sx + pw + 3);
+    const int ry0 = max(0, sy - 3), ry1 = min(mh, sy + ph + 3);
+    const int cw = cx1 - cx0;
+    // x pass over the stamp rows the y taps reach, [rlo, rhi), and the covered
+    // columns: tmp[r][ox] = sum_u fx[u] stamp[r][ox-sx-3+u]
     const int rlo = max(0, -sy - 3), rhi = min(ph, mh - sy + 3);
-    for (int idx = tid + rlo * mw; idx < rhi * mw; idx += nthr) {
-        const int r = idx / mw;
-        const int ox = idx - r * mw;
+    for (int idx = tid; cw > 0 && idx < (rhi - rlo) * cw; idx += nthr) {
+        const int rr = idx / cw;
+        const int r = rlo + rr;
+        const int ox = cx0 + (idx - rr * cw);
         float a = 0.f;
 #pragma unroll
         for (int u = 0; u < 7; ++u) {
             const int c = ox - sx - 3 + u;
             if (c >= 0 && c < pw) a += fx[u] * lstamp[r * pw + c];
         }
-        tmp[idx] = a;
+        tmp[r * mw + ox] = a;
     }
     __syncthreads();
     double acc = 0.0;
     const int n = mh * mw;
     auto yval = [&](int oy, int ox) -> float {
         float a = 0.f;
+        if (ox < cx0 || ox >= cx1 || oy < ry0 || oy >= ry1) return a;
 #pragma unroll
         for (int t = 0; t < 7; ++t) {
             const int r = oy - sy - 3 + t;
