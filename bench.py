@@ -407,9 +407,9 @@ def main_c3(args, torch):
     ab = fp.algorithmic_bytes()
     dm = fp.device_ms
     log('c3: %.1f ms per step; host planning %.1f ms; device ms %s; pcg iterations %d, resid %.2e; '
-        'nnz %d' % (dt * 1e3, fp.timing['plan_s'] * 1e3, dm, fp.iterations, fp.resid, len(fp.col)))
+        'nnz %d' % (dt * 1e3, fp.timing['plan_s'] * 1e3, dm, fp.iterations, fp.resid, fp.nnz))
     # the solver against an FP64 direct solve of the same device-built system
-    A = sp.csr_matrix((fp.vals.cpu().numpy(), fp.col, fp.row_ptr), shape=(n, n))
+    A = sp.csr_matrix((fp.vals.cpu().numpy(), fp.d_col.cpu().numpy(), fp.row_ptr), shape=(n, n))
     bb = fp.b.cpu().numpy()
     dg = A.diagonal()
     A = A + sp.diags((dg == 0).astype(np.float64))
@@ -460,7 +460,7 @@ def main_c3(args, torch):
         'cpu_baseline': cpu,
         'forced': {'host_plan_ms': round(fp.timing['plan_s'] * 1e3, 2),
                    'pcg_iterations': fp.iterations, 'pcg_rel_residual': fp.resid,
-                   'nnz': int(len(fp.col)), 'tiles': int(len(fp.tile_start) - 1),
+                   'nnz': int(fp.nnz), 'tiles': int(len(fp.tile_start) - 1),
                    'solver_max_err_over_sigma_vs_fp64_direct': solver_err,
                    'flux_pull_rms_vs_truth': float(np.sqrt(np.mean(ferr ** 2)))},
     }

@@ -238,6 +238,18 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  *                  arrays of `len` floats (bilinear PSF interpolation of
  *                  stamps or of spectra; replaces PixelizedPsfEx's
  *                  per-position stamp, tractor/psfex.py:481-520); n <= 65535.
+ *   fp_overlaps:   the sparsity pattern of A (the sources whose patches share
+ *                  pixels: the columns LsqrOptimizer stacks into its sparse
+ *                  matrix, tractor/lsqr_optimizer.py:386-431).  Sources are
+ *                  numbered in tile order: stile[s] = the cell (ntx x nty
+ *                  cells) holding s's position, tile_first[c] .. tile_end[c]
+ *                  = the sources of cell c; two windows can only meet when
+ *                  their cells are within `reach` in x and in y.  With
+ *                  counts: counts[s] = row length (diagonal included); else
+ *                  the columns of row s, sorted, at col[row_ptr[s]..], and
+ *                  diag[s] = the position of (s, s).
+ *   fp_pair_dots_csr: A_st for every entry of that pattern (t >= s computed,
+ *                  mirrored to (t, s)).
  *   fp_pair_dots:  A_st = sum u_s u_t inverr^2 over each overlapping pair
  *                  (s, t) = pairs[p] (FP32 products, FP64 sum), written to
  *                  the CSR values at pos[p][0] and pos[p][1].
@@ -255,6 +267,13 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  */
 int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
                     const float *weights, const int32_t *idx, float *out, int n, void *stream);
+int trh_fp_overlaps(trh_handle *h, int n, const int32_t *win, const int32_t *stile, int ntx,
+                    int nty, const int32_t *tile_first, const int32_t *tile_end, int reach,
+                    int32_t *counts, const int64_t *row_ptr, int32_t *col, int64_t *diag,
+                    void *stream);
+int trh_fp_pair_dots_csr(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
+                         const int32_t *win, const int64_t *poff, const float *patches,
+                         const float *inverr, int img_w, double *vals, void *stream);
 int trh_fp_pair_dots(trh_handle *h, const int32_t *pairs, const int64_t *pos, int npairs,
                      const int32_t *win, const int64_t *poff, const float *patches,
                      const float *inverr, int img_w, double *vals, void *stream);

@@ -63,3 +63,30 @@ def test_forced_photometry_matches_oracle():
     ok = (err <= 1e-4 * np.abs(ref)) | (err <= 0.01 * sig)
     print('flux parity: %d/%d; max err/sigma %.3g' % (ok.sum(), n, np.max(err / sig)), flush=True)
     assert ok.all()
+
+
+def test_device_overlap_pattern_matches_host_rule():
+    """trh_fp_overlaps (cells of tile x tile pixels, a per-source wavefront
+    scanning the cells within reach) gives exactly the CSR pattern of the
+    host's cell-list rule (overlap_pairs + csr_of_pairs), sorted columns and
+    the diagonal's positions included."""
+    import torch
+    from tractor_amd import forced as F, workloads as W
+    size, n = 512, 1500
+    rng = np.random.default_rng(5)
+    kind, pos, flux, shape = W.draw_forced_scene(n, size, rng=rng)
+    pos[:7] = [[-40., 10.], [600., 30.], [5., -3.], [511.5, 511.5], [0., 0.], [250., 700.], [-200., -200.]]
+    basis = W.psf_grid_stamps()
+    idx, w = W.bilinear_psf_weights(np.clip(pos[:, 0], 0, size - 1), np.clip(pos[:, 1], 0, size - 1),
+                                    size, size)
+    data = torch.zeros((size, size), dtype=torch.float32, device='cuda')
+    fp = F.BatchForcedPhotometry(data, torch.ones_like(data), basis, idx, w, kind, pos, shape, tile=64)
+    fp.plan()
+    torch.cuda.synchronize()
+    rp, col, _ = F.csr_of_pairs(F.overlap_pairs(fp.win), n)
+    assert np.array_equal(fp.row_ptr, rp)
+    dcol = fp.d_col.cpu().numpy()
+    assert np.array_equal(dcol, col)
+    diag = fp.d_diag.cpu().numpy()
+    assert np.array_equal(dcol[diag], np.arange(n))
+    assert fp.reach >= 1

@@ -52,6 +52,9 @@ struct trh_fitter {
     double cdi[4] = {1, 0, 0, 1};
     int maxrows[NFFT] = {8, 8, 8, 8, 8};
     int maxP_pt = 1;
+    // smallest FFT size any galaxy render can use: halfsize >= max(pw, ph) / 2
+    // (galaxy.py:337-351), so the buckets below it never hold a job
+    int min_sz = 32;
     int cx[NFFT] = {0, 0, 0, 0, 0}, cy[NFFT] = {0, 0, 0, 0, 0};
     // device
     int32_t *kind = nullptr, *P = nullptr;
@@ -837,6 +840,10 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
             if (P[s] > f->maxP_pt) f->maxP_pt = P[s];
         }
     }
+    {
+        const int hs = (int)ceil(fmax(pw / 2.0, ph / 2.0));
+        while (f->min_sz < 2 * hs && f->min_sz < kSizes[NFFT - 1]) f->min_sz <<= 1;
+    }
     for (int b = 0; b < NFFT; ++b) {
         int mr = max_rows_for(kSizes[b], h->lds_max);
         if (maxP_sz[b] > 0 && maxP_sz[b] < mr) mr = maxP_sz[b];
@@ -992,6 +999,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
     SLOT_END(SL_PREP);
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
+        if (kSizes[b] < f->min_sz) continue;
         SLOT_BEGIN(b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
                                  f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal, f->maxrows[b], f->spec[b], f->renders,
@@ -1030,6 +1038,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
                        f->bcnt_ls, f->list_ls, cap_ls, f->cnt + LS_CNT);
     if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
+        if (kSizes[b] < f->min_sz) continue;
         SLOT_BEGIN(SL_LS + b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
                                  f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],

@@ -217,6 +217,102 @@ int grid_waves(trh_handle *h, long units) {
     return (int)(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
+
+// The pattern of A: for each source s, the sources whose windows intersect
+// s's (s itself always).  Sources are numbered in tile order (tile x tile
+// pixel cells, ntx x nty of them; tile_first[T] .. tile_end[T] the sources of
+// cell T, stile[s] the cell of s), and two windows can only meet when their
+// cells are within `reach` of each other in x and in y.  One wavefront per
+// source scans those cells in increasing order, which is increasing source
+// order, so a row's columns come out sorted.  counts != NULL: write the row
+// lengths; else write the columns at row_ptr[s] and the position of the
+// diagonal entry at diag[s].
+__device__ __forceinline__ bool win_meet(const int32_t *__restrict__ win, int s, int t) {
+    const int sw = win[4 * s + 2], sh = win[4 * s + 3], tw = win[4 * t + 2], th = win[4 * t + 3];
+    if (sw <= 0 || sh <= 0 || tw <= 0 || th <= 0) return false;
+    const int sx0 = win[4 * s], sy0 = win[4 * s + 1], tx0 = win[4 * t], ty0 = win[4 * t + 1];
+    return max(sx0, tx0) < min(sx0 + sw, tx0 + tw) && max(sy0, ty0) < min(sy0 + sh, ty0 + th);
+}
+
+__global__ void __launch_bounds__(256)
+k_fp_overlaps(int n, const int32_t *__restrict__ win, const int32_t *__restrict__ stile, int ntx,
+              int nty, const int32_t *__restrict__ tile_first, const int32_t *__restrict__ tile_end,
+              int reach, int32_t *__restrict__ counts, const int64_t *__restrict__ row_ptr,
+              int32_t *__restrict__ col, int64_t *__restrict__ diag) {
+    const int lane = threadIdx.x & 63;
+    const int nwave = gridDim.x * (blockDim.x >> 6);
+    for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += nwave) {
+        const int T = stile[s];
+        const int ty = T / ntx, tx = T - ty * ntx;
+        int64_t base = counts ? 0 : row_ptr[s];
+        int found = 0;
+        for (int yy = max(0, ty - reach); yy <= min(nty - 1, ty + reach); ++yy)
+            for (int xx = max(0, tx - reach); xx <= min(ntx - 1, tx + reach); ++xx) {
+                const int c = yy * ntx + xx;
+                const int e = tile_end[c];
+                for (int t0 = tile_first[c]; t0 < e; t0 += 64) {
+                    const int t = t0 + lane;
+                    const bool hit = t < e && (t == s || win_meet(win, s, t));
+                    const unsigned long long m = __ballot(hit);
+                    if (!counts && hit) {
+                        const int64_t k = base + found + __popcll(m & ((1ull << lane) - 1ull));
+                        col[k] = t;
+                        if (t == s) diag[s] = k;
+                    }
+                    found += __popcll(m);
+                }
+            }
+        if (counts && lane == 0) counts[s] = found;
+    }
+}
+
+// One wavefront per row s of A: the entries with t >= s, each the sum of
+// u_s u_t inverr^2 over the intersection of the two windows, also written at
+// the mirror entry (t, s), found by binary search in row t.
+__global__ void __launch_bounds__(256)
+k_fp_pair_dots_csr(int n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                   const int32_t *__restrict__ win, const int64_t *__restrict__ poff,
+                   const float *__restrict__ patches, const float *__restrict__ inverr, int img_w,
+                   double *__restrict__ vals) {
+    const int lane = threadIdx.x & 63;
+    const int nwave = gridDim.x * (blockDim.x >> 6);
+    for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += nwave) {
+        const int sx0 = win[4 * s], sy0 = win[4 * s + 1], sw = win[4 * s + 2], sh = win[4 * s + 3];
+        const float *us = patches + poff[s];
+        for (int64_t k = row_ptr[s]; k < row_ptr[s + 1]; ++k) {
+            const int t = col[k];
+            if (t < s) continue;
+            const int tx0 = win[4 * t], ty0 = win[4 * t + 1], tw = win[4 * t + 2], th = win[4 * t + 3];
+            const int X0 = max(sx0, tx0), X1 = min(sx0 + sw, tx0 + tw);
+            const int Y0 = max(sy0, ty0), Y1 = min(sy0 + sh, ty0 + th);
+            const float *ut = patches + poff[t];
+            float acc = 0.f;
+            const int nx = X1 - X0;
+            for (int y = Y0; y < Y1; ++y) {
+                const float *rs = us + (int64_t)(y - sy0) * sw + (X0 - sx0);
+                const float *rt = ut + (int64_t)(y - ty0) * tw + (X0 - tx0);
+                const float *re = inverr + (int64_t)y * img_w + X0;
+                for (int x = lane; x < nx; x += 64) {
+                    const float e = re[x];
+                    acc += rs[x] * rt[x] * (e * e);
+                }
+            }
+            const double v = wave_sum_d((double)acc);
+            if (lane == 0) {
+                vals[k] = v;
+                if (t != s) {
+                    int64_t lo = row_ptr[t], hi = row_ptr[t + 1] - 1;
+                    while (lo < hi) {
+                        const int64_t mid = (lo + hi) >> 1;
+                        if (col[mid] < s) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    vals[lo] = v;
+                }
+            }
+        }
+    }
+}
 }  // namespace
 
 extern "C" int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
@@ -230,6 +326,33 @@ extern "C" int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, in
     hipLaunchKernelGGL(k_psf_combine, dim3((int)gx, n), dim3(256), 0, (hipStream_t)stream, basis,
                        nbasis, len, weights, idx, out, n);
     return hip_check(hipGetLastError(), "k_psf_combine");
+}
+
+extern "C" int trh_fp_overlaps(trh_handle *h, int n, const int32_t *win, const int32_t *stile,
+                               int ntx, int nty, const int32_t *tile_first,
+                               const int32_t *tile_end, int reach, int32_t *counts,
+                               const int64_t *row_ptr, int32_t *col, int64_t *diag,
+                               void *stream) {
+    if (!h || n <= 0 || !win || !stile || ntx <= 0 || nty <= 0 || !tile_first || !tile_end ||
+        reach < 0 || (!counts && (!row_ptr || !col || !diag)))
+        return set_err(TRH_E_ARG, "trh_fp_overlaps: bad arguments");
+    hipLaunchKernelGGL(k_fp_overlaps, dim3(grid_waves(h, n)), dim3(256), 0, (hipStream_t)stream, n,
+                       win, stile, ntx, nty, tile_first, tile_end, reach, counts, row_ptr, col,
+                       diag);
+    return hip_check(hipGetLastError(), "k_fp_overlaps");
+}
+
+extern "C" int trh_fp_pair_dots_csr(trh_handle *h, int n, const int64_t *row_ptr,
+                                    const int32_t *col, const int32_t *win, const int64_t *poff,
+                                    const float *patches, const float *inverr, int img_w,
+                                    double *vals, void *stream) {
+    if (!h || n <= 0 || !row_ptr || !col || !win || !poff || !patches || !inverr || img_w <= 0 ||
+        !vals)
+        return set_err(TRH_E_ARG, "trh_fp_pair_dots_csr: bad arguments");
+    hipLaunchKernelGGL(k_fp_pair_dots_csr, dim3(grid_waves(h, n)), dim3(256), 0,
+                       (hipStream_t)stream, n, row_ptr, col, win, poff, patches, inverr, img_w,
+                       vals);
+    return hip_check(hipGetLastError(), "k_fp_pair_dots_csr");
 }
 
 extern "C" int trh_fp_pair_dots(trh_handle *h, const int32_t *pairs, const int64_t *pos,

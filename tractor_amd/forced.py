@@ -214,10 +214,12 @@ class BatchForcedPhotometry(object):
         n = len(kind)
         self.n = n
         ntx = (self.W + tile - 1) // tile
+        self.tile, self.ntx, self.nty = tile, ntx, (self.H + tile - 1) // tile
         tid = (np.clip(pos[:, 1], 0, self.H - 1) // tile).astype(np.int64) * ntx + \
             (np.clip(pos[:, 0], 0, self.W - 1) // tile).astype(np.int64)
         self.order = np.argsort(tid, kind='stable')
         o = self.order
+        self.stile = tid[o].astype(np.int32)
         self.kind = np.asarray(kind)[o]
         self.pos = np.asarray(pos, np.float64)[o]
         self.shape = np.asarray(shape, np.float64)[o]
@@ -243,15 +245,49 @@ class BatchForcedPhotometry(object):
         self.npix = int(area.sum())
         self.jobs['out_offset'] = self.poff
         self.jobs['out_stride'] = np.maximum(self.win[:, 2], 1)
-        pairs = overlap_pairs(self.win)
-        self.row_ptr, self.col, pos = csr_of_pairs(pairs, self.n)
-        self.pairs = pairs.astype(np.int32)
-        self.pair_pos = pos
+        self.pattern()
         sizes = np.diff(self.tile_start).astype(np.int64)
         self.block_off = np.zeros(len(sizes), np.int64)
         self.block_off[1:] = np.cumsum(sizes ** 2)[:-1]
         self.max_tile = int(sizes.max())
         self.timing['plan_s'] = time.perf_counter() - t0
+        return self
+
+    def pattern(self):
+        """The sparsity pattern of A on the device (trh_fp_overlaps): CSR
+        row_ptr (kept on the host too), sorted columns and the diagonal's
+        position in each row.  Two windows can only share a pixel when the
+        cells of the sources' (clipped) positions are within `reach` cells,
+        reach from the farthest window edge from its source position."""
+        lib, h, st = self.lib, self.h.value, self.stream
+        n, win = self.n, self.win
+        px = np.clip(self.pos[:, 0], 0, self.W - 1)
+        py = np.clip(self.pos[:, 1], 0, self.H - 1)
+        ne = (win[:, 2] > 0) & (win[:, 3] > 0)
+        ext = np.maximum.reduce([px - win[:, 0], win[:, 0] + win[:, 2] - px,
+                                 py - win[:, 1], win[:, 1] + win[:, 3] - py])
+        emax = float(ext[ne].max()) if ne.any() else 0.
+        self.reach = int(math.floor(2. * emax / self.tile)) + 1
+        ncell = self.ntx * self.nty
+        cnt = np.bincount(self.stile, minlength=ncell)
+        first = np.zeros(ncell, np.int32)
+        first[1:] = np.cumsum(cnt)[:-1]
+        self.d_win = self._t(win.astype(np.int32))
+        self.d_stile = self._t(self.stile)
+        self.d_tfirst = self._t(first)
+        self.d_tend = self._t((first + cnt).astype(np.int32))
+        counts = torch.zeros(n, dtype=torch.int32, device=self.dev)
+        args = (h, n, ptr(self.d_win), ptr(self.d_stile), self.ntx, self.nty, ptr(self.d_tfirst),
+                ptr(self.d_tend), self.reach)
+        check(lib.trh_fp_overlaps(*args, ptr(counts), None, None, None, st), 'fp_overlaps')
+        self.row_ptr = np.zeros(n + 1, np.int64)
+        self.row_ptr[1:] = np.cumsum(counts.cpu().numpy())
+        self.nnz = int(self.row_ptr[-1])
+        self.d_row_ptr = self._t(self.row_ptr)
+        self.d_col = torch.empty(self.nnz, dtype=torch.int32, device=self.dev)
+        self.d_diag = torch.empty(n, dtype=torch.int64, device=self.dev)
+        check(lib.trh_fp_overlaps(*args, None, ptr(self.d_row_ptr), ptr(self.d_col),
+                                  ptr(self.d_diag), st), 'fp_overlaps')
         return self
 
     def _t(self, a, dt=None):
@@ -323,15 +359,13 @@ class BatchForcedPhotometry(object):
     def assemble(self, mod0=None):
         """A (CSR values) and b on the device."""
         lib, h, st = self.lib, self.h.value, self.stream
-        self.d_win = self._t(self.win.astype(np.int32))
         self.d_poff = self._t(self.poff)
-        self.d_pairs = self._t(self.pairs)
-        self.d_pos = self._t(self.pair_pos)
-        self.vals = torch.zeros(len(self.col), dtype=torch.float64, device=self.dev)
+        self.vals = torch.zeros(self.nnz, dtype=torch.float64, device=self.dev)
         self.b = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
-        check(lib.trh_fp_pair_dots(h, ptr(self.d_pairs), ptr(self.d_pos), len(self.pairs),
-                                   ptr(self.d_win), ptr(self.d_poff), ptr(self.patches),
-                                   ptr(self.inverr), self.W, ptr(self.vals), st), 'fp_pair_dots')
+        check(lib.trh_fp_pair_dots_csr(h, self.n, ptr(self.d_row_ptr), ptr(self.d_col),
+                                       ptr(self.d_win), ptr(self.d_poff), ptr(self.patches),
+                                       ptr(self.inverr), self.W, ptr(self.vals), st),
+              'fp_pair_dots_csr')
         check(lib.trh_fp_rhs(h, self.n, ptr(self.d_win), ptr(self.d_poff), ptr(self.patches),
                              ptr(self.data), ptr(mod0), ptr(self.inverr), self.W, ptr(self.b), st),
               'fp_rhs')
@@ -341,8 +375,8 @@ class BatchForcedPhotometry(object):
         """FP64 conjugate gradients preconditioned by the tile blocks' exact
         Cholesky factors.  Returns fluxes in the caller's order."""
         lib, h, st = self.lib, self.h.value, self.stream
-        d_rp = self._t(self.row_ptr)
-        d_col = self._t(self.col)
+        d_rp = self.d_row_ptr
+        d_col = self.d_col
         d_ts = self._t(self.tile_start)
         d_bo = self._t(self.block_off)
         nt = len(self.tile_start) - 1
@@ -383,7 +417,7 @@ class BatchForcedPhotometry(object):
         self.info = info.cpu().numpy()
         flux = np.empty(n)
         flux[self.order] = x.cpu().numpy()
-        diag = self.vals[self._t(self.pair_pos[:n, 0])].cpu().numpy()   # pairs start with (s, s)
+        diag = self.vals[self.d_diag].cpu().numpy()
         iv = np.empty(n)
         iv[self.order] = diag
         self.flux, self.iv = flux, iv
@@ -433,12 +467,15 @@ class BatchForcedPhotometry(object):
         gal = (self.kind != KIND_POINT) & (area > 0)
         spec = np.sum(8. * (self.sz[gal] // 2 + 1) * self.sz[gal])
         render = 4. * area.sum() + spec + 4. * self.ph * self.pw * np.sum((~gal) & (area > 0))
-        s, t = self.pairs[:, 0], self.pairs[:, 1]
+        col = self.d_col.cpu().numpy().astype(np.int64)
+        rows = np.repeat(np.arange(self.n), np.diff(self.row_ptr))
+        up = col >= rows
+        s, t = rows[up], col[up]
         x0 = np.maximum(self.win[s, 0], self.win[t, 0])
         x1 = np.minimum(self.win[s, 0] + self.win[s, 2], self.win[t, 0] + self.win[t, 2])
         y0 = np.maximum(self.win[s, 1], self.win[t, 1])
         y1 = np.minimum(self.win[s, 1] + self.win[s, 3], self.win[t, 1] + self.win[t, 3])
         ov = np.maximum(x1 - x0, 0) * np.maximum(y1 - y0, 0)
-        assemble = 12. * ov.sum() + 12. * area.sum() + 16. * len(self.col)
-        pcg = 12. * len(self.col) + 8. * 10 * self.n
+        assemble = 12. * ov.sum() + 12. * area.sum() + 16. * self.nnz
+        pcg = 12. * self.nnz + 8. * 10 * self.n
         return dict(render=render, assemble=assemble, pcg_iteration=pcg)
