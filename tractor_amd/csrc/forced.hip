@@ -266,49 +266,58 @@ k_fp_overlaps(int n, const int32_t *__restrict__ win, const int32_t *__restrict_
     }
 }
 
-// One wavefront per row s of A: the entries with t >= s, each the sum of
-// u_s u_t inverr^2 over the intersection of the two windows, also written at
-// the mirror entry (t, s), found by binary search in row t.
+// One wavefront per entry (s, t) of A with t >= s: the sum of u_s u_t
+// inverr^2 over the intersection of the two windows, also written at the
+// mirror entry (t, s), found by binary search in row t.  The row of entry k
+// is found by binary search in row_ptr.
 __global__ void __launch_bounds__(256)
 k_fp_pair_dots_csr(int n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                    const int32_t *__restrict__ win, const int64_t *__restrict__ poff,
                    const float *__restrict__ patches, const float *__restrict__ inverr, int img_w,
                    double *__restrict__ vals) {
     const int lane = threadIdx.x & 63;
-    const int nwave = gridDim.x * (blockDim.x >> 6);
-    for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += nwave) {
-        const int sx0 = win[4 * s], sy0 = win[4 * s + 1], sw = win[4 * s + 2], sh = win[4 * s + 3];
+    const int64_t nnz = row_ptr[n];
+    const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nnz;
+         k += nwave) {
+        // s = the row holding entry k: the last r with row_ptr[r] <= k
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (row_ptr[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int s = lo, t = col[k];
+        if (t < s) continue;
+        const int sx0 = win[4 * s], sy0 = win[4 * s + 1], sw = win[4 * s + 2];
+        const int tx0 = win[4 * t], ty0 = win[4 * t + 1], tw = win[4 * t + 2], th = win[4 * t + 3];
+        const int sh = win[4 * s + 3];
+        const int X0 = max(sx0, tx0), X1 = min(sx0 + sw, tx0 + tw);
+        const int Y0 = max(sy0, ty0), Y1 = min(sy0 + sh, ty0 + th);
         const float *us = patches + poff[s];
-        for (int64_t k = row_ptr[s]; k < row_ptr[s + 1]; ++k) {
-            const int t = col[k];
-            if (t < s) continue;
-            const int tx0 = win[4 * t], ty0 = win[4 * t + 1], tw = win[4 * t + 2], th = win[4 * t + 3];
-            const int X0 = max(sx0, tx0), X1 = min(sx0 + sw, tx0 + tw);
-            const int Y0 = max(sy0, ty0), Y1 = min(sy0 + sh, ty0 + th);
-            const float *ut = patches + poff[t];
-            float acc = 0.f;
-            const int nx = X1 - X0;
-            for (int y = Y0; y < Y1; ++y) {
-                const float *rs = us + (int64_t)(y - sy0) * sw + (X0 - sx0);
-                const float *rt = ut + (int64_t)(y - ty0) * tw + (X0 - tx0);
-                const float *re = inverr + (int64_t)y * img_w + X0;
-                for (int x = lane; x < nx; x += 64) {
-                    const float e = re[x];
-                    acc += rs[x] * rt[x] * (e * e);
-                }
+        const float *ut = patches + poff[t];
+        float acc = 0.f;
+        const int nx = X1 - X0;
+        for (int y = Y0; y < Y1; ++y) {
+            const float *rs = us + (int64_t)(y - sy0) * sw + (X0 - sx0);
+            const float *rt = ut + (int64_t)(y - ty0) * tw + (X0 - tx0);
+            const float *re = inverr + (int64_t)y * img_w + X0;
+            for (int x = lane; x < nx; x += 64) {
+                const float e = re[x];
+                acc += rs[x] * rt[x] * (e * e);
             }
-            const double v = wave_sum_d((double)acc);
-            if (lane == 0) {
-                vals[k] = v;
-                if (t != s) {
-                    int64_t lo = row_ptr[t], hi = row_ptr[t + 1] - 1;
-                    while (lo < hi) {
-                        const int64_t mid = (lo + hi) >> 1;
-                        if (col[mid] < s) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    vals[lo] = v;
+        }
+        const double v = wave_sum_d((double)acc);
+        if (lane == 0) {
+            vals[k] = v;
+            if (t != s) {
+                int64_t a = row_ptr[t], b = row_ptr[t + 1] - 1;
+                while (a < b) {
+                    const int64_t mid = (a + b) >> 1;
+                    if (col[mid] < s) a = mid + 1;
+                    else b = mid;
                 }
+                vals[a] = v;
             }
         }
     }
@@ -349,7 +358,7 @@ extern "C" int trh_fp_pair_dots_csr(trh_handle *h, int n, const int64_t *row_ptr
     if (!h || n <= 0 || !row_ptr || !col || !win || !poff || !patches || !inverr || img_w <= 0 ||
         !vals)
         return set_err(TRH_E_ARG, "trh_fp_pair_dots_csr: bad arguments");
-    hipLaunchKernelGGL(k_fp_pair_dots_csr, dim3(grid_waves(h, n)), dim3(256), 0,
+    hipLaunchKernelGGL(k_fp_pair_dots_csr, dim3(grid_waves(h, 64L * n)), dim3(256), 0,
                        (hipStream_t)stream, n, row_ptr, col, win, poff, patches, inverr, img_w,
                        vals);
     return hip_check(hipGetLastError(), "k_fp_pair_dots_csr");
