@@ -256,8 +256,11 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  *   fp_rhs:        b_s = sum u_s (data - mod0) inverr^2 (mod0 may be NULL).
  *   fp_block_chol: FP64 Cholesky of A's diagonal blocks, one per tile of
  *                  consecutive rows [tile_start[T], tile_start[T+1]), dense
- *                  at blocks + block_off[T]; info[T] = k if pivot k fails.
- *   fp_precond:    z = blockdiag(L L^T)^-1 r (max_n >= every tile size).
+ *                  at blocks + block_off[T]: L_ii on the diagonal, L below it
+ *                  and the transpose of L^-1's strict lower part above it;
+ *                  info[T] = k if pivot k fails.
+ *   fp_precond:    z = blockdiag(L L^T)^-1 r = L^-T L^-1 r from those blocks
+ *                  (max_n >= every tile size).
  *   fp_spmv:       y = A x (CSR, FP64).
  *   fp_add_patches: img += sum_s flux[s] u_s (model image; Tractor.
  *                  getModelImage, tractor/engine.py:424-445, for these

@@ -106,8 +106,13 @@ k_fp_rhs(int n, const int32_t *__restrict__ win, const int64_t *__restrict__ pof
 }
 
 // One workgroup per tile of rows [t0, t1): copy A's diagonal block into a
-// dense FP64 matrix and factor it (lower Cholesky, right-looking).  A zero
-// diagonal (a source with no pixels) becomes 1 so its flux solves to 0.
+// dense FP64 matrix and factor it (lower Cholesky L, right-looking); then
+// invert the factor, so that applying the preconditioner is two dense
+// triangular products with no per-row barrier.  Layout after the call: the
+// diagonal holds L_ii, the lower triangle L, the strict upper triangle the
+// transpose of the strict lower part of L^-1 (B[j][i] = (L^-1)_ij, i > j;
+// (L^-1)_ii = 1 / L_ii).  A zero diagonal (a source with no pixels) becomes 1
+// so its flux solves to 0.
 __global__ void __launch_bounds__(256)
 k_fp_block_chol(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                 const double *__restrict__ vals, const int32_t *__restrict__ tile_start,
@@ -149,34 +154,42 @@ k_fp_block_chol(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__
         }
         __syncthreads();
     }
+    // column j of L^-1 by forward substitution (one thread per column); it
+    // reads only the lower triangle and writes row j of the upper one
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+        for (int i = j + 1; i < n; ++i) {
+            double acc = B[i * n + j] / B[j * n + j];   // k = j term: L_ij (L^-1)_jj
+            for (int k = j + 1; k < i; ++k) acc += B[i * n + k] * B[j * n + k];
+            B[j * n + i] = -acc / B[i * n + i];
+        }
 }
 
-// z = (L L^T)^-1 r on each tile (forward then backward substitution).
+// z = (L L^T)^-1 r = L^-T (L^-1 r) on each tile, from the inverse factor
+// stored by k_fp_block_chol: two dense triangular products.
 __global__ void __launch_bounds__(256)
 k_fp_precond(const int32_t *__restrict__ tile_start, const int64_t *__restrict__ block_off,
              const double *__restrict__ blocks, const double *__restrict__ r,
              double *__restrict__ z) {
     extern __shared__ double y[];
+    double *rr = y + (tile_start[blockIdx.x + 1] - tile_start[blockIdx.x]);
     const int T = blockIdx.x;
     const int t0 = tile_start[T], n = tile_start[T + 1] - t0;
-    const double *L = blocks + block_off[T];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = r[t0 + i];
+    const double *B = blocks + block_off[T];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) rr[i] = r[t0 + i];
     __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        const double yj = y[j] / L[j * n + j];
-        __syncthreads();
-        if (threadIdx.x == 0) y[j] = yj;
-        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= L[i * n + j] * yj;
-        __syncthreads();
+    // y_i = sum_{k < i} (L^-1)_ik r_k + r_i / L_ii,  (L^-1)_ik = B[k][i]
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double acc = rr[i] / B[i * n + i];
+        for (int k = 0; k < i; ++k) acc += B[k * n + i] * rr[k];
+        y[i] = acc;
     }
-    for (int j = n - 1; j >= 0; --j) {
-        const double yj = y[j] / L[j * n + j];
-        __syncthreads();
-        if (threadIdx.x == 0) y[j] = yj;
-        for (int i = threadIdx.x; i < j; i += blockDim.x) y[i] -= L[j * n + i] * yj;
-        __syncthreads();
+    __syncthreads();
+    // z_i = sum_{k > i} (L^-1)_ki y_k + y_i / L_ii,  (L^-1)_ki = B[i][k]
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double acc = y[i] / B[i * n + i];
+        for (int k = i + 1; k < n; ++k) acc += B[i * n + k] * y[k];
+        z[t0 + i] = acc;
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) z[t0 + i] = y[i];
 }
 
 // y = A x (CSR, FP64), one wavefront per row.
@@ -405,9 +418,9 @@ extern "C" int trh_fp_precond(trh_handle *h, const int32_t *tile_start, const in
                               int ntile, int max_n, const double *blocks, const double *r,
                               double *z, void *stream) {
     if (!h || !tile_start || !block_off || !blocks || !r || !z || ntile <= 0 || max_n <= 0 ||
-        (size_t)max_n * sizeof(double) > (size_t)h->lds_max)
+        2 * (size_t)max_n * sizeof(double) > (size_t)h->lds_max)
         return set_err(TRH_E_ARG, "trh_fp_precond: bad arguments (max_n %d)", max_n);
-    hipLaunchKernelGGL(k_fp_precond, dim3(ntile), dim3(256), sizeof(double) * (size_t)max_n,
+    hipLaunchKernelGGL(k_fp_precond, dim3(ntile), dim3(256), 2 * sizeof(double) * (size_t)max_n,
                        (hipStream_t)stream, tile_start, block_off, blocks, r, z);
     return hip_check(hipGetLastError(), "k_fp_precond");
 }
