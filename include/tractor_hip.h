@@ -254,19 +254,30 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  *                  (s, t) = pairs[p] (FP32 products, FP64 sum), written to
  *                  the CSR values at pos[p][0] and pos[p][1].
  *   fp_rhs:        b_s = sum u_s (data - mod0) inverr^2 (mod0 may be NULL).
- *   fp_block_chol: FP64 Cholesky of A's diagonal blocks, one per tile of
- *                  consecutive rows [tile_start[T], tile_start[T+1]), dense
- *                  at blocks + block_off[T]: L_ii on the diagonal, L below it
- *                  and the transpose of L^-1's strict lower part above it;
- *                  info[T] = k if pivot k fails.
- *   fp_precond:    z = blockdiag(L L^T)^-1 r = L^-T L^-1 r from those blocks
- *                  (max_n >= every tile size).
+ *   fp_block_inv:  the inverse of each of A's diagonal blocks (FP64,
+ *                  Gauss-Jordan in LDS), one block per tile of consecutive
+ *                  rows [tile_start[T], tile_start[T+1]) of at most max_n rows
+ *                  (max_n^2 + max_n doubles and max_n + 1 ints must fit in
+ *                  LDS), written dense and symmetric at blocks +
+ *                  block_off[T].  info[T] = k if pivot k is negative, -1 if
+ *                  the tile has more than max_n rows.
+ *   fp_precond:    z = blockdiag(A_TT)^-1 r from those inverses (one dense
+ *                  product per tile; max_n >= every tile size).
  *   fp_spmv:       y = A x (CSR, FP64).
+ *   fp_pcg:        A x = b by FP64 conjugate gradients preconditioned with
+ *                  those blocks, every scalar kept on the device.  it0 == 0
+ *                  starts from x = 0; then `iters` iterations k = it0 ..
+ *                  it0+iters-1 are enqueued (call again with it0 advanced
+ *                  to continue).  Stops when |r|^2 <= tol^2 |b|^2:
+ *                  status[0] = 0 running, 1 converged, 2 breakdown
+ *                  (p.Ap <= 0); status[1] = iterations taken; launches after
+ *                  the stop return at once.  work: 4 n + 5 ntile + 2 doubles,
+ *                  kept between calls; its last two hold |r|^2 and |b|^2.
  *   fp_add_patches: img += sum_s flux[s] u_s (model image; Tractor.
  *                  getModelImage, tractor/engine.py:424-445, for these
  *                  patches).
- * (No reference counterpart for fp_block_chol, fp_precond and fp_spmv: the
- * reference calls scipy's LSQR, lsqr_optimizer.py:586.)
+ * (No reference counterpart for fp_block_inv, fp_precond, fp_spmv and
+ * fp_pcg: the reference calls scipy's LSQR, lsqr_optimizer.py:586.)
  */
 int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
                     const float *weights, const int32_t *idx, float *out, int n, void *stream);
@@ -283,13 +294,17 @@ int trh_fp_pair_dots(trh_handle *h, const int32_t *pairs, const int64_t *pos, in
 int trh_fp_rhs(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
                const float *patches, const float *data, const float *mod0, const float *inverr,
                int img_w, double *b, void *stream);
-int trh_fp_block_chol(trh_handle *h, const int64_t *row_ptr, const int32_t *col,
-                      const double *vals, const int32_t *tile_start, const int64_t *block_off,
-                      int ntile, double *blocks, int32_t *info, void *stream);
+int trh_fp_block_inv(trh_handle *h, const int64_t *row_ptr, const int32_t *col,
+                     const double *vals, const int32_t *tile_start, const int64_t *block_off,
+                     int ntile, int max_n, double *blocks, int32_t *info, void *stream);
 int trh_fp_precond(trh_handle *h, const int32_t *tile_start, const int64_t *block_off, int ntile,
                    int max_n, const double *blocks, const double *r, double *z, void *stream);
 int trh_fp_spmv(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
                 const double *vals, const double *x, double *y, void *stream);
+int trh_fp_pcg(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
+               const double *vals, const int32_t *tile_start, const int64_t *block_off, int ntile,
+               int max_n, const double *blocks, const double *b, double tol, int it0, int iters,
+               double *x, double *work, int32_t *status, void *stream);
 int trh_fp_add_patches(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
                        const float *patches, const float *flux, int img_w, float *img,
                        void *stream);

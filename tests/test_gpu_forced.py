@@ -90,3 +90,88 @@ def test_device_overlap_pattern_matches_host_rule():
     diag = fp.d_diag.cpu().numpy()
     assert np.array_equal(dcol[diag], np.arange(n))
     assert fp.reach >= 1
+
+
+def _tiled_spd_system(rng, sizes, density=0.15):
+    """A symmetric positive definite sparse matrix over tiles of the given
+    sizes (couplings inside and between neighbouring tiles), as CSR."""
+    n = int(sum(sizes))
+    A = np.zeros((n, n))
+    mask = rng.random((n, n)) < density
+    mask = np.triu(mask | mask.T, 1)
+    A[mask] = rng.normal(size=mask.sum())
+    A = A + A.T
+    A += np.diag(np.abs(A).sum(1) * 1.02 + rng.uniform(0.05, 2.0, n))   # strictly dominant
+    rows, cols = np.nonzero(A)
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.add.at(row_ptr, rows + 1, 1)
+    row_ptr = np.cumsum(row_ptr)
+    tile_start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    block_off = np.concatenate([[0], np.cumsum(np.asarray(sizes, np.int64) ** 2)[:-1]])
+    return A, row_ptr, cols.astype(np.int32), A[rows, cols], tile_start, block_off
+
+
+def _device_pcg(A, row_ptr, col, vals, tile_start, block_off, b, tol, chunks):
+    import ctypes
+    import torch
+    from tractor_amd import _lib
+    lib = _lib.load()
+    h = _lib.handle(0)
+    dev = torch.device('cuda', 0)
+    n, nt = len(b), len(tile_start) - 1
+    max_n = int(np.diff(tile_start).max())
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d_rp, d_col, d_vals, d_ts, d_bo, d_b = map(t, (row_ptr, col, vals, tile_start, block_off, b))
+    blocks = torch.empty(int(block_off[-1] + (tile_start[-1] - tile_start[-2]) ** 2),
+                         dtype=torch.float64, device=dev)
+    info = torch.zeros(nt, dtype=torch.int32, device=dev)
+    P = lambda x: ctypes.c_void_p(x.data_ptr())
+    st = torch.cuda.current_stream(dev).cuda_stream
+    assert lib.trh_fp_block_inv(h.value, P(d_rp), P(d_col), P(d_vals), P(d_ts), P(d_bo), nt,
+                                 max_n, P(blocks), P(info), st) == 0
+    x = torch.empty(n, dtype=torch.float64, device=dev)
+    work = torch.empty(4 * n + 5 * nt + 2, dtype=torch.float64, device=dev)
+    status = torch.empty(2, dtype=torch.int32, device=dev)
+    it0 = 0
+    for k in chunks:
+        assert lib.trh_fp_pcg(h.value, n, P(d_rp), P(d_col), P(d_vals), P(d_ts), P(d_bo), nt,
+                              max_n, P(blocks), P(d_b), tol, it0, k, P(x), P(work), P(status),
+                              st) == 0, lib.trh_last_error()
+        it0 += k
+    torch.cuda.synchronize()
+    assert int(info.abs().sum()) == 0
+    return x.cpu().numpy(), status.cpu().numpy(), work[-2:].cpu().numpy()
+
+
+@pytest.mark.parametrize('sizes', [[7, 12, 1, 30, 9], [64] * 6, [3]])
+def test_device_pcg_matches_dense_solve(sizes):
+    """trh_fp_pcg against numpy's dense FP64 solve; the stop test is the
+    relative residual, so the error is bounded by tol * cond(A)."""
+    rng = np.random.default_rng(len(sizes) * 7 + sizes[0])
+    A, row_ptr, col, vals, ts, bo = _tiled_spd_system(rng, sizes)
+    b = rng.normal(size=A.shape[0])
+    ref = np.linalg.solve(A, b)
+    print('pcg case', sizes, flush=True)
+    # iterations enqueued in uneven chunks; launches after the stop are no-ops
+    x, status, norms = _device_pcg(A, row_ptr, col, vals, ts, bo, b, 1e-12, [1, 5, 200])
+    assert status[0] == 1 and 0 < status[1] <= 206
+    cond = np.linalg.cond(A)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * cond * np.max(np.abs(ref)) * 10
+    r = b - A @ x
+    assert np.sqrt(norms[0]) <= 1e-12 * np.linalg.norm(b) * 1.0001
+    assert norms[1] == pytest.approx(b @ b, rel=1e-12)
+    # recursive residual stays close to the true one
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b)
+    # a single tile is solved exactly by its preconditioner in one step
+    if len(sizes) == 1:
+        assert status[1] == 1
+
+
+def test_device_pcg_zero_rhs_and_iteration_cap():
+    rng = np.random.default_rng(5)
+    A, row_ptr, col, vals, ts, bo = _tiled_spd_system(rng, [20, 20, 20])
+    x, status, norms = _device_pcg(A, row_ptr, col, vals, ts, bo, np.zeros(60), 1e-12, [4])
+    assert status[0] == 1 and status[1] == 0 and np.all(x == 0) and norms[1] == 0
+    b = rng.normal(size=60)
+    x, status, _ = _device_pcg(A, row_ptr, col, vals, ts, bo, b, 0.0, [2])
+    assert status[0] == 0 and status[1] == 2

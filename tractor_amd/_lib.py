@@ -69,9 +69,10 @@ SIGNATURES = {
     'trh_fp_pair_dots_csr': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     'trh_fp_pair_dots': (_I, [_P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _P]),
     'trh_fp_rhs': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
-    'trh_fp_block_chol': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
+    'trh_fp_block_inv': (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
     'trh_fp_precond': (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P]),
     'trh_fp_spmv': (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    'trh_fp_pcg': (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _I, _P, _P, _D, _I, _I, _P, _P, _P, _P]),
     'trh_fp_add_patches': (_I, [_P, _I, _P, _P, _P, _P, _I, _P, _P]),
     'trh_cnn_conv_fwd': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
     'trh_cnn_conv_bwd_data': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),

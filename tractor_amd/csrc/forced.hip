@@ -105,90 +105,299 @@ k_fp_rhs(int n, const int32_t *__restrict__ win, const int64_t *__restrict__ pof
     }
 }
 
-// One workgroup per tile of rows [t0, t1): copy A's diagonal block into a
-// dense FP64 matrix and factor it (lower Cholesky L, right-looking); then
-// invert the factor, so that applying the preconditioner is two dense
-// triangular products with no per-row barrier.  Layout after the call: the
-// diagonal holds L_ii, the lower triangle L, the strict upper triangle the
-// transpose of the strict lower part of L^-1 (B[j][i] = (L^-1)_ij, i > j;
-// (L^-1)_ii = 1 / L_ii).  A zero diagonal (a source with no pixels) becomes 1
-// so its flux solves to 0.
+// One workgroup per tile of rows [t0, t1) (n = t1 - t0 rows, the block in
+// LDS): copy A's diagonal block from the CSR and invert it in place by
+// Gauss-Jordan elimination without pivoting (stable for a symmetric positive
+// definite block: every pivot is a Schur complement's diagonal), every thread
+// updating its share of the block at each pivot.  The inverse is symmetrised
+// and written dense at blocks + block_off[T].  A zero pivot (a source with no
+// pixels) becomes 1 so its flux solves to 0; a negative one sets info[T] =
+// its index + 1 (and becomes 1); a tile of more than max_n rows sets -1.
 __global__ void __launch_bounds__(256)
-k_fp_block_chol(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
-                const double *__restrict__ vals, const int32_t *__restrict__ tile_start,
-                const int64_t *__restrict__ block_off, double *__restrict__ blocks,
-                int32_t *__restrict__ info) {
+k_fp_block_inv(const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+               const double *__restrict__ vals, const int32_t *__restrict__ tile_start,
+               const int64_t *__restrict__ block_off, int max_n, double *__restrict__ blocks,
+               int32_t *__restrict__ info) {
+    // LDS: B (n * n), then column k (n), then the tile's row_ptr (n + 1, as
+    // offsets from its first entry)
+    extern __shared__ double B[];
     const int T = blockIdx.x;
     const int t0 = tile_start[T], n = tile_start[T + 1] - t0;
-    double *B = blocks + block_off[T];
+    if (n > max_n) {   // the block does not fit the LDS reserved for max_n
+        if (threadIdx.x == 0) info[T] = -1;
+        return;
+    }
+    double *ck = B + n * n;
+    int *rp = (int *)(ck + n);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int64_t e0 = row_ptr[t0];
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) rp[i] = (int)(row_ptr[t0 + i] - e0);
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) B[i] = 0.0;
     __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        const int row = t0 + r;
-        for (int64_t k = row_ptr[row]; k < row_ptr[row + 1]; ++k) {
-            const int c = col[k] - t0;
-            if (c >= 0 && c < n) B[r * n + c] = vals[k];
+    // the tile's entries, 256 at a time; an entry's row by bisection of rp
+    const int ne = rp[n];
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int c = col[e0 + e] - t0;
+        const double v = vals[e0 + e];
+        int lo = 0, hi = n;   // rp[lo] <= e < rp[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rp[mid] <= e) lo = mid;
+            else hi = mid;
         }
+        if (c >= 0 && c < n) B[lo * n + c] = v;
     }
     __syncthreads();
-    __shared__ double dj;
-    for (int j = 0; j < n; ++j) {
-        if (threadIdx.x == 0) {
-            double d = B[j * n + j];
-            if (d == 0.0) d = 1.0;
-            if (!(d > 0.0)) {
-                info[T] = j + 1;
-                d = 1.0;
-            }
-            d = sqrt(d);
-            B[j * n + j] = d;
-            dj = d;
+    // rows over waves, columns over lanes
+    for (int k = 0; k < n; ++k) {
+        double piv = B[k * n + k];
+        if (piv == 0.0) piv = 1.0;
+        if (!(piv > 0.0)) {
+            if (threadIdx.x == 0) info[T] = k + 1;
+            piv = 1.0;
         }
+        const double ip = 1.0 / piv;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) ck[i] = i == k ? 0.0 : B[i * n + k];
         __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) B[i * n + j] /= dj;
+        // row k scaled by 1 / pivot (its pivot entry replaced by 1 first)
+        for (int j = threadIdx.x; j < n; j += blockDim.x)
+            B[k * n + j] = (j == k ? 1.0 : B[k * n + j]) * ip;
         __syncthreads();
-        const int m = n - j - 1;
-        for (int q = threadIdx.x; q < m * m; q += blockDim.x) {
-            const int i = j + 1 + q / m, k = j + 1 + q % m;
-            if (k <= i) B[i * n + k] -= B[i * n + j] * B[k * n + j];
+        // a_ij -= a_ik a_kj for i != k, with a_ik = 0 first (so column k
+        // becomes -a_ik / pivot)
+        for (int j = lane; j < n; j += 64) {
+            const double bkj = B[k * n + j];
+            for (int i = wv; i < n; i += nw) {
+                if (i == k) continue;
+                B[i * n + j] = (j == k ? 0.0 : B[i * n + j]) - ck[i] * bkj;
+            }
         }
         __syncthreads();
     }
-    // column j of L^-1 by forward substitution (one thread per column); it
-    // reads only the lower triangle and writes row j of the upper one
-    for (int j = threadIdx.x; j < n; j += blockDim.x)
-        for (int i = j + 1; i < n; ++i) {
-            double acc = B[i * n + j] / B[j * n + j];   // k = j term: L_ij (L^-1)_jj
-            for (int k = j + 1; k < i; ++k) acc += B[i * n + k] * B[j * n + k];
-            B[j * n + i] = -acc / B[i * n + i];
-        }
+    double *M = blocks + block_off[T];
+    for (int i = wv; i < n; i += nw)
+        for (int j = lane; j < n; j += 64) M[i * n + j] = 0.5 * (B[i * n + j] + B[j * n + i]);
 }
 
-// z = (L L^T)^-1 r = L^-T (L^-1 r) on each tile, from the inverse factor
-// stored by k_fp_block_chol: two dense triangular products.
+// z = M r on one tile of n rows (M the block inverse, symmetric: z_i =
+// sum_k M[k][i] r_k, so each wave reads rows of M with lanes along i).  Four
+// waves split k; their partials meet in LDS in a fixed order.  rr (the tile's
+// r) and part (4 * 64 doubles) are LDS; blockDim.x must be 256.  Returns this
+// thread's share of sum_i r_i z_i.
+__device__ double tile_precond(const double *__restrict__ M, int n, const double *rr, double *part,
+                               double *__restrict__ z) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double rz = 0.0;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const int i = i0 + lane;
+        double acc = 0.0;
+        if (i < n) {
+#pragma unroll 8
+            for (int k = wv; k < n; k += 4) acc += M[k * n + i] * rr[k];
+        }
+        part[wv * 64 + lane] = acc;
+        __syncthreads();
+        if (wv == 0 && i < n) {
+            const double zi = part[lane] + part[64 + lane] + part[128 + lane] + part[192 + lane];
+            z[i] = zi;
+            rz += rr[i] * zi;
+        }
+        __syncthreads();
+    }
+    return rz;
+}
+
 __global__ void __launch_bounds__(256)
 k_fp_precond(const int32_t *__restrict__ tile_start, const int64_t *__restrict__ block_off,
-             const double *__restrict__ blocks, const double *__restrict__ r,
+             int max_n, const double *__restrict__ blocks, const double *__restrict__ r,
              double *__restrict__ z) {
-    extern __shared__ double y[];
-    double *rr = y + (tile_start[blockIdx.x + 1] - tile_start[blockIdx.x]);
+    extern __shared__ double part[];
     const int T = blockIdx.x;
-    const int t0 = tile_start[T], n = tile_start[T + 1] - t0;
-    const double *B = blocks + block_off[T];
+    const int t0 = tile_start[T], n = min(tile_start[T + 1] - t0, max_n);
+    double *rr = part + 256;
     for (int i = threadIdx.x; i < n; i += blockDim.x) rr[i] = r[t0 + i];
     __syncthreads();
-    // y_i = sum_{k < i} (L^-1)_ik r_k + r_i / L_ii,  (L^-1)_ik = B[k][i]
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double acc = rr[i] / B[i * n + i];
-        for (int k = 0; k < i; ++k) acc += B[k * n + i] * rr[k];
-        y[i] = acc;
+    tile_precond(blocks + block_off[T], n, rr, part, z + t0);
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients with every scalar on the device, two
+// launches per iteration:
+//   step k: alpha = r.z / p.q; x += alpha p; r -= alpha q; z = M r; new r.z, r.r
+//   dir k+1: stop if r.r <= tol^2 b.b; beta = r.z(new) / r.z(old);
+//            p = z + beta p; q = A z + beta q (= A p); p.q
+// Every launch has one 256-thread block per tile.  The dot products are
+// per-tile partials (one double per tile) that every block of the next launch
+// reduces in the same fixed order, so all blocks agree on alpha, beta and the
+// stopping test bit for bit, with no host round trip and no atomics.
+// status[0]: 0 running, 1 converged, 2 breakdown (p.Ap <= 0); status[1]:
+// steps taken.  Once status[0] is set the remaining launches return at once.
+constexpr int PCG_TB = 256;
+
+// v[j] <- the block's sum of v[j] (the same value in every thread; fixed
+// order, so every block reducing the same data gets the same bits).
+template <int NV>
+__device__ void block_reduce(double (&v)[NV]) {
+    __shared__ double red[NV][PCG_TB / 64];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const double w = wave_sum_d(v[j]);
+        if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = w;
     }
     __syncthreads();
-    // z_i = sum_{k > i} (L^-1)_ki y_k + y_i / L_ii,  (L^-1)_ki = B[i][k]
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double acc = y[i] / B[i * n + i];
-        for (int k = i + 1; k < n; ++k) acc += B[i * n + k] * y[k];
-        z[t0 + i] = acc;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        double s = 0.0;
+        for (int w = 0; w < PCG_TB / 64; ++w) s += red[j][w];
+        v[j] = s;
+    }
+    __syncthreads();
+}
+
+// out[j] = sum_i a[j][i] over m per-tile partials, and whether status[0] is
+// set.  status[0] is read once per block (its load overlapping the partials'):
+// another block may set it while this one runs, and every wave of a block
+// must take the same branch.
+template <int NV>
+__device__ bool partial_sums(const double *const (&a)[NV], int m, const int32_t *status,
+                             double (&out)[NV]) {
+    __shared__ int st;
+    if (threadIdx.x == 0) st = status[0];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) out[j] = 0.0;
+    for (int i = threadIdx.x; i < m; i += PCG_TB)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) out[j] += a[j][i];
+    block_reduce(out);   // its barriers publish st
+    return st != 0;
+}
+
+// x = 0, r = b, z = M b; per-tile r.z, r.r (= b.b) and b.b.
+__global__ void __launch_bounds__(PCG_TB)
+k_pcg_init(const int32_t *__restrict__ tile_start, const int64_t *__restrict__ block_off,
+           int max_n, const double *__restrict__ blocks, const double *__restrict__ b, double *__restrict__ x,
+           double *__restrict__ r, double *__restrict__ z, double *__restrict__ rzp,
+           double *__restrict__ rrp, double *__restrict__ bbp, int32_t *__restrict__ status) {
+    extern __shared__ double part[];
+    const int T = blockIdx.x;
+    const int t0 = tile_start[T], n = min(tile_start[T + 1] - t0, max_n);
+    double *rr = part + PCG_TB;
+    double v[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += PCG_TB) {
+        const double bi = b[t0 + i];
+        rr[i] = bi;
+        r[t0 + i] = bi;
+        x[t0 + i] = 0.0;
+        v[1] += bi * bi;
+    }
+    if (T == 0 && threadIdx.x == 0) {
+        status[0] = 0;
+        status[1] = 0;
+    }
+    __syncthreads();
+    v[0] = tile_precond(blocks + block_off[T], n, rr, part, z + t0);
+    block_reduce(v);
+    if (threadIdx.x == 0) {
+        rzp[T] = v[0];
+        rrp[T] = v[1];
+        bbp[T] = v[1];
+    }
+}
+
+// The stopping test, then p = z + beta p and q = A z + beta q over the tile's
+// rows (8 lanes per row) and the tile's p.q.  first: beta = 0 (p = z).
+__global__ void __launch_bounds__(PCG_TB)
+k_pcg_dir(const int32_t *__restrict__ tile_start, const int64_t *__restrict__ row_ptr,
+          const int32_t *__restrict__ col, const double *__restrict__ vals, int ntile,
+          const double *__restrict__ z, double *__restrict__ p, double *__restrict__ q,
+          const double *__restrict__ rz_new, const double *__restrict__ rz_old,
+          const double *__restrict__ rrp, const double *__restrict__ bbp,
+          double *__restrict__ pqp, double tol2, int32_t *__restrict__ status,
+          double *__restrict__ norms, int k, int first) {
+    const double *const parts[4] = {rz_new, rz_old, rrp, bbp};
+    double s[4];
+    if (partial_sums(parts, ntile, status, s)) return;
+    const bool done = s[2] <= tol2 * s[3];
+    if (threadIdx.x == 0 && (done || blockIdx.x == 0)) {
+        if (done) status[0] = 1;
+        status[1] = k;
+        norms[0] = s[2];
+        norms[1] = s[3];
+    }
+    if (done) return;
+    const double beta = first ? 0.0 : s[0] / s[1];
+    const int T = blockIdx.x;
+    const int t0 = tile_start[T], t1 = tile_start[T + 1];
+    const int g = threadIdx.x >> 3, gl = threadIdx.x & 7;
+    double v[1] = {0.0};
+    for (int row = t0 + g; row < t1; row += PCG_TB / 8) {
+        double acc = 0.0;
+        const int64_t e1 = row_ptr[row + 1];
+        // four entries per lane per pass: their loads issue together
+        for (int64_t e = row_ptr[row] + gl; e < e1; e += 32) {
+            int c[4];
+            double a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = e + 8 * u < e1;
+                c[u] = ok ? col[e + 8 * u] : row;
+                a[u] = ok ? vals[e + 8 * u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += a[u] * z[c[u]];
+        }
+#pragma unroll
+        for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (gl == 0) {
+            const double pi = first ? z[row] : z[row] + beta * p[row];
+            const double qi = first ? acc : acc + beta * q[row];
+            p[row] = pi;
+            q[row] = qi;
+            v[0] += pi * qi;
+        }
+    }
+    block_reduce(v);
+    if (threadIdx.x == 0) pqp[T] = v[0];
+}
+
+// alpha = r.z / p.q; x += alpha p; r -= alpha q; z = M r; the tile's new r.z
+// (into rz_next) and r.r.
+__global__ void __launch_bounds__(PCG_TB)
+k_pcg_step(const int32_t *__restrict__ tile_start, const int64_t *__restrict__ block_off,
+           int max_n, const double *__restrict__ blocks, int ntile, const double *__restrict__ p,
+           const double *__restrict__ q, double *__restrict__ x, double *__restrict__ r,
+           double *__restrict__ z, const double *__restrict__ rz_cur,
+           double *__restrict__ rz_next, const double *__restrict__ pqp, double *__restrict__ rrp,
+           int32_t *__restrict__ status, int k) {
+    extern __shared__ double part[];
+    const double *const parts[2] = {rz_cur, pqp};
+    double s[2];
+    if (partial_sums(parts, ntile, status, s)) return;
+    if (!(s[1] > 0.0)) {   // A not positive on p (the stopping test caught r = 0)
+        if (threadIdx.x == 0) {
+            status[0] = 2;
+            status[1] = k;
+        }
+        return;
+    }
+    const double alpha = s[0] / s[1];
+    const int T = blockIdx.x;
+    const int t0 = tile_start[T], n = min(tile_start[T + 1] - t0, max_n);
+    double *rr = part + PCG_TB;
+    double v[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += PCG_TB) {
+        x[t0 + i] += alpha * p[t0 + i];
+        const double ri = r[t0 + i] - alpha * q[t0 + i];
+        r[t0 + i] = ri;
+        rr[i] = ri;
+        v[1] += ri * ri;
+    }
+    __syncthreads();
+    v[0] = tile_precond(blocks + block_off[T], n, rr, part, z + t0);
+    block_reduce(v);
+    if (threadIdx.x == 0) {
+        rz_next[T] = v[0];
+        rrp[T] = v[1];
     }
 }
 
@@ -402,26 +611,32 @@ extern "C" int trh_fp_rhs(trh_handle *h, int n, const int32_t *win, const int64_
     return hip_check(hipGetLastError(), "k_fp_rhs");
 }
 
-extern "C" int trh_fp_block_chol(trh_handle *h, const int64_t *row_ptr, const int32_t *col,
-                                 const double *vals, const int32_t *tile_start,
-                                 const int64_t *block_off, int ntile, double *blocks,
-                                 int32_t *info, void *stream) {
+extern "C" int trh_fp_block_inv(trh_handle *h, const int64_t *row_ptr, const int32_t *col,
+                                const double *vals, const int32_t *tile_start,
+                                const int64_t *block_off, int ntile, int max_n, double *blocks,
+                                int32_t *info, void *stream) {
+    const size_t lds = ((size_t)max_n * max_n + max_n) * sizeof(double) + (max_n + 1) * 4;
     if (!h || !row_ptr || !col || !vals || !tile_start || !block_off || !blocks || !info ||
-        ntile <= 0)
-        return set_err(TRH_E_ARG, "trh_fp_block_chol: bad arguments");
-    hipLaunchKernelGGL(k_fp_block_chol, dim3(ntile), dim3(256), 0, (hipStream_t)stream, row_ptr,
-                       col, vals, tile_start, block_off, blocks, info);
-    return hip_check(hipGetLastError(), "k_fp_block_chol");
+        ntile <= 0 || max_n <= 0 || lds + 256 > (size_t)h->lds_max)
+        return set_err(TRH_E_ARG, "trh_fp_block_inv: bad arguments (max_n %d)", max_n);
+    hipLaunchKernelGGL(k_fp_block_inv, dim3(ntile), dim3(256), lds, (hipStream_t)stream, row_ptr,
+                       col, vals, tile_start, block_off, max_n, blocks, info);
+    return hip_check(hipGetLastError(), "k_fp_block_inv");
+}
+
+static bool pcg_lds_ok(const trh_handle *h, int max_n) {
+    return max_n > 0 && ((size_t)PCG_TB + max_n) * sizeof(double) + 256 <= (size_t)h->lds_max;
 }
 
 extern "C" int trh_fp_precond(trh_handle *h, const int32_t *tile_start, const int64_t *block_off,
                               int ntile, int max_n, const double *blocks, const double *r,
                               double *z, void *stream) {
-    if (!h || !tile_start || !block_off || !blocks || !r || !z || ntile <= 0 || max_n <= 0 ||
-        2 * (size_t)max_n * sizeof(double) > (size_t)h->lds_max)
+    if (!h || !tile_start || !block_off || !blocks || !r || !z || ntile <= 0 ||
+        !pcg_lds_ok(h, max_n))
         return set_err(TRH_E_ARG, "trh_fp_precond: bad arguments (max_n %d)", max_n);
-    hipLaunchKernelGGL(k_fp_precond, dim3(ntile), dim3(256), 2 * sizeof(double) * (size_t)max_n,
-                       (hipStream_t)stream, tile_start, block_off, blocks, r, z);
+    hipLaunchKernelGGL(k_fp_precond, dim3(ntile), dim3(PCG_TB),
+                       ((size_t)PCG_TB + max_n) * sizeof(double), (hipStream_t)stream, tile_start,
+                       block_off, max_n, blocks, r, z);
     return hip_check(hipGetLastError(), "k_fp_precond");
 }
 
@@ -432,6 +647,45 @@ extern "C" int trh_fp_spmv(trh_handle *h, int n, const int64_t *row_ptr, const i
     hipLaunchKernelGGL(k_fp_spmv, dim3(grid_waves(h, n)), dim3(256), 0, (hipStream_t)stream, n,
                        row_ptr, col, vals, x, y);
     return hip_check(hipGetLastError(), "k_fp_spmv");
+}
+
+extern "C" int trh_fp_pcg(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
+                          const double *vals, const int32_t *tile_start, const int64_t *block_off,
+                          int ntile, int max_n, const double *blocks, const double *b, double tol,
+                          int it0, int iters, double *x, double *work, int32_t *status,
+                          void *stream) {
+    if (!h || n <= 0 || !row_ptr || !col || !vals || !tile_start || !block_off || ntile <= 0 ||
+        ntile > n || !blocks || !b || !x || !work || !status || it0 < 0 || iters < 0 ||
+        !(tol >= 0.0) || !pcg_lds_ok(h, max_n))
+        return set_err(TRH_E_ARG, "trh_fp_pcg: bad arguments (n %d, ntile %d, max_n %d)", n,
+                       ntile, max_n);
+    hipStream_t st = (hipStream_t)stream;
+    double *r = work, *z = r + n, *p = z + n, *q = p + n;
+    double *rzp = q + n, *rrp = rzp + 2 * (size_t)ntile, *pqp = rrp + ntile,
+           *bbp = pqp + ntile, *norms = bbp + ntile;
+    const size_t lds = ((size_t)PCG_TB + max_n) * sizeof(double);
+    const double tol2 = tol * tol;
+    // r.z after k steps lives in rzp[k & 1]
+    auto rz = [&](int k) { return rzp + (size_t)(k & 1) * ntile; };
+    int rc;
+    if (it0 == 0) {
+        hipLaunchKernelGGL(k_pcg_init, dim3(ntile), dim3(PCG_TB), lds, st, tile_start, block_off,
+                           max_n, blocks, b, x, r, z, rz(0), rrp, bbp, status);
+        hipLaunchKernelGGL(k_pcg_dir, dim3(ntile), dim3(PCG_TB), 0, st, tile_start, row_ptr, col,
+                           vals, ntile, z, p, q, rz(0), rz(1), rrp, bbp, pqp, tol2, status, norms,
+                           0, 1);
+        if ((rc = hip_check(hipGetLastError(), "k_pcg_init"))) return rc;
+    }
+    for (int k = it0; k < it0 + iters; ++k) {
+        hipLaunchKernelGGL(k_pcg_step, dim3(ntile), dim3(PCG_TB), lds, st, tile_start, block_off,
+                           max_n, blocks, ntile, p, q, x, r, z, rz(k), rz(k + 1), pqp, rrp,
+                           status, k);
+        hipLaunchKernelGGL(k_pcg_dir, dim3(ntile), dim3(PCG_TB), 0, st, tile_start, row_ptr, col,
+                           vals, ntile, z, p, q, rz(k + 1), rz(k), rrp, bbp, pqp, tol2, status,
+                           norms, k + 1, 0);
+        if ((rc = hip_check(hipGetLastError(), "k_pcg"))) return rc;
+    }
+    return TRH_OK;
 }
 
 extern "C" int trh_fp_add_patches(trh_handle *h, int n, const int32_t *win, const int64_t *poff,

@@ -13,11 +13,12 @@ a spatially varying pixelized PSF:
    (HoggGalaxy._getUnitFluxPatchSize, galaxy.py:678-691 and the FFT path,
    galaxy.py:336-508; PixelizedPSF.getPointSourcePatch, psf.py:137-207),
    rendered in batches -- trh_render_fft_batch / trh_render_point_batch;
-3. the normal equations over overlapping pairs (FP32 products, FP64 sums) --
-   trh_fp_pair_dots / trh_fp_rhs;
-4. an FP64 solve: conjugate gradients preconditioned by the exact Cholesky
-   factors of the per-tile diagonal blocks -- trh_fp_block_chol,
-   trh_fp_precond, trh_fp_spmv.
+3. the sparsity pattern of the normal equations (trh_fp_overlaps) and their
+   values over overlapping pairs (FP32 products, FP64 sums) --
+   trh_fp_pair_dots_csr / trh_fp_rhs;
+4. an FP64 solve: conjugate gradients preconditioned by the exact inverses
+   of the diagonal blocks of at most 64 consecutive sources in tile order --
+   trh_fp_block_inv, trh_fp_pcg.
 
 The result is the least-squares flux vector LSQR converges to; `iv` are the
 inverse variances sum (u inverr)^2 (Optimizer._get_iv, optimize.py:378-423).
@@ -195,11 +196,13 @@ class BatchForcedPhotometry(object):
     PSF grid; psf_idx / psf_w: (n, 4) bilinear weights of each source.
     kind: 0 point, 1 exp, 2 dev; pos: (n, 2) pixels; shape: (n, 3)
     EllipseESoft (log r_e, ee1, ee2) of the galaxies.  Sources are processed
-    in tile order (tile x tile pixel cells) so the preconditioner blocks are
-    contiguous; results come back in the caller's order."""
+    in tile order (tile x tile pixel cells; by default the tile holds about
+    40 sources), Morton order inside a tile, so the preconditioner blocks (at
+    most max_block consecutive sources of one tile) are contiguous and
+    spatially compact; results come back in the caller's order."""
 
-    def __init__(self, data, inverr, psf_basis, psf_idx, psf_w, kind, pos, shape, tile=128,
-                 device=0):
+    def __init__(self, data, inverr, psf_basis, psf_idx, psf_w, kind, pos, shape, tile=None,
+                 max_block=64, device=0):
         self.lib = _lib.load()
         self.h = _lib.handle(device)
         self.dev = torch.device('cuda', device)
@@ -213,11 +216,22 @@ class BatchForcedPhotometry(object):
         assert self.ph % 2 == 1 and self.pw % 2 == 1
         n = len(kind)
         self.n = n
+        if tile is None:
+            # about 40 sources per tile (multiple of 16 pixels, 32..256)
+            tile = int(np.clip(round(math.sqrt(40. * self.H * self.W / max(n, 1)) / 16) * 16,
+                               32, 256))
         ntx = (self.W + tile - 1) // tile
         self.tile, self.ntx, self.nty = tile, ntx, (self.H + tile - 1) // tile
         tid = (np.clip(pos[:, 1], 0, self.H - 1) // tile).astype(np.int64) * ntx + \
             (np.clip(pos[:, 0], 0, self.W - 1) // tile).astype(np.int64)
-        self.order = np.argsort(tid, kind='stable')
+        # inside a tile, Morton order over 8x8 sub-cells, so that a run of
+        # consecutive sources (a preconditioner block) is spatially compact
+        cx = (np.clip(pos[:, 0], 0, self.W - 1) % tile * 8 // tile).astype(np.int64)
+        cy = (np.clip(pos[:, 1], 0, self.H - 1) % tile * 8 // tile).astype(np.int64)
+        mort = np.zeros(n, np.int64)
+        for bit in range(3):
+            mort |= ((cx >> bit) & 1) << (2 * bit) | ((cy >> bit) & 1) << (2 * bit + 1)
+        self.order = np.argsort(tid * 64 + mort, kind='stable')
         o = self.order
         self.stile = tid[o].astype(np.int32)
         self.kind = np.asarray(kind)[o]
@@ -227,6 +241,10 @@ class BatchForcedPhotometry(object):
         self.psf_w = np.ascontiguousarray(np.asarray(psf_w)[o], np.float32)
         ts = np.bincount(tid[o], minlength=int(tid.max()) + 1)
         ts = ts[ts > 0]
+        # preconditioner blocks: each tile's sources, split into near-equal
+        # runs of at most max_block (each block is factored in LDS)
+        parts = -(-ts // max_block)
+        ts = np.concatenate([np.full(k, c // k) + (np.arange(k) < c % k) for c, k in zip(ts, parts)])
         self.tile_start = np.zeros(len(ts) + 1, np.int32)
         self.tile_start[1:] = np.cumsum(ts)
         self.stream = torch.cuda.current_stream(self.dev).cuda_stream
@@ -371,9 +389,12 @@ class BatchForcedPhotometry(object):
               'fp_rhs')
         return self
 
-    def solve(self, tol=1e-12, maxiter=1000, check_every=10):
+    def solve(self, tol=1e-12, maxiter=1000, chunk=8):
         """FP64 conjugate gradients preconditioned by the tile blocks' exact
-        Cholesky factors.  Returns fluxes in the caller's order."""
+        Cholesky factors (trh_fp_pcg: every scalar stays on the device).
+        Iterations are enqueued `chunk` at a time; the stop flag of one chunk
+        is read while the next one runs.  Returns fluxes in the caller's
+        order."""
         lib, h, st = self.lib, self.h.value, self.stream
         d_rp = self.d_row_ptr
         d_col = self.d_col
@@ -383,37 +404,37 @@ class BatchForcedPhotometry(object):
         blocks = torch.empty(int(np.sum(np.diff(self.tile_start).astype(np.int64) ** 2)),
                              dtype=torch.float64, device=self.dev)
         info = torch.zeros(nt, dtype=torch.int32, device=self.dev)
-        check(lib.trh_fp_block_chol(h, ptr(d_rp), ptr(d_col), ptr(self.vals), ptr(d_ts), ptr(d_bo),
-                                    nt, ptr(blocks), ptr(info), st), 'fp_block_chol')
+        check(lib.trh_fp_block_inv(h, ptr(d_rp), ptr(d_col), ptr(self.vals), ptr(d_ts), ptr(d_bo),
+                                    nt, self.max_tile, ptr(blocks), ptr(info), st), 'fp_block_inv')
         n = self.n
-        x = torch.zeros(n, dtype=torch.float64, device=self.dev)
-        r = self.b.clone()
-        z = torch.empty_like(r)
-        q = torch.empty_like(r)
-
-        def M(src, dst):
-            check(lib.trh_fp_precond(h, ptr(d_ts), ptr(d_bo), nt, self.max_tile, ptr(blocks),
-                                     ptr(src), ptr(dst), st), 'fp_precond')
-
-        M(r, z)
-        p = z.clone()
-        rz = torch.dot(r, z)
-        bnorm = float(torch.linalg.norm(self.b))
-        it = 0
-        for it in range(1, maxiter + 1):
-            check(lib.trh_fp_spmv(h, n, ptr(d_rp), ptr(d_col), ptr(self.vals), ptr(p), ptr(q), st),
-                  'fp_spmv')
-            alpha = rz / torch.dot(p, q)
-            x.add_(alpha * p)
-            r.sub_(alpha * q)
-            if it % check_every == 0 and float(torch.linalg.norm(r)) <= tol * max(bnorm, 1e-300):
-                break
-            M(r, z)
-            rz_new = torch.dot(r, z)
-            p.mul_(rz_new / rz).add_(z)
-            rz = rz_new
-        self.iterations = it
-        self.resid = float(torch.linalg.norm(r)) / max(bnorm, 1e-300)
+        x = torch.empty(n, dtype=torch.float64, device=self.dev)
+        work = torch.empty(4 * n + 5 * nt + 2, dtype=torch.float64, device=self.dev)
+        status = torch.empty(2, dtype=torch.int32, device=self.dev)
+        pending = []
+        it0 = 0
+        while True:
+            if it0 < maxiter:
+                k = min(chunk, maxiter - it0)
+                check(lib.trh_fp_pcg(h, n, ptr(d_rp), ptr(d_col), ptr(self.vals), ptr(d_ts),
+                                     ptr(d_bo), nt, self.max_tile, ptr(blocks), ptr(self.b),
+                                     float(tol), it0, k, ptr(x), ptr(work), ptr(status), st),
+                      'fp_pcg')
+                it0 += k
+                hs = torch.empty(2, dtype=torch.int32, pin_memory=True)
+                hs.copy_(status, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record()
+                pending.append((hs, ev))
+            if len(pending) > 1 or it0 >= maxiter:
+                hs, ev = pending.pop(0)
+                ev.synchronize()
+                if int(hs[0]) != 0 or not pending:
+                    break
+        torch.cuda.synchronize(self.dev)
+        self.status = int(hs[0])
+        self.iterations = int(hs[1])
+        rr, bb = work[-2:].cpu().tolist()
+        self.resid = float(np.sqrt(rr / bb)) if bb > 0 else 0.0
         self.info = info.cpu().numpy()
         flux = np.empty(n)
         flux[self.order] = x.cpu().numpy()
