@@ -238,6 +238,23 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  *                  arrays of `len` floats (bilinear PSF interpolation of
  *                  stamps or of spectra; replaces PixelizedPsfEx's
  *                  per-position stamp, tractor/psfex.py:481-520); n <= 65535.
+ *   fp_keys:       sort key of each source (src rows: x, y, kind, log r_e,
+ *                  ee1, ee2): its tile cell (tile x tile pixels of the
+ *                  clipped position, row-major) * 64 + the Morton index of
+ *                  its 8 x 8 sub-cell.
+ *   fp_plan:       for q < n, source order[q]'s unit-flux patch window
+ *                  (win[q] = x0, y0, w, h inside the image), render bucket
+ *                  (0 point, 1..5 FFT size 32..512, 6 empty window, 7 FFT
+ *                  size out of range), cell stile[q] = key >> 6, area,
+ *                  reach extent and TRH_STORE render job (out_offset and psf
+ *                  left 0): the no-modelMask patches of
+ *                  PixelizedPSF.getPointSourcePatch (tractor/psf.py:137-207)
+ *                  and of the galaxies' FFT path (HoggGalaxy.
+ *                  _getUnitFluxPatchSize, tractor/galaxy.py:678-691;
+ *                  galaxy.py:336-508).  mix: two rows (exp, dev) of
+ *                  1 + 2 TRH_MAXK doubles: K, amplitudes, variances.
+ *   fp_jobs_bucketed: dst[j] = src[border[j]] with out_offset = poff[q]
+ *                  and psf = j - bstart[bucket[q]] (q = border[j]).
  *   fp_overlaps:   the sparsity pattern of A (the sources whose patches share
  *                  pixels: the columns LsqrOptimizer stacks into its sparse
  *                  matrix, tractor/lsqr_optimizer.py:386-431).  Sources are
@@ -281,6 +298,15 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  */
 int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
                     const float *weights, const int32_t *idx, float *out, int n, void *stream);
+int trh_fp_keys(trh_handle *h, int n, const double *src, int img_h, int img_w, int tile,
+                int32_t *key, void *stream);
+int trh_fp_plan(trh_handle *h, int n, const int64_t *order, const double *src, const int32_t *key,
+                int img_h, int img_w, int ph, int pw, const double *mix, int32_t *win,
+                int32_t *bucket, int32_t *stile, int64_t *area, double *ext, trh_fft_job *jobs,
+                void *stream);
+int trh_fp_jobs_bucketed(trh_handle *h, int n, const int64_t *border, const int32_t *bucket,
+                         const int64_t *bstart, const int64_t *poff, const trh_fft_job *src,
+                         trh_fft_job *dst, void *stream);
 int trh_fp_overlaps(trh_handle *h, int n, const int32_t *win, const int32_t *stile, int ntx,
                     int nty, const int32_t *tile_first, const int32_t *tile_end, int reach,
                     int32_t *counts, const int64_t *row_ptr, int32_t *col, int64_t *diag,

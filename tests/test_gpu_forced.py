@@ -92,6 +92,56 @@ def test_device_overlap_pattern_matches_host_rule():
     assert fp.reach >= 1
 
 
+def test_device_plan_matches_host_plan():
+    """trh_fp_keys + trh_fp_plan + trh_fp_jobs_bucketed against the host
+    restatement (plan_patches) on a scene with sources off the image and on
+    its edges: windows, render buckets, job fields, patch offsets and each
+    job's rank inside its bucket."""
+    import torch
+    from tractor_amd import forced as F, workloads as W
+    from tractor_amd._lib import FFT_JOB
+    size, n = 512, 1500
+    rng = np.random.default_rng(11)
+    kind, pos, flux, shape = W.draw_forced_scene(n, size, rng=rng)
+    pos[:9] = [[-40., 10.], [600., 30.], [5., -3.], [511.5, 511.5], [0., 0.], [250., 700.],
+               [-200., -200.], [-0.5, 100.], [511.49, 3.]]
+    basis = W.psf_grid_stamps()
+    idx, w = W.bilinear_psf_weights(np.clip(pos[:, 0], 0, size - 1), np.clip(pos[:, 1], 0, size - 1),
+                                    size, size)
+    data = torch.zeros((size, size), dtype=torch.float32, device='cuda')
+    fp = F.BatchForcedPhotometry(data, torch.ones_like(data), basis, idx, w, kind, pos, shape, tile=64)
+    fp.plan()
+    torch.cuda.synchronize()
+    o = fp.order
+    assert np.array_equal(np.sort(o), np.arange(n))
+    P = F.plan_patches(np.asarray(kind)[o], pos[o], shape[o], (size, size), basis.shape[1:])
+    assert np.array_equal(fp.win, P['win'])
+    area = P['win'][:, 2] * P['win'][:, 3]
+    sz = P['sz']
+    want = np.where(sz == 0, 0, 1 + np.log2(np.maximum(sz // 32, 1)).astype(np.int64))
+    want = np.where((sz > 0) & ((sz < 32) | (sz > 512)), 7, want)
+    want = np.where(area == 0, 6, want)
+    assert np.array_equal(fp.bucket, want)
+    assert np.array_equal(fp.poff, np.cumsum(area) - area)
+    border = fp.d_border.cpu().numpy()
+    Jb = fp.d_jobs.cpu().numpy().view(FFT_JOB).reshape(-1)
+    J = np.empty_like(Jb)
+    J[border] = Jb
+    ref = P['job']
+    for f in ('sx', 'sy', 'mw', 'mh', 'K', 'mode', 'scale', 'amp'):
+        assert np.array_equal(J[f], ref[f]), f
+    assert np.array_equal(J['out_stride'], np.maximum(P['win'][:, 2], 1))
+    assert np.array_equal(J['out_offset'], fp.poff)
+    np.testing.assert_allclose(J['mux'], ref['mux'], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(J['muy'], ref['muy'], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(J['var'], ref['var'], rtol=2e-7, atol=0)
+    # rank inside the bucket, in tile order
+    bk = fp.bucket[border]
+    starts = np.concatenate([[0], np.cumsum(np.bincount(bk, minlength=8))[:-1]])
+    assert np.array_equal(Jb['psf'], np.arange(n) - starts[bk])
+    assert np.all(np.diff(bk) >= 0) and np.all(np.diff(border)[np.diff(bk) == 0] > 0)
+
+
 def _tiled_spd_system(rng, sizes, density=0.15):
     """A symmetric positive definite sparse matrix over tiles of the given
     sizes (couplings inside and between neighbouring tiles), as CSR."""

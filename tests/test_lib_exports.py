@@ -65,6 +65,9 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_solve_normal_f64': (N, N, 3, 0., 0, N, N, N),
         'trh_set_fft_config': (N, 64, 0, 0),
         'trh_psf_combine': (N, N, 9, 100, N, N, N, 1, N),
+        'trh_fp_keys': (N, 1, N, 10, 10, 4, N, N),
+        'trh_fp_plan': (N, 1, N, N, N, 10, 10, 3, 3, N, N, N, N, N, N, N, N),
+        'trh_fp_jobs_bucketed': (N, 1, N, N, N, N, N, N, N),
         'trh_fp_overlaps': (N, 1, N, N, 1, 1, N, N, 0, N, N, N, N, N),
         'trh_fp_pair_dots_csr': (N, 1, N, N, N, N, N, N, 10, N, N),
         'trh_fp_pair_dots': (N, N, N, 1, N, N, N, N, 10, N, N),

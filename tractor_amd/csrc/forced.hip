@@ -544,6 +544,179 @@ k_fp_pair_dots_csr(int n, const int64_t *__restrict__ row_ptr, const int32_t *__
         }
     }
 }
+// ---------------------------------------------------------------------------
+// Planning on the device: sort keys, the unit-flux patch of every source and
+// the render jobs grouped by render kind.
+
+// Sort key of source s: the tile cell of its clipped position (tile x tile
+// pixels, row-major over ntx cells) times 64 plus the Morton index of its
+// 8 x 8 sub-cell, so that a run of consecutive sources is spatially compact.
+__global__ void __launch_bounds__(256)
+k_fp_keys(int n, const double *__restrict__ src, int img_h, int img_w, int tile, int ntx,
+          int32_t *__restrict__ key) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double x = fmin(fmax(src[6 * (int64_t)s], 0.0), (double)(img_w - 1));
+    const double y = fmin(fmax(src[6 * (int64_t)s + 1], 0.0), (double)(img_h - 1));
+    const int cx = (int)floor(x / tile), cy = (int)floor(y / tile);
+    const int ux = min(7, (int)floor(fmod(x, (double)tile) * 8.0 / tile));
+    const int uy = min(7, (int)floor(fmod(y, (double)tile) * 8.0 / tile));
+    int m = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) m |= ((ux >> b) & 1) << (2 * b) | ((uy >> b) & 1) << (2 * b + 1);
+    key[s] = (cy * ntx + cx) * 64 + m;
+}
+
+__device__ __forceinline__ int64_t pad_center(int64_t sz, int64_t p) {
+    return sz >= p ? (sz - p) / 2 + p / 2 : p / 2 - (p - sz) / 2;
+}
+
+// Source order[q] -> its unit-flux patch window (clipped to the image), its
+// render bucket, window area, reach extent and render job, written at q.
+// src rows: x, y, kind, log r_e, ee1, ee2 (EllipseESoft).  mix rows (exp,
+// dev): K, then TRH_MAXK amplitudes, then TRH_MAXK variances (circular, in
+// units of r_e^2).
+//  * point sources (PixelizedPSF.getPointSourcePatch, psf.py:137-207, no
+//    modelMask): the Lanczos-shifted stamp at (round(x) - pw/2,
+//    round(y) - ph/2), clipped;
+//  * galaxies: HoggGalaxy._getUnitFluxPatchSize (galaxy.py:678-691, NullWCS
+//    pixel scale 1; half-size nre r_e, nre = 4 exp / 8 dev, at least 1,
+//    at most the distance to the far image edge, at least the PSF's
+//    half-diagonal), the window of get_overlapping_region (patch.pyx:20-66),
+//    and the FFT job of galaxy.py:336-508 (size the next power of two >= 2 hs,
+//    the PSF centred by the same rule as the spectrum padding, the sheared
+//    MoG var' = v T T^T, T the EllipseESoft basis in pixels,
+//    ellipses.py:126-155, 232-255).
+// bucket: 0 point, 1 + log2(sz / 32) for sz in 32..512, 6 empty window,
+// 7 an FFT size outside 32..512.  ext: the largest distance from the clipped
+// position to the window's far edges (0 for an empty window).
+__global__ void __launch_bounds__(256)
+k_fp_plan(int n, const int64_t *__restrict__ order, const double *__restrict__ src,
+          const int32_t *__restrict__ key, int img_h, int img_w, int ph, int pw,
+          const double *__restrict__ mix, int32_t *__restrict__ win, int32_t *__restrict__ bucket,
+          int32_t *__restrict__ stile, int64_t *__restrict__ area, double *__restrict__ ext,
+          trh_fft_job *__restrict__ jobs) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t s = order[q];
+    const double *r = src + 6 * s;
+    const double gx = r[0], gy = r[1];
+    const int kind = (int)r[2];
+    const int64_t W = img_w, H = img_h;
+    trh_fft_job J;
+    J.out_offset = 0;
+    J.psf = 0;
+    J.K = 0;
+    J.mode = TRH_STORE;
+    J.slot = 0;
+    J.pad0 = 0;
+    J.scale = 1.f;
+    J.pad1 = 0.f;
+    for (int k = 0; k < TRH_MAXK; ++k) {
+        J.amp[k] = 0.f;
+        J.var[k][0] = J.var[k][1] = J.var[k][2] = 0.f;
+    }
+    int64_t wx0, wy0, ww, wh;
+    int b;
+    if (kind == 0) {
+        const double ix = rint(gx), iy = rint(gy);
+        const int64_t x0 = (int64_t)ix - pw / 2, y0 = (int64_t)iy - ph / 2;
+        const int64_t X0 = max(x0, (int64_t)0), Y0 = max(y0, (int64_t)0);
+        const int64_t X1 = min(x0 + pw, W), Y1 = min(y0 + ph, H);
+        wx0 = X0;
+        wy0 = Y0;
+        ww = max(X1 - X0, (int64_t)0);
+        wh = max(Y1 - Y0, (int64_t)0);
+        J.sx = (int)(x0 - X0);
+        J.sy = (int)(y0 - Y0);
+        J.mux = gx - ix;
+        J.muy = gy - iy;
+        b = 0;
+    } else {
+        const double *m = mix + (kind == 1 ? 0 : 1) * (1 + 2 * TRH_MAXK);
+        const double nre = kind == 1 ? 4.0 : 8.0;
+        const double re = exp(r[3]);
+        double hs = fmax(1.0, nre * re);
+        const double imsize = fmax(fmax(gx, (double)(W - 1) - gx), fmax(gy, (double)(H - 1) - gy)) + 1.0;
+        hs = fmin(hs, imsize);
+        hs = fmax(hs, hypot(ph / 2.0, pw / 2.0));
+        const int64_t ihs = (int64_t)ceil(hs);
+        const int64_t xlo = (int64_t)floor(gx - ihs), xhi = (int64_t)ceil(gx + ihs + 1);
+        const int64_t ylo = (int64_t)floor(gy - ihs), yhi = (int64_t)ceil(gy + ihs + 1);
+        const int64_t gx0 = max(xlo, (int64_t)0), gx1 = min(xhi, W - 1) + 1;
+        const int64_t gy0 = max(ylo, (int64_t)0), gy1 = min(yhi, H - 1) + 1;
+        int64_t sz = 1;
+        while (sz < 2 * ihs) sz <<= 1;
+        const double dx = gx - pad_center(sz, pw), dy = gy - pad_center(sz, ph);
+        const double rx = rint(dx), ry = rint(dy);
+        const int64_t ix0 = (int64_t)rx, iy0 = (int64_t)ry;
+        const int64_t WX0 = max(ix0, gx0), WY0 = max(iy0, gy0);
+        const int64_t WX1 = min(ix0 + sz, gx1), WY1 = min(iy0 + sz, gy1);
+        const bool empty = xlo > W - 1 || xhi < 0 || ylo > H - 1 || yhi < 0;
+        wx0 = empty ? 0 : WX0;
+        wy0 = empty ? 0 : WY0;
+        ww = empty ? 0 : max(WX1 - WX0, (int64_t)0);
+        wh = empty ? 0 : max(WY1 - WY0, (int64_t)0);
+        J.sx = (int)(ix0 - WX0);
+        J.sy = (int)(iy0 - WY0);
+        J.mux = dx - rx;
+        J.muy = dy - ry;
+        const double ee = hypot(r[4], r[5]);
+        const double e = 1.0 - exp(-ee);
+        const double th = atan2(r[5], r[4]) / 2.0;
+        const double ct = cos(th), st = sin(th);
+        const double ab = e >= 1.0 ? 1000.0 : fmin(1000.0, (1.0 + e) / fmax(1.0 - e, 1e-300));
+        const double T00 = re * ct / ab, T01 = re * st, T10 = -re * st / ab, T11 = re * ct;
+        const double M00 = T00 * T00 + T01 * T01, M01 = T00 * T10 + T01 * T11,
+                     M11 = T10 * T10 + T11 * T11;
+        const int K = (int)m[0];
+        J.K = K;
+        for (int k = 0; k < K; ++k) {
+            const double v = m[1 + TRH_MAXK + k];
+            J.amp[k] = (float)m[1 + k];
+            J.var[k][0] = (float)(v * M00);
+            J.var[k][1] = (float)(v * M01);
+            J.var[k][2] = (float)(v * M11);
+        }
+        b = sz == 32 ? 1 : sz == 64 ? 2 : sz == 128 ? 3 : sz == 256 ? 4 : sz == 512 ? 5 : 7;
+    }
+    if (ww == 0 || wh == 0) b = 6;
+    J.mw = (int)ww;
+    J.mh = (int)wh;
+    J.out_stride = (int)max(ww, (int64_t)1);
+    win[4 * q] = (int)wx0;
+    win[4 * q + 1] = (int)wy0;
+    win[4 * q + 2] = (int)ww;
+    win[4 * q + 3] = (int)wh;
+    bucket[q] = b;
+    stile[q] = key[s] >> 6;
+    area[q] = ww * wh;
+    double e = 0.0;
+    if (ww > 0 && wh > 0) {
+        const double px = fmin(fmax(gx, 0.0), (double)(W - 1));
+        const double py = fmin(fmax(gy, 0.0), (double)(H - 1));
+        e = fmax(fmax(px - wx0, wx0 + ww - px), fmax(py - wy0, wy0 + wh - py));
+    }
+    ext[q] = e;
+    jobs[q] = J;
+}
+
+// Jobs regrouped by render bucket: dst[j] = src[border[j]] with the patch's
+// offset poff[q], and psf = the job's rank inside its bucket (the index of its
+// interpolated stamp or spectrum).
+__global__ void __launch_bounds__(256)
+k_fp_jobs_bucketed(int n, const int64_t *__restrict__ border, const int32_t *__restrict__ bucket,
+                   const int64_t *__restrict__ bstart, const int64_t *__restrict__ poff,
+                   const trh_fft_job *__restrict__ src, trh_fft_job *__restrict__ dst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t q = border[j];
+    trh_fft_job J = src[q];
+    J.out_offset = poff[q];
+    J.psf = (int)(j - bstart[bucket[q]]);
+    dst[j] = J;
+}
+
 }  // namespace
 
 extern "C" int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
@@ -697,4 +870,45 @@ extern "C" int trh_fp_add_patches(trh_handle *h, int n, const int32_t *win, cons
     hipLaunchKernelGGL(k_fp_add_patches, dim3(grid_waves(h, n)), dim3(256), 0,
                        (hipStream_t)stream, n, win, poff, patches, flux, img_w, img);
     return hip_check(hipGetLastError(), "k_fp_add_patches");
+}
+
+extern "C" int trh_fp_keys(trh_handle *h, int n, const double *src, int img_h, int img_w,
+                           int tile, int32_t *key, void *stream) {
+    if (!h || n < 0 || (n > 0 && (!src || !key)) || img_h <= 0 || img_w <= 0 || tile <= 0)
+        return set_err(TRH_E_ARG, "trh_fp_keys: bad arguments");
+    const int ntx = (img_w + tile - 1) / tile, nty = (img_h + tile - 1) / tile;
+    if ((int64_t)ntx * nty * 64 > INT32_MAX)
+        return set_err(TRH_E_ARG, "trh_fp_keys: %d x %d cells exceed the key range", ntx, nty);
+    if (n == 0) return TRH_OK;
+    hipLaunchKernelGGL(k_fp_keys, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, src,
+                       img_h, img_w, tile, ntx, key);
+    return hip_check(hipGetLastError(), "k_fp_keys");
+}
+
+extern "C" int trh_fp_plan(trh_handle *h, int n, const int64_t *order, const double *src,
+                           const int32_t *key, int img_h, int img_w, int ph, int pw,
+                           const double *mix, int32_t *win, int32_t *bucket, int32_t *stile,
+                           int64_t *area, double *ext, trh_fft_job *jobs, void *stream) {
+    if (!h || n < 0 ||
+        (n > 0 && (!order || !src || !key || !mix || !win || !bucket || !stile || !area || !ext ||
+                   !jobs)) ||
+        img_h <= 0 || img_w <= 0 || ph <= 0 || pw <= 0)
+        return set_err(TRH_E_ARG, "trh_fp_plan: bad arguments");
+    if (n == 0) return TRH_OK;
+    hipLaunchKernelGGL(k_fp_plan, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                       order, src, key, img_h, img_w, ph, pw, mix, win, bucket, stile, area, ext,
+                       jobs);
+    return hip_check(hipGetLastError(), "k_fp_plan");
+}
+
+extern "C" int trh_fp_jobs_bucketed(trh_handle *h, int n, const int64_t *border,
+                                    const int32_t *bucket, const int64_t *bstart,
+                                    const int64_t *poff, const trh_fft_job *src, trh_fft_job *dst,
+                                    void *stream) {
+    if (!h || n < 0 || (n > 0 && (!border || !bucket || !bstart || !poff || !src || !dst)))
+        return set_err(TRH_E_ARG, "trh_fp_jobs_bucketed: bad arguments");
+    if (n == 0) return TRH_OK;
+    hipLaunchKernelGGL(k_fp_jobs_bucketed, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, n, border, bucket, bstart, poff, src, dst);
+    return hip_check(hipGetLastError(), "k_fp_jobs_bucketed");
 }

@@ -190,6 +190,22 @@ def csr_of_pairs(pairs, n):
     return row_ptr, cols[order].astype(np.int32), pos
 
 
+FFT_SIZES = (32, 64, 128, 256, 512)
+B_POINT, B_EMPTY, B_BAD, NBUCKET = 0, 6, 7, 8   # trh_fp_plan's buckets; 1..5 FFT sizes
+
+
+def mixture_table():
+    """The exp and dev profiles as trh_fp_plan's mix rows: K, amplitudes,
+    variances (circular, in units of r_e^2)."""
+    t = np.zeros((2, 1 + 2 * _lib.TRH_MAXK))
+    for row, mix in enumerate((get_exp_mixture(), get_dev_mixture())):
+        K = len(mix.amp)
+        t[row, 0] = K
+        t[row, 1:1 + K] = mix.amp
+        t[row, 1 + _lib.TRH_MAXK:1 + _lib.TRH_MAXK + K] = mix.var[:, 0, 0]
+    return t
+
+
 class BatchForcedPhotometry(object):
     """Forced photometry of n sources on one image (device tensors data,
     inverr of shape (H, W), float32).  psf_basis: (G, ph, pw) stamps of the
@@ -199,7 +215,10 @@ class BatchForcedPhotometry(object):
     in tile order (tile x tile pixel cells; by default the tile holds about
     40 sources), Morton order inside a tile, so the preconditioner blocks (at
     most max_block consecutive sources of one tile) are contiguous and
-    spatially compact; results come back in the caller's order."""
+    spatially compact; results come back in the caller's order.  The whole
+    plan (order, windows, render jobs, sparsity pattern, blocks) is built on
+    the device; win, poff, order, row_ptr and tile_start are host copies made
+    on first use."""
 
     def __init__(self, data, inverr, psf_basis, psf_idx, psf_w, kind, pos, shape, tile=None,
                  max_block=64, device=0):
@@ -215,93 +234,108 @@ class BatchForcedPhotometry(object):
         G, self.ph, self.pw = self.basis.shape
         assert self.ph % 2 == 1 and self.pw % 2 == 1
         n = len(kind)
+        assert n > 0
         self.n = n
         if tile is None:
             # about 40 sources per tile (multiple of 16 pixels, 32..256)
-            tile = int(np.clip(round(math.sqrt(40. * self.H * self.W / max(n, 1)) / 16) * 16,
-                               32, 256))
-        ntx = (self.W + tile - 1) // tile
-        self.tile, self.ntx, self.nty = tile, ntx, (self.H + tile - 1) // tile
-        tid = (np.clip(pos[:, 1], 0, self.H - 1) // tile).astype(np.int64) * ntx + \
-            (np.clip(pos[:, 0], 0, self.W - 1) // tile).astype(np.int64)
-        # inside a tile, Morton order over 8x8 sub-cells, so that a run of
-        # consecutive sources (a preconditioner block) is spatially compact
-        cx = (np.clip(pos[:, 0], 0, self.W - 1) % tile * 8 // tile).astype(np.int64)
-        cy = (np.clip(pos[:, 1], 0, self.H - 1) % tile * 8 // tile).astype(np.int64)
-        mort = np.zeros(n, np.int64)
-        for bit in range(3):
-            mort |= ((cx >> bit) & 1) << (2 * bit) | ((cy >> bit) & 1) << (2 * bit + 1)
-        self.order = np.argsort(tid * 64 + mort, kind='stable')
-        o = self.order
-        self.stile = tid[o].astype(np.int32)
-        self.kind = np.asarray(kind)[o]
-        self.pos = np.asarray(pos, np.float64)[o]
-        self.shape = np.asarray(shape, np.float64)[o]
-        self.psf_idx = np.ascontiguousarray(np.asarray(psf_idx)[o], np.int32)
-        self.psf_w = np.ascontiguousarray(np.asarray(psf_w)[o], np.float32)
-        ts = np.bincount(tid[o], minlength=int(tid.max()) + 1)
-        ts = ts[ts > 0]
-        # preconditioner blocks: each tile's sources, split into near-equal
-        # runs of at most max_block (each block is factored in LDS)
-        parts = -(-ts // max_block)
-        ts = np.concatenate([np.full(k, c // k) + (np.arange(k) < c % k) for c, k in zip(ts, parts)])
-        self.tile_start = np.zeros(len(ts) + 1, np.int32)
-        self.tile_start[1:] = np.cumsum(ts)
+            tile = int(np.clip(round(math.sqrt(40. * self.H * self.W / n) / 16) * 16, 32, 256))
+        self.tile = tile
+        self.ntx, self.nty = (self.W + tile - 1) // tile, (self.H + tile - 1) // tile
+        self.max_block = max_block
+        src = np.empty((n, 6))
+        src[:, 0:2] = pos
+        src[:, 2] = kind
+        src[:, 3:6] = shape
+        self._src = src
+        self._pidx = np.ascontiguousarray(psf_idx, np.int32)
+        self._pw = np.ascontiguousarray(psf_w, np.float32)
         self.stream = torch.cuda.current_stream(self.dev).cuda_stream
         self.timing = {}
+        self._host = {}
 
     # ------------------------------------------------------------------
     def plan(self):
+        """Sort keys -> tile order (stable sort) -> per-source windows, render
+        buckets and jobs (trh_fp_plan) -> bucketed jobs, patch offsets,
+        preconditioner blocks and the sparsity pattern.  Two small reads
+        come back to the host: the sizes to allocate, then nnz."""
         t0 = time.perf_counter()
-        P = plan_patches(self.kind, self.pos, self.shape, (self.H, self.W), (self.ph, self.pw))
-        self.win = P['win']
-        self.sz = P['sz']
-        self.jobs = P['job']
-        area = self.win[:, 2] * self.win[:, 3]
-        self.poff = np.zeros(self.n, np.int64)
-        self.poff[1:] = np.cumsum(area)[:-1]
-        self.npix = int(area.sum())
-        self.jobs['out_offset'] = self.poff
-        self.jobs['out_stride'] = np.maximum(self.win[:, 2], 1)
+        lib, h, st, dev, n = self.lib, self.h.value, self.stream, self.dev, self.n
+        i32, i64, f64 = torch.int32, torch.int64, torch.float64
+        d_src = self._t(self._src)
+        key = torch.empty(n, dtype=i32, device=dev)
+        check(lib.trh_fp_keys(h, n, ptr(d_src), self.H, self.W, self.tile, ptr(key), st),
+              'fp_keys')
+        order = torch.sort(key, stable=True).indices
+        win = torch.empty((n, 4), dtype=i32, device=dev)
+        bucket = torch.empty(n, dtype=i32, device=dev)
+        stile = torch.empty(n, dtype=i32, device=dev)
+        area = torch.empty(n, dtype=i64, device=dev)
+        ext = torch.empty(n, dtype=f64, device=dev)
+        jobs = torch.empty((n, FFT_JOB.itemsize), dtype=torch.uint8, device=dev)
+        mix = self._t(mixture_table())
+        check(lib.trh_fp_plan(h, n, ptr(order), ptr(d_src), ptr(key), self.H, self.W, self.ph,
+                              self.pw, ptr(mix), ptr(win), ptr(bucket), ptr(stile), ptr(area),
+                              ptr(ext), ptr(jobs), st), 'fp_plan')
+        # cells, and the preconditioner blocks: each cell's run of sources
+        # split into near-equal blocks of at most max_block
+        ncell = self.ntx * self.nty
+        ccount = torch.bincount(stile, minlength=ncell)
+        nb = (ccount + self.max_block - 1) // self.max_block
+        qd = ccount // nb.clamp(min=1)
+        rd = ccount - qd * nb
+        bcount = torch.bincount(bucket, minlength=NBUCKET)
+        sizes = torch.cat([bcount.to(f64), area.sum().to(f64).view(1), ext.max().view(1),
+                           nb.sum().to(f64).view(1), (qd + (rd > 0).to(i64)).max().to(f64).view(1),
+                           (rd * (qd + 1) ** 2 + (nb - rd) * qd ** 2).sum().to(f64).view(1)])
+        sizes = sizes.cpu().numpy()
+        bc = sizes[:NBUCKET].astype(np.int64)
+        self.npix, self.emax = int(sizes[NBUCKET]), float(sizes[NBUCKET + 1])
+        self.ntile, self.max_tile, self.blocks_len = (int(v) for v in sizes[NBUCKET + 2:])
+        self.bcount = bc
+        self.bstart = np.concatenate([[0], np.cumsum(bc)[:-1]])
+        # jobs grouped by bucket, each with its patch offset
+        self.d_poff = torch.cumsum(area, 0) - area
+        border = torch.sort(bucket, stable=True).indices
+        d_bstart = torch.cumsum(bcount, 0) - bcount
+        jobs_b = torch.empty_like(jobs)
+        check(lib.trh_fp_jobs_bucketed(h, n, ptr(border), ptr(bucket), ptr(d_bstart),
+                                       ptr(self.d_poff), ptr(jobs), ptr(jobs_b), st),
+              'fp_jobs_bucketed')
+        cell_of = torch.repeat_interleave(torch.arange(ncell, device=dev), nb,
+                                          output_size=self.ntile)
+        j = torch.arange(self.ntile, device=dev) - (torch.cumsum(nb, 0) - nb)[cell_of]
+        bsize = qd[cell_of] + (j < rd[cell_of]).to(i64)
+        self.d_ts = torch.cat([torch.zeros(1, dtype=i64, device=dev),
+                               torch.cumsum(bsize, 0)]).to(i32)
+        self.d_bo = torch.cumsum(bsize * bsize, 0) - bsize * bsize
+        cfirst = torch.cumsum(ccount, 0) - ccount
+        self.d_tfirst = cfirst.to(i32)
+        self.d_tend = (cfirst + ccount).to(i32)
+        self.d_order, self.d_win, self.d_stile, self.d_bucket = order, win, stile, bucket
+        self.d_border, self.d_jobs = border, jobs_b
+        self.d_pidx = self._t(self._pidx)[order]
+        self.d_pw = self._t(self._pw)[order]
+        self._host = {}
         self.pattern()
-        sizes = np.diff(self.tile_start).astype(np.int64)
-        self.block_off = np.zeros(len(sizes), np.int64)
-        self.block_off[1:] = np.cumsum(sizes ** 2)[:-1]
-        self.max_tile = int(sizes.max())
         self.timing['plan_s'] = time.perf_counter() - t0
         return self
 
     def pattern(self):
         """The sparsity pattern of A on the device (trh_fp_overlaps): CSR
-        row_ptr (kept on the host too), sorted columns and the diagonal's
-        position in each row.  Two windows can only share a pixel when the
-        cells of the sources' (clipped) positions are within `reach` cells,
-        reach from the farthest window edge from its source position."""
-        lib, h, st = self.lib, self.h.value, self.stream
-        n, win = self.n, self.win
-        px = np.clip(self.pos[:, 0], 0, self.W - 1)
-        py = np.clip(self.pos[:, 1], 0, self.H - 1)
-        ne = (win[:, 2] > 0) & (win[:, 3] > 0)
-        ext = np.maximum.reduce([px - win[:, 0], win[:, 0] + win[:, 2] - px,
-                                 py - win[:, 1], win[:, 1] + win[:, 3] - py])
-        emax = float(ext[ne].max()) if ne.any() else 0.
-        self.reach = int(math.floor(2. * emax / self.tile)) + 1
-        ncell = self.ntx * self.nty
-        cnt = np.bincount(self.stile, minlength=ncell)
-        first = np.zeros(ncell, np.int32)
-        first[1:] = np.cumsum(cnt)[:-1]
-        self.d_win = self._t(win.astype(np.int32))
-        self.d_stile = self._t(self.stile)
-        self.d_tfirst = self._t(first)
-        self.d_tend = self._t((first + cnt).astype(np.int32))
+        row_ptr, sorted columns and the diagonal's position in each row.  Two
+        windows can only share a pixel when the cells of the sources'
+        (clipped) positions are within `reach` cells, reach from the farthest
+        window edge from its source position."""
+        lib, h, st, n = self.lib, self.h.value, self.stream, self.n
+        self.reach = int(math.floor(2. * self.emax / self.tile)) + 1
         counts = torch.zeros(n, dtype=torch.int32, device=self.dev)
         args = (h, n, ptr(self.d_win), ptr(self.d_stile), self.ntx, self.nty, ptr(self.d_tfirst),
                 ptr(self.d_tend), self.reach)
         check(lib.trh_fp_overlaps(*args, ptr(counts), None, None, None, st), 'fp_overlaps')
-        self.row_ptr = np.zeros(n + 1, np.int64)
-        self.row_ptr[1:] = np.cumsum(counts.cpu().numpy())
-        self.nnz = int(self.row_ptr[-1])
-        self.d_row_ptr = self._t(self.row_ptr)
+        self.d_row_ptr = torch.zeros(n + 1, dtype=torch.int64, device=self.dev)
+        torch.cumsum(counts, 0, out=self.d_row_ptr[1:])
+        self.nnz = int(self.d_row_ptr[-1])
         self.d_col = torch.empty(self.nnz, dtype=torch.int32, device=self.dev)
         self.d_diag = torch.empty(n, dtype=torch.int64, device=self.dev)
         check(lib.trh_fp_overlaps(*args, None, ptr(self.d_row_ptr), ptr(self.d_col),
@@ -313,71 +347,74 @@ class BatchForcedPhotometry(object):
         t = torch.from_numpy(a).to(self.dev)
         return t if dt is None else t.to(dt)
 
+    def _hosted(self, name, fn):
+        if name not in self._host:
+            self._host[name] = fn()
+        return self._host[name]
+
+    # host copies of the device plan (tile order), for tests and byte counts
+    win = property(lambda self: self._hosted('win', lambda: self.d_win.cpu().numpy()))
+    poff = property(lambda self: self._hosted('poff', lambda: self.d_poff.cpu().numpy()))
+    order = property(lambda self: self._hosted('order', lambda: self.d_order.cpu().numpy()))
+    row_ptr = property(lambda self: self._hosted('row_ptr', lambda: self.d_row_ptr.cpu().numpy()))
+    tile_start = property(lambda self: self._hosted('ts', lambda: self.d_ts.cpu().numpy()))
+    bucket = property(lambda self: self._hosted('bucket', lambda: self.d_bucket.cpu().numpy()))
+
     def render(self):
-        """Per-source PSFs and unit-flux patches on the device."""
+        """Per-source PSFs and unit-flux patches on the device, one batched
+        render per bucket of the bucketed jobs."""
         lib, h, st = self.lib, self.h.value, self.stream
+        if self.bcount[B_BAD]:
+            raise ValueError('%d galaxies need an FFT size outside 32..512' % self.bcount[B_BAD])
         self.patches = torch.zeros(max(self.npix, 1), dtype=torch.float32, device=self.dev)
         basis = self._t(self.basis.reshape(len(self.basis), -1))
-        d_idx = self._t(self.psf_idx)
-        d_w = self._t(self.psf_w)
-        # point sources: interpolated stamps, then one batched Lanczos render
-        pt = np.flatnonzero((self.kind == KIND_POINT) & (self.win[:, 2] > 0) & (self.win[:, 3] > 0))
-        if len(pt):
-            stamps = torch.empty((len(pt), self.ph * self.pw), dtype=torch.float32, device=self.dev)
-            ip = self._t(pt.astype(np.int64))
-            for c0 in range(0, len(pt), 65535):
-                c1 = min(c0 + 65535, len(pt))
-                # keep the gathered arguments alive until the launch is queued
-                wq = d_w[ip[c0:c1]].contiguous()
-                iq = d_idx[ip[c0:c1]].contiguous()
-                check(lib.trh_psf_combine(h, ptr(basis), len(self.basis), self.ph * self.pw,
-                                          ptr(wq), ptr(iq),
-                                          ptr(stamps) + 4 * c0 * self.ph * self.pw, c1 - c0, st),
-                      'psf_combine')
-            J = self.jobs[pt].copy()
-            J['psf'] = np.arange(len(pt))
-            dj = R.jobs_to_device(J, self.dev)
-            check(lib.trh_render_point_batch(h, ptr(dj), len(pt), int(J['mw'].max()), ptr(stamps),
-                                             self.ph, self.pw, ptr(self.patches), None, None, None,
-                                             st), 'render_point_batch')
-            self._keep = (stamps, dj)
-        # galaxies: per FFT size, the grid's spectra, interpolated per source
-        gal = (self.kind != KIND_POINT) & (self.win[:, 2] > 0) & (self.win[:, 3] > 0)
+        rec = FFT_JOB.itemsize
         keep = []
-        for sz in np.unique(self.sz[gal]):
-            sz = int(sz)
-            gi = np.flatnonzero(gal & (self.sz == sz))
+
+        def combine(src, length, b0, c, out):
+            qi = self.d_border[b0:b0 + c]
+            wq = self.d_pw[qi]
+            iq = self.d_pidx[qi]
+            keep.append((wq, iq))
+            for c0 in range(0, c, 65535):
+                c1 = min(c0 + 65535, c)
+                check(lib.trh_psf_combine(h, ptr(src), len(self.basis), length,
+                                          ptr(wq) + 16 * c0, ptr(iq) + 16 * c0,
+                                          ptr(out) + 4 * c0 * length, c1 - c0, st), 'psf_combine')
+        # point sources: interpolated stamps, then one batched Lanczos render
+        b0, c = int(self.bstart[B_POINT]), int(self.bcount[B_POINT])
+        if c:
+            stamps = torch.empty((c, self.ph * self.pw), dtype=torch.float32, device=self.dev)
+            combine(basis, self.ph * self.pw, b0, c, stamps)
+            check(lib.trh_render_point_batch(h, ptr(self.d_jobs) + rec * b0, c, self.pw,
+                                             ptr(stamps), self.ph, self.pw, ptr(self.patches),
+                                             None, None, None, st), 'render_point_batch')
+            keep.append(stamps)
+        # galaxies: per FFT size, the grid's spectra, interpolated per source
+        import ctypes
+        for bi, sz in enumerate(FFT_SIZES, start=1):
+            b0, c = int(self.bstart[bi]), int(self.bcount[bi])
+            if not c:
+                continue
             nv = sz // 2 + 1
             spec_basis = torch.empty((len(self.basis), nv * sz * 2), dtype=torch.float32,
                                      device=self.dev)
-            import ctypes
             cx, cy = ctypes.c_int(), ctypes.c_int()
-            dst = self._t(self.basis)
-            check(lib.trh_psf_spectrum_batch(h, ptr(dst), len(self.basis), self.ph, self.pw, sz,
+            check(lib.trh_psf_spectrum_batch(h, ptr(basis), len(self.basis), self.ph, self.pw, sz,
                                              ptr(spec_basis), ctypes.byref(cx), ctypes.byref(cy),
                                              st), 'psf_spectrum_batch')
-            spec = torch.empty((len(gi), nv * sz * 2), dtype=torch.float32, device=self.dev)
-            ig = self._t(gi.astype(np.int64))
-            for c0 in range(0, len(gi), 65535):
-                c1 = min(c0 + 65535, len(gi))
-                wq = d_w[ig[c0:c1]].contiguous()
-                iq = d_idx[ig[c0:c1]].contiguous()
-                check(lib.trh_psf_combine(h, ptr(spec_basis), len(self.basis), nv * sz * 2,
-                                          ptr(wq), ptr(iq), ptr(spec) + 4 * c0 * nv * sz * 2,
-                                          c1 - c0, st), 'psf_combine')
-            J = self.jobs[gi].copy()
-            J['psf'] = np.arange(len(gi))
-            dj = R.jobs_to_device(J, self.dev)
-            check(lib.trh_render_fft_batch(h, sz, ptr(dj), len(gi), ptr(spec), ptr(self.patches),
-                                           None, None, None, st), 'render_fft_batch')
-            keep.append((spec_basis, spec, dj, dst))
-        self._keep_g = keep
+            spec = torch.empty((c, nv * sz * 2), dtype=torch.float32, device=self.dev)
+            combine(spec_basis, nv * sz * 2, b0, c, spec)
+            check(lib.trh_render_fft_batch(h, sz, ptr(self.d_jobs) + rec * b0, c, ptr(spec),
+                                           ptr(self.patches), None, None, None, st),
+                  'render_fft_batch')
+            keep.append((spec_basis, spec))
+        self._keep = (basis, keep)
         return self
 
     def assemble(self, mod0=None):
         """A (CSR values) and b on the device."""
         lib, h, st = self.lib, self.h.value, self.stream
-        self.d_poff = self._t(self.poff)
         self.vals = torch.zeros(self.nnz, dtype=torch.float64, device=self.dev)
         self.b = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
         check(lib.trh_fp_pair_dots_csr(h, self.n, ptr(self.d_row_ptr), ptr(self.d_col),
@@ -390,22 +427,18 @@ class BatchForcedPhotometry(object):
         return self
 
     def solve(self, tol=1e-12, maxiter=1000, chunk=8):
-        """FP64 conjugate gradients preconditioned by the tile blocks' exact
-        Cholesky factors (trh_fp_pcg: every scalar stays on the device).
-        Iterations are enqueued `chunk` at a time; the stop flag of one chunk
-        is read while the next one runs.  Returns fluxes in the caller's
-        order."""
+        """FP64 conjugate gradients preconditioned by the exact inverses of
+        the diagonal blocks (trh_fp_block_inv, trh_fp_pcg: every scalar stays
+        on the device).  Iterations are enqueued `chunk` at a time; the stop
+        flag of one chunk is read while the next one runs.  Returns fluxes in
+        the caller's order."""
         lib, h, st = self.lib, self.h.value, self.stream
-        d_rp = self.d_row_ptr
-        d_col = self.d_col
-        d_ts = self._t(self.tile_start)
-        d_bo = self._t(self.block_off)
-        nt = len(self.tile_start) - 1
-        blocks = torch.empty(int(np.sum(np.diff(self.tile_start).astype(np.int64) ** 2)),
-                             dtype=torch.float64, device=self.dev)
+        d_rp, d_col, d_ts, d_bo = self.d_row_ptr, self.d_col, self.d_ts, self.d_bo
+        nt = self.ntile
+        blocks = torch.empty(max(self.blocks_len, 1), dtype=torch.float64, device=self.dev)
         info = torch.zeros(nt, dtype=torch.int32, device=self.dev)
         check(lib.trh_fp_block_inv(h, ptr(d_rp), ptr(d_col), ptr(self.vals), ptr(d_ts), ptr(d_bo),
-                                    nt, self.max_tile, ptr(blocks), ptr(info), st), 'fp_block_inv')
+                                   nt, self.max_tile, ptr(blocks), ptr(info), st), 'fp_block_inv')
         n = self.n
         x = torch.empty(n, dtype=torch.float64, device=self.dev)
         work = torch.empty(4 * n + 5 * nt + 2, dtype=torch.float64, device=self.dev)
@@ -436,22 +469,18 @@ class BatchForcedPhotometry(object):
         rr, bb = work[-2:].cpu().tolist()
         self.resid = float(np.sqrt(rr / bb)) if bb > 0 else 0.0
         self.info = info.cpu().numpy()
-        flux = np.empty(n)
-        flux[self.order] = x.cpu().numpy()
-        diag = self.vals[self.d_diag].cpu().numpy()
-        iv = np.empty(n)
-        iv[self.order] = diag
-        self.flux, self.iv = flux, iv
-        return flux
+        out = torch.empty((2, n), dtype=torch.float64, device=self.dev)
+        out[0, self.d_order] = x
+        out[1, self.d_order] = self.vals[self.d_diag]
+        self.flux, self.iv = out.cpu().numpy()
+        return self.flux
 
     def model_image(self, flux, out=None):
         """sum_s flux_s u_s on the device (flux in the caller's order)."""
         if out is None:
             out = torch.zeros((self.H, self.W), dtype=torch.float32, device=self.dev)
-        f = self._t(np.asarray(flux, np.float32)[self.order])
-        dwin = self._t(self.win.astype(np.int32))
-        dpoff = self._t(self.poff)
-        check(self.lib.trh_fp_add_patches(self.h.value, self.n, ptr(dwin), ptr(dpoff),
+        f = self._t(np.asarray(flux, np.float32))[self.d_order].contiguous()
+        check(self.lib.trh_fp_add_patches(self.h.value, self.n, ptr(self.d_win), ptr(self.d_poff),
                                           ptr(self.patches), ptr(f), self.W, ptr(out), self.stream),
               'fp_add_patches')
         return out
@@ -485,9 +514,11 @@ class BatchForcedPhotometry(object):
         overlap, u/data/inverr over every patch) and one PCG iteration (CSR
         values and columns, gathered x)."""
         area = self.win[:, 2] * self.win[:, 3]
-        gal = (self.kind != KIND_POINT) & (area > 0)
-        spec = np.sum(8. * (self.sz[gal] // 2 + 1) * self.sz[gal])
-        render = 4. * area.sum() + spec + 4. * self.ph * self.pw * np.sum((~gal) & (area > 0))
+        bk = self.bucket
+        gal = (bk >= 1) & (bk <= len(FFT_SIZES))
+        sz = 32 << np.where(gal, bk - 1, 0)
+        spec = np.sum(8. * (sz[gal] // 2 + 1) * sz[gal])
+        render = 4. * area.sum() + spec + 4. * self.ph * self.pw * np.sum(bk == B_POINT)
         col = self.d_col.cpu().numpy().astype(np.int64)
         rows = np.repeat(np.arange(self.n), np.diff(self.row_ptr))
         up = col >= rows
