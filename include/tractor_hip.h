@@ -263,14 +263,19 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  *                  = the sources of cell c; two windows can only meet when
  *                  their cells are within `reach` in x and in y.  With
  *                  counts: counts[s] = row length (diagonal included); else
- *                  the columns of row s, sorted, at col[row_ptr[s]..], and
- *                  diag[s] = the position of (s, s).
- *   fp_pair_dots_csr: A_st for every entry of that pattern (t >= s computed,
- *                  mirrored to (t, s)).
+ *                  the columns of row s, sorted, at col[row_ptr[s]..],
+ *                  diag[s] = the position of (s, s) and, if erow is given,
+ *                  erow[k] = the row of entry k.
+ *   fp_pair_dots_csr: A_st = sum w_s u_t for every entry of that pattern
+ *                  (t >= s computed, mirrored to (t, s)), w_s = u_s inverr^2
+ *                  the weighted patches written by fp_rhs (FP32 products,
+ *                  FP64 sums).
  *   fp_pair_dots:  A_st = sum u_s u_t inverr^2 over each overlapping pair
  *                  (s, t) = pairs[p] (FP32 products, FP64 sum), written to
  *                  the CSR values at pos[p][0] and pos[p][1].
- *   fp_rhs:        b_s = sum u_s (data - mod0) inverr^2 (mod0 may be NULL).
+ *   fp_rhs:        b_s = sum u_s (data - mod0) inverr^2 (mod0 may be NULL) and,
+ *                  if wpatch is given, the weighted patches w_s = u_s inverr^2
+ *                  (same layout as the patches).
  *   fp_block_inv:  the inverse of each of A's diagonal blocks (FP64,
  *                  Gauss-Jordan in LDS), one block per tile of consecutive
  *                  rows [tile_start[T], tile_start[T+1]) of at most max_n rows
@@ -310,16 +315,16 @@ int trh_fp_jobs_bucketed(trh_handle *h, int n, const int64_t *border, const int3
 int trh_fp_overlaps(trh_handle *h, int n, const int32_t *win, const int32_t *stile, int ntx,
                     int nty, const int32_t *tile_first, const int32_t *tile_end, int reach,
                     int32_t *counts, const int64_t *row_ptr, int32_t *col, int64_t *diag,
-                    void *stream);
+                    int32_t *erow, void *stream);
 int trh_fp_pair_dots_csr(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
-                         const int32_t *win, const int64_t *poff, const float *patches,
-                         const float *inverr, int img_w, double *vals, void *stream);
+                         const int32_t *erow, const int32_t *win, const int64_t *poff,
+                         const float *patches, const float *wpatch, double *vals, void *stream);
 int trh_fp_pair_dots(trh_handle *h, const int32_t *pairs, const int64_t *pos, int npairs,
                      const int32_t *win, const int64_t *poff, const float *patches,
                      const float *inverr, int img_w, double *vals, void *stream);
 int trh_fp_rhs(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
                const float *patches, const float *data, const float *mod0, const float *inverr,
-               int img_w, double *b, void *stream);
+               int img_w, double *b, float *wpatch, void *stream);
 int trh_fp_block_inv(trh_handle *h, const int64_t *row_ptr, const int32_t *col,
                      const double *vals, const int32_t *tile_start, const int64_t *block_off,
                      int ntile, int max_n, double *blocks, int32_t *info, void *stream);

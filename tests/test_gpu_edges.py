@@ -30,7 +30,7 @@ def test_empty_batches_are_no_ops(dev):
     out = torch.full((16,), 7.0, device=dev)
     assert lib.trh_render_fft_batch(h.value, 64, None, 0, _lib.ptr(out), _lib.ptr(out), None, None,
                                     None, None) == 0
-    assert lib.trh_fp_rhs(h.value, 0, None, None, None, None, None, None, 4, None, None) == 0
+    assert lib.trh_fp_rhs(h.value, 0, None, None, None, None, None, None, 4, None, None, None) == 0
     assert lib.trh_fp_pair_dots(h.value, None, None, 0, None, None, None, None, 4, None, None) == 0
     torch.cuda.synchronize()
     assert torch.all(out == 7.0)

@@ -68,10 +68,10 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_fp_keys': (N, 1, N, 10, 10, 4, N, N),
         'trh_fp_plan': (N, 1, N, N, N, 10, 10, 3, 3, N, N, N, N, N, N, N, N),
         'trh_fp_jobs_bucketed': (N, 1, N, N, N, N, N, N, N),
-        'trh_fp_overlaps': (N, 1, N, N, 1, 1, N, N, 0, N, N, N, N, N),
-        'trh_fp_pair_dots_csr': (N, 1, N, N, N, N, N, N, 10, N, N),
+        'trh_fp_overlaps': (N, 1, N, N, 1, 1, N, N, 0, N, N, N, N, N, N),
+        'trh_fp_pair_dots_csr': (N, 1, N, N, N, N, N, N, N, N, N),
         'trh_fp_pair_dots': (N, N, N, 1, N, N, N, N, 10, N, N),
-        'trh_fp_rhs': (N, 1, N, N, N, N, N, N, 10, N, N),
+        'trh_fp_rhs': (N, 1, N, N, N, N, N, N, 10, N, N, N),
         'trh_fp_block_inv': (N, N, N, N, N, N, 1, 8, N, N, N),
         'trh_fp_precond': (N, N, N, 1, 8, N, N, N, N),
         'trh_fp_spmv': (N, 4, N, N, N, N, N, N),

@@ -68,10 +68,10 @@ SIGNATURES = {
     'trh_fp_keys': (_I, [_P, _I, _P, _I, _I, _I, _P, _P]),
     'trh_fp_plan': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_fp_jobs_bucketed': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
-    'trh_fp_overlaps': (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P]),
-    'trh_fp_pair_dots_csr': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    'trh_fp_overlaps': (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P]),
+    'trh_fp_pair_dots_csr': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_fp_pair_dots': (_I, [_P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _P]),
-    'trh_fp_rhs': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    'trh_fp_rhs': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
     'trh_fp_block_inv': (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
     'trh_fp_precond': (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P]),
     'trh_fp_spmv': (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),

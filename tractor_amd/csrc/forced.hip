@@ -79,30 +79,69 @@ k_fp_pair_dots(const int32_t *__restrict__ pairs, const int64_t *__restrict__ po
     }
 }
 
-// One wavefront per source: b_s = sum u_s (data - mod0) inverr^2 over its
-// window (mod0 may be NULL: zero).
+// (y, x) of element i of a row-major nx-wide block, stepped by `stride`
+// elements at a time without a division per step.
+struct FlatWalk {
+    int y, x, dy, dx, nx;
+    __device__ FlatWalk(int i, int nx_, int stride) : nx(nx_) {
+        y = i / nx;
+        x = i - y * nx;
+        dy = stride / nx;
+        dx = stride - dy * nx;
+    }
+    __device__ void step() {
+        x += dx;
+        y += dy;
+        if (x >= nx) {
+            x -= nx;
+            ++y;
+        }
+    }
+};
+
+// One 256-thread workgroup per source, the window flattened over the
+// threads, four elements per thread in flight:
+// b_s = sum u_s (data - mod0) inverr^2 (mod0 may be NULL: zero; FP32
+// products, FP64 sum) and, when wpatch is given, the weighted patch
+// w_s = u_s inverr^2 at the patch's offset.
 __global__ void __launch_bounds__(256)
 k_fp_rhs(int n, const int32_t *__restrict__ win, const int64_t *__restrict__ poff,
          const float *__restrict__ patches, const float *__restrict__ data,
          const float *__restrict__ mod0, const float *__restrict__ inverr, int img_w,
-         double *__restrict__ b) {
-    const int lane = threadIdx.x & 63;
-    const int nwave = gridDim.x * (blockDim.x >> 6);
-    for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += nwave) {
-        const int x0 = win[4 * s], y0 = win[4 * s + 1], w = win[4 * s + 2], h = win[4 * s + 3];
+         double *__restrict__ b, float *__restrict__ wpatch) {
+    __shared__ double red[4];
+    const int s = blockIdx.x;
+    const int x0 = win[4 * s], y0 = win[4 * s + 1], w = win[4 * s + 2], h = win[4 * s + 3];
+    double acc = 0.0;
+    if (w > 0 && h > 0) {
         const float *u = patches + poff[s];
-        float acc = 0.f;
-        for (int y = 0; y < h; ++y) {
-            const int64_t ib = (int64_t)(y0 + y) * img_w + x0;
-            for (int x = lane; x < w; x += 64) {
-                const float e = inverr[ib + x];
-                const float d = data[ib + x] - (mod0 ? mod0[ib + x] : 0.f);
-                acc += u[(int64_t)y * w + x] * d * (e * e);
+        float *wo = wpatch ? wpatch + poff[s] : nullptr;
+        const int m = w * h;
+        FlatWalk fw(threadIdx.x, w, 256);
+        for (int i0 = threadIdx.x; i0 < m; i0 += 1024) {
+            float uu[4], ee[4], dd[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + 256 * k;
+                const bool ok = i < m;
+                const int64_t ib = (int64_t)(y0 + fw.y) * img_w + x0 + fw.x;
+                uu[k] = ok ? u[i] : 0.f;
+                ee[k] = ok ? inverr[ib] : 0.f;
+                dd[k] = ok ? data[ib] - (mod0 ? mod0[ib] : 0.f) : 0.f;
+                fw.step();
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float wu = uu[k] * (ee[k] * ee[k]);
+                if (wo && i0 + 256 * k < m) wo[i0 + 256 * k] = wu;
+                acc += (double)(wu * dd[k]);
             }
         }
-        const double v = wave_sum_d((double)acc);
-        if (lane == 0) b[s] = v;
     }
+    acc = wave_sum_d(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) b[s] = red[0] + red[1] + red[2] + red[3];
 }
 
 // One workgroup per tile of rows [t0, t1) (n = t1 - t0 rows, the block in
@@ -460,7 +499,7 @@ __global__ void __launch_bounds__(256)
 k_fp_overlaps(int n, const int32_t *__restrict__ win, const int32_t *__restrict__ stile, int ntx,
               int nty, const int32_t *__restrict__ tile_first, const int32_t *__restrict__ tile_end,
               int reach, int32_t *__restrict__ counts, const int64_t *__restrict__ row_ptr,
-              int32_t *__restrict__ col, int64_t *__restrict__ diag) {
+              int32_t *__restrict__ col, int64_t *__restrict__ diag, int32_t *__restrict__ erow) {
     const int lane = threadIdx.x & 63;
     const int nwave = gridDim.x * (blockDim.x >> 6);
     for (int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); s < n; s += nwave) {
@@ -479,6 +518,7 @@ k_fp_overlaps(int n, const int32_t *__restrict__ win, const int32_t *__restrict_
                     if (!counts && hit) {
                         const int64_t k = base + found + __popcll(m & ((1ull << lane) - 1ull));
                         col[k] = t;
+                        if (erow) erow[k] = s;
                         if (t == s) diag[s] = k;
                     }
                     found += __popcll(m);
@@ -488,62 +528,63 @@ k_fp_overlaps(int n, const int32_t *__restrict__ win, const int32_t *__restrict_
     }
 }
 
-// One wavefront per entry (s, t) of A with t >= s: the sum of u_s u_t
-// inverr^2 over the intersection of the two windows, also written at the
-// mirror entry (t, s), found by binary search in row t.  The row of entry k
-// is found by binary search in row_ptr.
+// One wavefront per entry (s, t) of A with t >= s (its row from erow): the
+// sum of w_s u_t over the intersection of the two windows, w_s = u_s
+// inverr^2 the weighted patch (FP32 products, FP64 sums).  The intersection
+// is flattened over the lanes, four elements per lane in flight.  The value
+// is also written at the mirror entry (t, s), found by bisection of row t.
 __global__ void __launch_bounds__(256)
 k_fp_pair_dots_csr(int n, const int64_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
-                   const int32_t *__restrict__ win, const int64_t *__restrict__ poff,
-                   const float *__restrict__ patches, const float *__restrict__ inverr, int img_w,
-                   double *__restrict__ vals) {
+                   const int32_t *__restrict__ erow, const int32_t *__restrict__ win,
+                   const int64_t *__restrict__ poff, const float *__restrict__ patches,
+                   const float *__restrict__ wpatch, double *__restrict__ vals) {
     const int lane = threadIdx.x & 63;
     const int64_t nnz = row_ptr[n];
     const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nnz;
          k += nwave) {
-        // s = the row holding entry k: the last r with row_ptr[r] <= k
-        int lo = 0, hi = n - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (row_ptr[mid] <= k) lo = mid;
-            else hi = mid - 1;
-        }
-        const int s = lo, t = col[k];
+        const int s = erow[k], t = col[k];
         if (t < s) continue;
-        const int sx0 = win[4 * s], sy0 = win[4 * s + 1], sw = win[4 * s + 2];
+        const int sx0 = win[4 * s], sy0 = win[4 * s + 1], sw = win[4 * s + 2], sh = win[4 * s + 3];
         const int tx0 = win[4 * t], ty0 = win[4 * t + 1], tw = win[4 * t + 2], th = win[4 * t + 3];
-        const int sh = win[4 * s + 3];
         const int X0 = max(sx0, tx0), X1 = min(sx0 + sw, tx0 + tw);
         const int Y0 = max(sy0, ty0), Y1 = min(sy0 + sh, ty0 + th);
-        const float *us = patches + poff[s];
-        const float *ut = patches + poff[t];
-        float acc = 0.f;
-        const int nx = X1 - X0;
-        for (int y = Y0; y < Y1; ++y) {
-            const float *rs = us + (int64_t)(y - sy0) * sw + (X0 - sx0);
-            const float *rt = ut + (int64_t)(y - ty0) * tw + (X0 - tx0);
-            const float *re = inverr + (int64_t)y * img_w + X0;
-            for (int x = lane; x < nx; x += 64) {
-                const float e = re[x];
-                acc += rs[x] * rt[x] * (e * e);
+        const int nx = X1 - X0, ny = Y1 - Y0;
+        double acc = 0.0;
+        if (nx > 0 && ny > 0) {
+            const float *ws = wpatch + poff[s] + (int64_t)(Y0 - sy0) * sw + (X0 - sx0);
+            const float *ut = patches + poff[t] + (int64_t)(Y0 - ty0) * tw + (X0 - tx0);
+            const int m = nx * ny;
+            FlatWalk fw(lane, nx, 64);
+            for (int i = lane; i < m; i += 256) {
+                float a[4], c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool ok = i + 64 * u < m;
+                    a[u] = ok ? ws[fw.y * sw + fw.x] : 0.f;
+                    c[u] = ok ? ut[fw.y * tw + fw.x] : 0.f;
+                    fw.step();
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc += (double)(a[u] * c[u]);
             }
         }
-        const double v = wave_sum_d((double)acc);
+        const double v = wave_sum_d(acc);
         if (lane == 0) {
             vals[k] = v;
             if (t != s) {
-                int64_t a = row_ptr[t], b = row_ptr[t + 1] - 1;
-                while (a < b) {
-                    const int64_t mid = (a + b) >> 1;
-                    if (col[mid] < s) a = mid + 1;
-                    else b = mid;
+                int64_t lo = row_ptr[t], hi = row_ptr[t + 1] - 1;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (col[mid] < s) lo = mid + 1;
+                    else hi = mid;
                 }
-                vals[a] = v;
+                vals[lo] = v;
             }
         }
     }
 }
+
 // ---------------------------------------------------------------------------
 // Planning on the device: sort keys, the unit-flux patch of every source and
 // the render jobs grouped by render kind.
@@ -736,25 +777,25 @@ extern "C" int trh_fp_overlaps(trh_handle *h, int n, const int32_t *win, const i
                                int ntx, int nty, const int32_t *tile_first,
                                const int32_t *tile_end, int reach, int32_t *counts,
                                const int64_t *row_ptr, int32_t *col, int64_t *diag,
-                               void *stream) {
+                               int32_t *erow, void *stream) {
     if (!h || n <= 0 || !win || !stile || ntx <= 0 || nty <= 0 || !tile_first || !tile_end ||
         reach < 0 || (!counts && (!row_ptr || !col || !diag)))
         return set_err(TRH_E_ARG, "trh_fp_overlaps: bad arguments");
     hipLaunchKernelGGL(k_fp_overlaps, dim3(grid_waves(h, n)), dim3(256), 0, (hipStream_t)stream, n,
                        win, stile, ntx, nty, tile_first, tile_end, reach, counts, row_ptr, col,
-                       diag);
+                       diag, counts ? nullptr : erow);
     return hip_check(hipGetLastError(), "k_fp_overlaps");
 }
 
 extern "C" int trh_fp_pair_dots_csr(trh_handle *h, int n, const int64_t *row_ptr,
-                                    const int32_t *col, const int32_t *win, const int64_t *poff,
-                                    const float *patches, const float *inverr, int img_w,
-                                    double *vals, void *stream) {
-    if (!h || n <= 0 || !row_ptr || !col || !win || !poff || !patches || !inverr || img_w <= 0 ||
+                                    const int32_t *col, const int32_t *erow, const int32_t *win,
+                                    const int64_t *poff, const float *patches,
+                                    const float *wpatch, double *vals, void *stream) {
+    if (!h || n <= 0 || !row_ptr || !col || !erow || !win || !poff || !patches || !wpatch ||
         !vals)
         return set_err(TRH_E_ARG, "trh_fp_pair_dots_csr: bad arguments");
     hipLaunchKernelGGL(k_fp_pair_dots_csr, dim3(grid_waves(h, 64L * n)), dim3(256), 0,
-                       (hipStream_t)stream, n, row_ptr, col, win, poff, patches, inverr, img_w,
+                       (hipStream_t)stream, n, row_ptr, col, erow, win, poff, patches, wpatch,
                        vals);
     return hip_check(hipGetLastError(), "k_fp_pair_dots_csr");
 }
@@ -775,12 +816,13 @@ extern "C" int trh_fp_pair_dots(trh_handle *h, const int32_t *pairs, const int64
 
 extern "C" int trh_fp_rhs(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
                           const float *patches, const float *data, const float *mod0,
-                          const float *inverr, int img_w, double *b, void *stream) {
+                          const float *inverr, int img_w, double *b, float *wpatch,
+                          void *stream) {
     if (!h || (n > 0 && (!win || !poff || !patches || !data || !inverr || !b)) || img_w <= 0)
         return set_err(TRH_E_ARG, "trh_fp_rhs: bad arguments");
     if (n <= 0) return TRH_OK;
-    hipLaunchKernelGGL(k_fp_rhs, dim3(grid_waves(h, n)), dim3(256), 0, (hipStream_t)stream, n,
-                       win, poff, patches, data, mod0, inverr, img_w, b);
+    hipLaunchKernelGGL(k_fp_rhs, dim3(n), dim3(256), 0, (hipStream_t)stream, n, win, poff,
+                       patches, data, mod0, inverr, img_w, b, wpatch);
     return hip_check(hipGetLastError(), "k_fp_rhs");
 }
 

@@ -332,14 +332,15 @@ class BatchForcedPhotometry(object):
         counts = torch.zeros(n, dtype=torch.int32, device=self.dev)
         args = (h, n, ptr(self.d_win), ptr(self.d_stile), self.ntx, self.nty, ptr(self.d_tfirst),
                 ptr(self.d_tend), self.reach)
-        check(lib.trh_fp_overlaps(*args, ptr(counts), None, None, None, st), 'fp_overlaps')
+        check(lib.trh_fp_overlaps(*args, ptr(counts), None, None, None, None, st), 'fp_overlaps')
         self.d_row_ptr = torch.zeros(n + 1, dtype=torch.int64, device=self.dev)
         torch.cumsum(counts, 0, out=self.d_row_ptr[1:])
         self.nnz = int(self.d_row_ptr[-1])
         self.d_col = torch.empty(self.nnz, dtype=torch.int32, device=self.dev)
         self.d_diag = torch.empty(n, dtype=torch.int64, device=self.dev)
+        self.d_erow = torch.empty(self.nnz, dtype=torch.int32, device=self.dev)
         check(lib.trh_fp_overlaps(*args, None, ptr(self.d_row_ptr), ptr(self.d_col),
-                                  ptr(self.d_diag), st), 'fp_overlaps')
+                                  ptr(self.d_diag), ptr(self.d_erow), st), 'fp_overlaps')
         return self
 
     def _t(self, a, dt=None):
@@ -413,17 +414,19 @@ class BatchForcedPhotometry(object):
         return self
 
     def assemble(self, mod0=None):
-        """A (CSR values) and b on the device."""
+        """b and the weighted patches u inverr^2 (trh_fp_rhs), then A's CSR
+        values from them (trh_fp_pair_dots_csr), on the device."""
         lib, h, st = self.lib, self.h.value, self.stream
         self.vals = torch.zeros(self.nnz, dtype=torch.float64, device=self.dev)
         self.b = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
-        check(lib.trh_fp_pair_dots_csr(h, self.n, ptr(self.d_row_ptr), ptr(self.d_col),
-                                       ptr(self.d_win), ptr(self.d_poff), ptr(self.patches),
-                                       ptr(self.inverr), self.W, ptr(self.vals), st),
-              'fp_pair_dots_csr')
+        self.wpatches = torch.empty_like(self.patches)
         check(lib.trh_fp_rhs(h, self.n, ptr(self.d_win), ptr(self.d_poff), ptr(self.patches),
-                             ptr(self.data), ptr(mod0), ptr(self.inverr), self.W, ptr(self.b), st),
-              'fp_rhs')
+                             ptr(self.data), ptr(mod0), ptr(self.inverr), self.W, ptr(self.b),
+                             ptr(self.wpatches), st), 'fp_rhs')
+        check(lib.trh_fp_pair_dots_csr(h, self.n, ptr(self.d_row_ptr), ptr(self.d_col),
+                                       ptr(self.d_erow), ptr(self.d_win), ptr(self.d_poff),
+                                       ptr(self.patches), ptr(self.wpatches), ptr(self.vals), st),
+              'fp_pair_dots_csr')
         return self
 
     def solve(self, tol=1e-12, maxiter=1000, chunk=8):
@@ -510,9 +513,10 @@ class BatchForcedPhotometry(object):
 
     def algorithmic_bytes(self):
         """HBM bytes each stage must move: render (patch writes, per-source
-        spectrum or stamp reads), assemble (u_s, u_t and inverr over every
-        overlap, u/data/inverr over every patch) and one PCG iteration (CSR
-        values and columns, gathered x)."""
+        spectrum or stamp reads), assemble (u, data, inverr reads and the
+        weighted patch's write over every patch, then w_s and u_t over every
+        overlap, and the CSR values) and one PCG iteration (CSR values and
+        columns, gathered x)."""
         area = self.win[:, 2] * self.win[:, 3]
         bk = self.bucket
         gal = (bk >= 1) & (bk <= len(FFT_SIZES))
@@ -528,6 +532,6 @@ class BatchForcedPhotometry(object):
         y0 = np.maximum(self.win[s, 1], self.win[t, 1])
         y1 = np.minimum(self.win[s, 1] + self.win[s, 3], self.win[t, 1] + self.win[t, 3])
         ov = np.maximum(x1 - x0, 0) * np.maximum(y1 - y0, 0)
-        assemble = 12. * ov.sum() + 12. * area.sum() + 16. * self.nnz
+        assemble = 8. * ov.sum() + 16. * area.sum() + 16. * self.nnz
         pcg = 12. * self.nnz + 8. * 10 * self.n
         return dict(render=render, assemble=assemble, pcg_iteration=pcg)
