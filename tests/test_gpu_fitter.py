@@ -169,3 +169,28 @@ def test_graph_replay_matches_eager(problem):
     f.set_params(start)
     f.fit(max_iter=100, graph=True)
     assert np.array_equal(a, f.get_params())
+
+
+def test_per_launch_profile(problem):
+    """Per-launch profiling over several eager iterations: no HIP error
+    (the 63-pixel PSF makes every FFT at least 64, so the 32 bucket is never
+    launched and its slot must be skipped), the same parameters as without
+    profiling, and times and launch counts only on launched slots."""
+    say('per-launch profile')
+    f = problem['fitter']
+    f.set_params(problem['start'])
+    for _ in range(3):
+        f.iterate(0., 0.)
+    ref = f.get_params()
+    f.set_params(problem['start'])
+    f.profile(True)
+    try:
+        for _ in range(3):
+            f.iterate(0., 0.)
+        prof = f.profile_read()
+    finally:
+        f.profile(False)
+    assert np.array_equal(f.get_params(), ref)
+    assert prof[0, 1] == 0 and prof[8, 1] == 0          # the 32 buckets: never launched
+    assert prof[6, 1] == 3 and prof[7, 1] == 3          # normal equations, solve
+    assert np.all(prof[:, 0] >= 0) and prof[6, 0] > 0

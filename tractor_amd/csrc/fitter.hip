@@ -1065,7 +1065,11 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
             return rc;
         for (int k = 0; k < TRH_NSLOT; ++k) {
             const bool fft = (k < NFFT) || (k >= SL_LS && k < SL_LS + NFFT);
-            const bool launched = fft ? f->n_gal > 0 : ((k == SL_JPT || k == SL_LPT) ? f->n_pt > 0 : true);
+            // a slot whose events were not recorded this iteration is skipped
+            // (asking for their elapsed time would leave a sticky HIP error)
+            const bool launched =
+                fft ? f->n_gal > 0 && kSizes[k < NFFT ? k : k - SL_LS] >= f->min_sz
+                    : ((k == SL_JPT || k == SL_LPT) ? f->n_pt > 0 : true);
             if (!launched) continue;
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, f->pev[k][0], f->pev[k][1]) != hipSuccess) continue;
