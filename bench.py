@@ -421,7 +421,7 @@ def main_c3(args, torch):
     ferr = (fp.flux - flux) * np.sqrt(np.maximum(fp.iv, 1e-300))
     stages = {'render': (dm['render'], ab['render']),
               'assemble (pair dots + rhs)': (dm['assemble'], ab['assemble']),
-              'solve (block Cholesky + PCG)': (dm['solve'], ab['pcg_iteration'] * fp.iterations)}
+              'solve (block inverse + PCG)': (dm['solve'], ab['pcg_iteration'] * fp.iterations)}
     kname = max(stages, key=lambda k: stages[k][0])
     ms, nbytes = stages[kname]
     achieved = nbytes / (ms / 1e3) / 1e9

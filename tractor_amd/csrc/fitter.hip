@@ -397,32 +397,41 @@ __device__ __forceinline__ double wave_allsum_d(double v) {
 //   flux  = render * inverr
 //   shape = ((R_k - R_0) * (counts / 0.01)) * inverr
 //   b     = (data - counts * R_0) * inverr
+// One pixel's inputs to the normal equations: the model render, its four
+// neighbours (central differences for the position columns), the three
+// derivative renders of a galaxy, inverr and data.
+struct NeqIn {
+    float r0, dx, dy, g[3], e, d;
+};
+
 struct NeqPix {
     const float *R0, *D, *E;
     int P, PP, kd;
     float invP, counts, cstep, c00, c01, c10, c11;
-    __device__ __forceinline__ void add(int idx, float (&acc)[27], double &chi2) const {
+    __device__ __forceinline__ void load(int idx, NeqIn &in) const {
         // exact for PP < 2^24: (idx + 0.5) / P is within 0.5/P of y + 0.5
         const int y = (int)(((float)idx + 0.5f) * invP);
         const int x = idx - y * P;
-        const float r0 = R0[idx];
-        float dxp = 0.f, dyp = 0.f;
-        if (x > 0 && x < P - 1 && y > 0 && y < P - 1) {
-            dxp = (R0[idx - 1] - R0[idx + 1]) / 2.f;
-            dyp = (R0[idx - P] - R0[idx + P]) / 2.f;
-        }
-        const float e = E[idx];
-        float col[6];
-        col[0] = ((dxp * c00 + dyp * c10) * counts) * e;
-        col[1] = ((dxp * c01 + dyp * c11) * counts) * e;
-        col[2] = r0 * e;
-        if (kd) {
+        const bool inner = x > 0 && x < P - 1 && y > 0 && y < P - 1;
+        in.r0 = R0[idx];
+        const float l = inner ? R0[idx - 1] : 0.f, r = inner ? R0[idx + 1] : 0.f;
+        const float u = inner ? R0[idx - P] : 0.f, w = inner ? R0[idx + P] : 0.f;
+        in.dx = (l - r) / 2.f;
+        in.dy = (u - w) / 2.f;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) col[3 + k] = ((R0[(k + 1) * PP + idx] - r0) * cstep) * e;
-        } else {
-            col[3] = col[4] = col[5] = 0.f;
-        }
-        const float b = (D[idx] - r0 * counts) * e;
+        for (int k = 0; k < 3; ++k) in.g[k] = kd ? R0[(k + 1) * PP + idx] : 0.f;
+        in.e = E[idx];
+        in.d = D[idx];
+    }
+    __device__ __forceinline__ void add(const NeqIn &in, float (&acc)[27], double &chi2) const {
+        const float e = in.e, r0 = in.r0;
+        float col[6];
+        col[0] = ((in.dx * c00 + in.dy * c10) * counts) * e;
+        col[1] = ((in.dx * c01 + in.dy * c11) * counts) * e;
+        col[2] = r0 * e;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) col[3 + k] = kd ? ((in.g[k] - r0) * cstep) * e : 0.f;
+        const float b = (in.d - r0 * counts) * e;
         int q = 0;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
@@ -437,10 +446,13 @@ struct NeqPix {
 // J^T W J, J^T W r (FP32 products and sums) and chi^2 (FP64) for one source
 // per NEQ_TB-thread block, grid-stride over the sources in order of decreasing
 // P (sources span 64x in pixels; big ones first keeps the tail short).  Each
-// thread takes four pixels per step so their loads are in flight together;
-// each wave reduce-scatters its 27 sums (wave_reduce_scatter32) and the waves
+// thread takes four pixels per step and issues all their loads before any of
+// their arithmetic (written as four inlined per-pixel steps, the compiler
+// interleaved loads and arithmetic: 2.9 TB/s against 4.0 on one config-2
+// iteration); each wave reduce-scatters its 27 sums (wave_reduce_scatter32) and the waves
 // combine through LDS.
 #define NEQ_TB 512   // threads per source in k_fit_normal_eq
+#define NEQ_U 4      // pixels per thread in flight
 
 __global__ void __launch_bounds__(NEQ_TB)
 k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
@@ -473,13 +485,19 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         for (int k = 0; k < 27; ++k) acc[k] = 0.f;
         double chi2 = 0.0;
         int idx = threadIdx.x;
-        for (; idx + 3 * NEQ_TB < px.PP; idx += 4 * NEQ_TB) {
-            px.add(idx, acc, chi2);
-            px.add(idx + NEQ_TB, acc, chi2);
-            px.add(idx + 2 * NEQ_TB, acc, chi2);
-            px.add(idx + 3 * NEQ_TB, acc, chi2);
+        for (; idx + (NEQ_U - 1) * NEQ_TB < px.PP; idx += NEQ_U * NEQ_TB) {
+            // all the pixels' loads issue before any of their arithmetic
+            NeqIn in[NEQ_U];
+#pragma unroll
+            for (int u = 0; u < NEQ_U; ++u) px.load(idx + u * NEQ_TB, in[u]);
+#pragma unroll
+            for (int u = 0; u < NEQ_U; ++u) px.add(in[u], acc, chi2);
         }
-        for (; idx < px.PP; idx += NEQ_TB) px.add(idx, acc, chi2);
+        for (; idx < px.PP; idx += NEQ_TB) {
+            NeqIn in;
+            px.load(idx, in);
+            px.add(in, acc, chi2);
+        }
         float v32[32];
 #pragma unroll
         for (int k = 0; k < 32; ++k) v32[k] = k < 27 ? acc[k] : 0.f;
