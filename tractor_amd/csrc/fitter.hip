@@ -423,7 +423,8 @@ struct NeqPix {
         in.e = E[idx];
         in.d = D[idx];
     }
-    __device__ __forceinline__ void add(const NeqIn &in, float (&acc)[27], double &chi2) const {
+    __device__ __forceinline__ void add(const NeqIn &in, float (&acc)[27], double &chi2,
+                                        double &chi0) const {
         const float e = in.e, r0 = in.r0;
         float col[6];
         col[0] = ((in.dx * c00 + in.dy * c10) * counts) * e;
@@ -440,6 +441,10 @@ struct NeqPix {
 #pragma unroll
         for (int i = 0; i < 6; ++i) acc[21 + i] += col[i] * b;
         chi2 += (double)b * (double)b;
+        if (!kd) {
+            const float c0 = in.d * e;   // the chi of a zero model
+            chi0 += (double)c0 * (double)c0;
+        }
     }
 };
 
@@ -461,7 +466,7 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
                 const float *__restrict__ renders, const double *__restrict__ params,
                 const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
                 double cdi11, double *__restrict__ neq, const int32_t *__restrict__ order) {
-    __shared__ double red[NEQ_TB / 64][28];
+    __shared__ double red[NEQ_TB / 64][29];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = blockIdx.x; i < S; i += gridDim.x) {
         const int s = order[i];
@@ -483,7 +488,7 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         float acc[27];
 #pragma unroll
         for (int k = 0; k < 27; ++k) acc[k] = 0.f;
-        double chi2 = 0.0;
+        double chi2 = 0.0, chi0 = 0.0;
         int idx = threadIdx.x;
         for (; idx + (NEQ_U - 1) * NEQ_TB < px.PP; idx += NEQ_U * NEQ_TB) {
             // all the pixels' loads issue before any of their arithmetic
@@ -491,12 +496,12 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
 #pragma unroll
             for (int u = 0; u < NEQ_U; ++u) px.load(idx + u * NEQ_TB, in[u]);
 #pragma unroll
-            for (int u = 0; u < NEQ_U; ++u) px.add(in[u], acc, chi2);
+            for (int u = 0; u < NEQ_U; ++u) px.add(in[u], acc, chi2, chi0);
         }
         for (; idx < px.PP; idx += NEQ_TB) {
             NeqIn in;
             px.load(idx, in);
-            px.add(in, acc, chi2);
+            px.add(in, acc, chi2, chi0);
         }
         float v32[32];
 #pragma unroll
@@ -505,10 +510,14 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         const int q = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 +
                       ((lane >> 2) & 1) * 2 + ((lane >> 1) & 1);
         chi2 = wave_allsum_d(chi2);
+        chi0 = wave_allsum_d(chi0);
         if (!(lane & 1) && q < 27) red[wid][q] = (double)sum;
-        if (lane == 0) red[wid][27] = chi2;
+        if (lane == 0) {
+            red[wid][27] = chi2;
+            red[wid][28] = chi0;
+        }
         __syncthreads();
-        if (threadIdx.x < 28) {
+        if (threadIdx.x < 29) {
             double t = 0.0;
             for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
             neq[32 * s + threadIdx.x] = t;
@@ -685,6 +694,7 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
         trh_fft_job &J = sj[threadIdx.x];
         if (kd == 0) {
             build_point_job(J, p[0], p[1], P, ph, pw);
+            J.pad0 = 32 * s + 28;   // the window's zero-model chi^2 in neq
             b = PTB;
         } else {
             const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
@@ -698,7 +708,15 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
         if (acc && b >= 0) {
             const int sl = b == PTB ? SL_LPT : SL_LS + b;
             atomicAdd(&acc[2 * sl], 1ull);
-            atomicAdd(&acc[2 * sl + 1], 8ull * P * P);  // data + inverr read
+            // data + inverr read: the window, or for a point source the
+            // stamp's support in it (3 pixels of Lanczos taps around it)
+            unsigned long long px = (unsigned long long)P * P;
+            if (b == PTB) {
+                const int w = max(0, min(P, J.sx + pw + 3) - max(0, J.sx - 3));
+                const int hh = max(0, min(P, J.sy + ph + 3) - max(0, J.sy - 3));
+                px = (unsigned long long)w * hh;
+            }
+            atomicAdd(&acc[2 * sl + 1], 8ull * px);
         }
     }
     __shared__ int8_t built[PREP_TB];
@@ -980,13 +998,15 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
     int grid = f->h->n_cu * wg;
     if (bound < grid) grid = bound;
     if (chi2)
+        // the window's zero-model chi^2 comes from the normal equations of
+        // the same iteration (neq[32 s + 28], indexed by the job's pad0)
         hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds, st, jobs, list,
                            count, bound, f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data,
-                           f->inverr, chi2);
+                           f->inverr, chi2, f->neq);
     else
         hipLaunchKernelGGL(k_render_point<TRH_STORE>, dim3(grid), dim3(256), lds, st, jobs, list,
                            count, bound, f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data,
-                           f->inverr, chi2);
+                           f->inverr, chi2, nullptr);
     return hip_check(hipGetLastError(), "k_render_point");
 }
 

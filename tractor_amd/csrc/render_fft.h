@@ -663,7 +663,8 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
                                  const float *__restrict__ stamp, bool load_stamp, int ph, int pw,
                                  float *lstamp, float *tmp, RenderScratch &S,
                                  float *__restrict__ out, const float *__restrict__ data,
-                                 const float *__restrict__ inverr, double *__restrict__ chi2) {
+                                 const float *__restrict__ inverr, double *__restrict__ chi2,
+                                 const double *__restrict__ chi0) {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int64_t off = Jp->out_offset;
     const int stride = Jp->out_stride;
@@ -710,7 +711,34 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
         }
         return a;
     };
-    if (MODE == TRH_CHI2) {
+    if (MODE == TRH_CHI2 && chi0) {
+        // chi0[pad0] = sum (data inverr)^2 over the whole window (the chi^2 of
+        // a zero model), so only the shifted stamp's support is visited:
+        // chi^2 = chi0 + sum_support [((d - m scale) ie)^2 - (d ie)^2]
+        const int rh = ry1 - ry0;
+        const int ns = (cw > 0 && rh > 0) ? cw * rh : 0;
+        for (int base = tid; base < ns; base += 4 * nthr) {
+            float dv[4], iv[4];
+            int oyv[4], oxv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idc = min(base + u * nthr, ns - 1);
+                const int r = idc / cw;
+                oyv[u] = ry0 + r;
+                oxv[u] = cx0 + (idc - r * cw);
+                const int64_t g = off + (int64_t)oyv[u] * stride + oxv[u];
+                dv[u] = data[g];
+                iv[u] = inverr[g];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float chi = (dv[u] - yval(oyv[u], oxv[u]) * scale) * iv[u];
+                const float c0 = dv[u] * iv[u];
+                if (base + u * nthr < ns) acc += (double)chi * (double)chi - (double)c0 * (double)c0;
+            }
+        }
+        if (tid == 0) acc += chi0[Jp->pad0];
+    } else if (MODE == TRH_CHI2) {
         // four pixels per thread per step, their data / inverr loads issued
         // together (indices clamped to the last pixel, extra terms dropped)
         for (int base = tid; base < n; base += 4 * nthr) {
@@ -794,13 +822,15 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
 }
 
 // MODE: TRH_STORE / TRH_CHI2 fixed at compile time, or -1 (read from each job).
+// chi0 (CHI2 only, may be NULL): per-window chi^2 of a zero model, at
+// chi0[job.pad0]; with it a chi^2 render visits only the stamp's support.
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
                const int *__restrict__ count, int njobs, const float *__restrict__ stamp,
                int ph, int pw, int maxw, float *__restrict__ out,
                const float *__restrict__ data, const float *__restrict__ inverr,
-               double *__restrict__ chi2) {
+               double *__restrict__ chi2, const double *__restrict__ chi0) {
     extern __shared__ float2 smem[];
     __shared__ RenderScratch S;
     float *lstamp = reinterpret_cast<float *>(smem);
@@ -813,7 +843,7 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
         // the job's psf field selects one of the stamps at stamp + psf * ph * pw
         const int psf = jobs[jid].psf;
         render_point_one<MODE>(jobs + jid, stamp + (size_t)psf * ph * pw, psf != cur, ph, pw,
-                               lstamp, tmp, S, out, data, inverr, chi2);
+                               lstamp, tmp, S, out, data, inverr, chi2, chi0);
         cur = psf;
     }
 }

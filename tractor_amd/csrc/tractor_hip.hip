@@ -590,11 +590,11 @@ extern "C" int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, in
     if (!out && chi2 && data && inverr)
         hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds,
                            (hipStream_t)stream, jobs, nullptr, nullptr, njobs, stamp, ph, pw, maxw,
-                           out, data, inverr, chi2);
+                           out, data, inverr, chi2, nullptr);
     else
         hipLaunchKernelGGL(k_render_point<-1>, dim3(grid), dim3(256), lds, (hipStream_t)stream,
                            jobs, nullptr, nullptr, njobs, stamp, ph, pw, maxw, out, data, inverr,
-                           chi2);
+                           chi2, nullptr);
     return hip_check(hipGetLastError(), "k_render_point");
 }
 
