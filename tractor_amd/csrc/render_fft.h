@@ -684,57 +684,78 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     const int ry0 = max(0, sy - 3), ry1 = min(mh, sy + ph + 3);
     const int cw = cx1 - cx0;
     // x pass over the stamp rows the y taps reach, [rlo, rhi), and the covered
-    // columns: tmp[r][ox] = sum_u fx[u] stamp[r][ox-sx-3+u]
+    // columns: tmp[ox][r] = sum_u fx[u] stamp[r][ox-sx-3+u], stored by
+    // column (row stride ts odd).  A thread takes one stamp row and PB
+    // consecutive columns (PB + 6 stamp reads for PB outputs); threads run
+    // along r, which the stamp's odd row stride spreads over the LDS banks.
+    constexpr int PB = 4;
+    const int ts = ph | 1;
     const int rlo = max(0, -sy - 3), rhi = min(ph, mh - sy + 3);
-    for (int idx = tid; cw > 0 && idx < (rhi - rlo) * cw; idx += nthr) {
-        const int rr = idx / cw;
-        const int r = rlo + rr;
-        const int ox = cx0 + (idx - rr * cw);
-        float a = 0.f;
+    const int nrow = rhi - rlo, ncb = (cw + PB - 1) / PB;
+    for (int idx = tid; cw > 0 && idx < nrow * ncb; idx += nthr) {
+        const int cb = idx / nrow;
+        const int r = rlo + (idx - cb * nrow);
+        const int ox0 = cx0 + PB * cb;
+        float v[PB + 6];
 #pragma unroll
-        for (int u = 0; u < 7; ++u) {
-            const int c = ox - sx - 3 + u;
-            if (c >= 0 && c < pw) a += fx[u] * lstamp[r * pw + c];
+        for (int k = 0; k < PB + 6; ++k) {
+            const int c = ox0 - sx - 3 + k;
+            v[k] = (c >= 0 && c < pw) ? lstamp[r * pw + c] : 0.f;
         }
-        tmp[r * mw + ox] = a;
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            float a = 0.f;
+#pragma unroll
+            for (int u = 0; u < 7; ++u) a += fx[u] * v[j + u];
+            if (ox0 + j < cx1) tmp[(ox0 + j) * ts + r] = a;
+        }
     }
     __syncthreads();
     double acc = 0.0;
     const int n = mh * mw;
+    // the y taps of an output pixel read tmp rows inside [rlo, rhi)
     auto yval = [&](int oy, int ox) -> float {
         float a = 0.f;
         if (ox < cx0 || ox >= cx1 || oy < ry0 || oy >= ry1) return a;
 #pragma unroll
         for (int t = 0; t < 7; ++t) {
             const int r = oy - sy - 3 + t;
-            if (r >= 0 && r < ph) a += fy[t] * tmp[r * mw + ox];
+            if (r >= 0 && r < ph) a += fy[t] * tmp[ox * ts + r];
         }
         return a;
     };
     if (MODE == TRH_CHI2 && chi0) {
         // chi0[pad0] = sum (data inverr)^2 over the whole window (the chi^2 of
         // a zero model), so only the shifted stamp's support is visited:
-        // chi^2 = chi0 + sum_support [((d - m scale) ie)^2 - (d ie)^2]
-        const int rh = ry1 - ry0;
-        const int ns = (cw > 0 && rh > 0) ? cw * rh : 0;
-        for (int base = tid; base < ns; base += 4 * nthr) {
-            float dv[4], iv[4];
-            int oyv[4], oxv[4];
+        // chi^2 = chi0 + sum_support [((d - m scale) ie)^2 - (d ie)^2].
+        // A thread takes one column and PB consecutive rows (PB + 6 reads of
+        // tmp for PB model pixels; threads run along the columns).
+        const int rh = ry1 - ry0, nrb = (rh + PB - 1) / PB;
+        const int nit = (cw > 0 && rh > 0) ? cw * nrb : 0;
+        for (int idx = tid; idx < nit; idx += nthr) {
+            const int rb = idx / cw;
+            const int ox = cx0 + (idx - rb * cw);
+            const int oy0 = ry0 + PB * rb;
+            float dv[PB], iv[PB], v[PB + 6];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idc = min(base + u * nthr, ns - 1);
-                const int r = idc / cw;
-                oyv[u] = ry0 + r;
-                oxv[u] = cx0 + (idc - r * cw);
-                const int64_t g = off + (int64_t)oyv[u] * stride + oxv[u];
-                dv[u] = data[g];
-                iv[u] = inverr[g];
+            for (int k = 0; k < PB; ++k) {
+                const int64_t g = off + (int64_t)min(oy0 + k, ry1 - 1) * stride + ox;
+                dv[k] = data[g];
+                iv[k] = inverr[g];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float chi = (dv[u] - yval(oyv[u], oxv[u]) * scale) * iv[u];
-                const float c0 = dv[u] * iv[u];
-                if (base + u * nthr < ns) acc += (double)chi * (double)chi - (double)c0 * (double)c0;
+            for (int k = 0; k < PB + 6; ++k) {
+                const int r = oy0 - sy - 3 + k;
+                v[k] = (r >= rlo && r < rhi) ? tmp[ox * ts + r] : 0.f;
+            }
+#pragma unroll
+            for (int j = 0; j < PB; ++j) {
+                float m = 0.f;
+#pragma unroll
+                for (int t = 0; t < 7; ++t) m += fy[t] * v[j + t];
+                const float chi = (dv[j] - m * scale) * iv[j];
+                const float c0 = dv[j] * iv[j];
+                if (oy0 + j < ry1) acc += (double)chi * (double)chi - (double)c0 * (double)c0;
             }
         }
         if (tid == 0) acc += chi0[Jp->pad0];
@@ -849,7 +870,7 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
 }
 
 __host__ __device__ inline size_t point_lds_bytes(int ph, int pw, int maxw) {
-    return sizeof(float) * ((((size_t)ph * pw + 3) & ~(size_t)3) + (size_t)ph * maxw);
+    return sizeof(float) * ((((size_t)ph * pw + 3) & ~(size_t)3) + (size_t)(ph | 1) * maxw);
 }
 
 }  // namespace trh
