@@ -295,6 +295,21 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  *                  (p.Ap <= 0); status[1] = iterations taken; launches after
  *                  the stop return at once.  work: 4 n + 5 ntile + 2 doubles,
  *                  kept between calls; its last two hold |r|^2 and |b|^2.
+ *   fp_fitstats:   Optimizer._get_fitstats (tractor/optimize.py:233-376) for
+ *                  one image, one unit-flux model per source: out[10 s ..] =
+ *                  prochi2, pronpix, promasked, the fracflux ratio
+ *                  sum(|m/c - u| |u|) / sum(u^2), sum(|u|) / |c|, proflux,
+ *                  npix, sum |u| over the window, 1 if counted, and the
+ *                  inverse variance sum (u scale inverr)^2 (_get_iv,
+ *                  optimize.py:378-423); sums over u != 0 and inverr > 0,
+ *                  zeros for counts 0 or no such pixel.  model: the image
+ *                  model without sky; chi = (data - model - sky) inverr;
+ *                  counts[s] = flux x photocal scale.  Over several images
+ *                  the columns add, then profracflux = sum(ratio) /
+ *                  max(1, sum(column 4)) and fracin = sum(column 7) /
+ *                  max(1, sum(column 8)).
+ *   fp_imstats:    out[0] = sum chi^2 over the image, out[1] = number of
+ *                  pixels with inverr > 0 (optimize.py:244-250).
  *   fp_add_patches: img += sum_s flux[s] u_s (model image; Tractor.
  *                  getModelImage, tractor/engine.py:424-445, for these
  *                  patches).
@@ -336,6 +351,12 @@ int trh_fp_pcg(trh_handle *h, int n, const int64_t *row_ptr, const int32_t *col,
                const double *vals, const int32_t *tile_start, const int64_t *block_off, int ntile,
                int max_n, const double *blocks, const double *b, double tol, int it0, int iters,
                double *x, double *work, int32_t *status, void *stream);
+int trh_fp_fitstats(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
+                    const float *patches, const double *counts, const float *model,
+                    const float *data, const float *inverr, int img_w, double scale, double sky,
+                    double *out, void *stream);
+int trh_fp_imstats(trh_handle *h, int64_t npix, const float *model, const float *data,
+                   const float *inverr, double sky, double *out, void *stream);
 int trh_fp_add_patches(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
                        const float *patches, const float *flux, int img_w, float *img,
                        void *stream);

@@ -21,6 +21,7 @@ __all__ = [
     'lanczos_taps3', 'correlate7f', 'lanczos_shift_image',
     'render_galaxy_fft', 'galaxy_patch_halfsize', 'mog_grid', 'mog_masked',
     'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch', 'forced_unit_patch', 'forced_phot_lsq',
+    'forced_fitstats',
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
@@ -1058,3 +1059,56 @@ def forced_phot_lsq(patches, data, inverr):
     f = spl.spsolve(N, A.T @ b)
     f[empty] = 0.
     return f, iv
+
+
+def forced_fitstats(patches, counts, model, data, inverr, scale=1., sky=0.):
+    """Optimizer._get_fitstats (optimize.py:233-376) for one image and one
+    unit-flux model per source, FP64.  patches: (x0, y0, u) or None per
+    source; counts: each source's counts (flux x photocal scale); model: the
+    image model without sky (sum_s counts_s u_s plus fixed sources); chi =
+    (data - (model + sky)) inverr.  Returns per-source arrays prochi2,
+    pronpix, promasked, profracflux, proflux, npix, fracin and the image
+    imchisq, imnpix; also iv = sum (u scale inverr)^2 (_get_iv,
+    optimize.py:378-423).  Over several images the sums add, except
+    profracflux = sum_img(num / den2) / max(1, sum_img den) and fracin =
+    sum_img fracin / max(1, number of images counted): the per-image parts
+    are returned as well (ffnum, ffden, counted)."""
+    model = np.asarray(model, np.float64)
+    data = np.asarray(data, np.float64)
+    inverr = np.asarray(inverr, np.float64)
+    chi = (data - (model + sky)) * inverr
+    n = len(patches)
+    z = lambda: np.zeros(n)
+    fs = dict(prochi2=z(), pronpix=z(), promasked=z(), ffnum=z(), ffden=z(), proflux=z(),
+              npix=np.zeros(n, int), fracin_num=z(), counted=z(), iv=z())
+    for s, pt in enumerate(patches):
+        if pt is None:
+            continue
+        x0, y0, u = pt
+        u = np.asarray(u, np.float64)
+        h, w = u.shape
+        sl = slice(y0, y0 + h), slice(x0, x0 + w)
+        ie = inverr[sl]
+        fs['iv'][s] = np.sum((u * scale * ie) ** 2)
+        c = counts[s]
+        if c == 0:
+            continue
+        nz = (u != 0) & (ie > 0)
+        if not nz.any():
+            continue
+        au = np.abs(u)
+        m = model[sl]
+        fs['prochi2'][s] = np.sum(au[nz] * chi[sl][nz] ** 2)
+        fs['pronpix'][s] = np.sum(au[nz])
+        fs['promasked'][s] = np.sum(au[ie == 0])
+        fs['ffnum'][s] = np.sum((np.abs(m / c - u) * au)[nz]) / np.sum((u ** 2)[nz])
+        fs['ffden'][s] = np.sum(au[nz]) / abs(c)
+        fs['proflux'][s] = np.sum((np.abs((m - u * c) / scale) * au)[nz])
+        fs['npix'][s] = int(nz.sum())
+        fs['fracin_num'][s] = np.sum(au)
+        fs['counted'][s] = 1.
+    fs['profracflux'] = fs['ffnum'] / np.maximum(1., fs['ffden'])
+    fs['fracin'] = fs['fracin_num'] / np.maximum(1., fs['counted'])
+    fs['imchisq'] = float(np.sum(chi ** 2))
+    fs['imnpix'] = int(np.sum(inverr > 0))
+    return fs

@@ -312,3 +312,65 @@ def test_lsqr_update_matches_dense_lstsq():
     b = tr.getChiImage(0).ravel()
     Xs = np.linalg.lstsq(A / sc, b, rcond=None)[0] / sc
     np.testing.assert_allclose(X, Xs, rtol=1e-3, atol=1e-4 * np.abs(Xs).max())
+
+
+def test_forced_phot_fitstats_and_variance():
+    """optimize_forced_photometry(fitstats=True, variance=True) (the FitStats
+    of optimize.py:233-376 and _get_iv, 378-423, on the device) against the
+    oracle restatement over the optimizer's own unit-flux models: two
+    images, one with a constant sky and a masked block, two overlapping point
+    sources and an isolated one."""
+    import oracle as O
+    W, H = 80, 60
+    rng = np.random.default_rng(8)
+    tims = []
+    for k, (sig, skyv) in enumerate(((2., 0.), (3., 1.5))):
+        ie = np.ones((H, W))
+        if k == 1:
+            ie[20:26, 30:41] = 0.
+        tims.append(T.Image(data=np.zeros((H, W)), inverr=ie,
+                            psf=T.NCircularGaussianPSF([sig], [1.]), photocal=T.LinearPhotoCal(2.),
+                            sky=T.ConstantSky(skyv)))
+    srcs = [T.PointSource(T.PixPos(30., 25.), T.Flux(40.)),
+            T.PointSource(T.PixPos(34., 27.), T.Flux(25.)),
+            T.PointSource(T.PixPos(65., 45.), T.Flux(60.))]
+    tr = T.Tractor(tims, srcs)
+    for tim, mod in zip(tims, tr.getModelImages()):
+        tim.data = mod + rng.normal(size=mod.shape)
+    for s in srcs:
+        s.getBrightness().setValue(10.)
+        s.freezeAllBut('brightness')
+    tr.freezeParam('images')
+    res = tr.optimize_forced_photometry(fitstats=True, variance=True)
+    fs = res.fitstats
+    opt = tr.optimizer
+    umodels = opt._get_umodels(tr, srcs, tims, 0.)
+    fluxes = np.array(tr.catalog.getParams())
+    per = []
+    for tim, umods in zip(tims, umodels):
+        scale = tim.getPhotoCal().getScale()
+        pts = []
+        for um in umods:
+            p = np.asarray(um.patch, np.float64)
+            X0, Y0 = max(um.x0, 0), max(um.y0, 0)
+            X1, Y1 = min(um.x0 + p.shape[1], W), min(um.y0 + p.shape[0], H)
+            pts.append((X0, Y0, p[Y0 - um.y0:Y1 - um.y0, X0 - um.x0:X1 - um.x0]))
+        skyv = tim.getSky().val
+        mod = np.asarray(tr.getModelImage(tim), np.float64) - skyv
+        per.append(O.forced_fitstats(pts, fluxes * scale, mod, tim.getImage(), tim.getInvError(),
+                                     scale=scale, sky=skyv))
+    tot = lambda k: sum(p[k] for p in per)
+    np.testing.assert_allclose(res.IV, tot('iv'), rtol=1e-5)
+    for k in ('prochi2', 'pronpix', 'promasked', 'proflux'):
+        np.testing.assert_allclose(getattr(fs, k), tot(k), rtol=1e-5, atol=1e-9, err_msg=k)
+    np.testing.assert_array_equal(fs.npix, tot('npix'))
+    np.testing.assert_allclose(fs.profracflux, tot('ffnum') / np.maximum(1., tot('ffden')),
+                               rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(fs.fracin, tot('fracin_num') / np.maximum(1., tot('counted')),
+                               rtol=1e-5)
+    np.testing.assert_allclose(fs.imchisq, [p['imchisq'] for p in per], rtol=1e-5)
+    np.testing.assert_array_equal(fs.imnpix, [p['imnpix'] for p in per])
+    np.testing.assert_allclose(fs.sky, [0., 1.5])
+    # the overlapping pair sees each other's flux, the isolated source none
+    assert fs.profracflux[0] > 0.05 and fs.profracflux[1] > 0.05
+    assert fs.profracflux[2] < 1e-6

@@ -63,6 +63,45 @@ def test_forced_photometry_matches_oracle():
     ok = (err <= 1e-4 * np.abs(ref)) | (err <= 0.01 * sig)
     print('flux parity: %d/%d; max err/sigma %.3g' % (ok.sum(), n, np.max(err / sig)), flush=True)
     assert ok.all()
+    # FitStats (optimize.py:233-376) of the device fit.  (1) The oracle
+    # restatement over the device's own patches at the device fluxes: the
+    # same pixels, so every statistic agrees to FP32 products / FP64 sums and
+    # npix (exact non-zeros) exactly.  (2) Over the oracle's patches, which
+    # agree to 2e-5 of their peaks: the profile-weighted sums to ~1e-3 (npix
+    # is left out there: FP32 and FP64 FFT renders disagree on exact zeros of
+    # far-wing pixels).
+    fs = fp.fitstats()
+    dev_patches = []
+    for s in range(n):
+        i = inv[s]
+        x0, y0, ww, hh = (int(v) for v in fp.win[i])
+        dev_patches.append(None if ww * hh == 0 else
+                           (x0, y0, buf[fp.poff[i]:fp.poff[i] + ww * hh].reshape(hh, ww)))
+
+    def model_of(pts):
+        m = np.zeros((size, size))
+        for pt, f in zip(pts, got):
+            if pt is not None:
+                x0, y0, p = pt
+                m[y0:y0 + p.shape[0], x0:x0 + p.shape[1]] += f * p
+        return m
+    d64, e64 = data.astype(np.float64), inverr.astype(np.float64)
+    ref_d = O.forced_fitstats(dev_patches, got, model_of(dev_patches), d64, e64)
+    for key in ('prochi2', 'pronpix', 'proflux', 'fracin', 'profracflux', 'promasked'):
+        np.testing.assert_allclose(getattr(fs, key), ref_d[key], rtol=1e-5, atol=2e-5,
+                                   err_msg=key)
+    np.testing.assert_array_equal(fs.npix, ref_d['npix'])
+    assert fs.imchisq[0] == pytest.approx(ref_d['imchisq'], rel=1e-6)
+    assert fs.imnpix[0] == ref_d['imnpix']
+    ref_fs = O.forced_fitstats(patches, got, model_of(patches), d64, e64)
+    # (proflux and profracflux of isolated sources sit at the rounding floor
+    # of m - u c in the float32 model image: absolute 2e-5)
+    for key, rtol, atol in (('prochi2', 3e-3, 1e-6), ('pronpix', 1e-4, 1e-6),
+                            ('proflux', 3e-3, 2e-5), ('fracin', 1e-4, 1e-6),
+                            ('profracflux', 3e-3, 2e-5)):
+        np.testing.assert_allclose(getattr(fs, key), ref_fs[key], rtol=rtol, atol=atol,
+                                   err_msg=key)
+    assert fs.imchisq[0] == pytest.approx(ref_fs['imchisq'], rel=1e-4)
 
 
 def test_device_overlap_pattern_matches_host_rule():

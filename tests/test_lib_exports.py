@@ -76,6 +76,8 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_fp_precond': (N, N, N, 1, 8, N, N, N, N),
         'trh_fp_spmv': (N, 4, N, N, N, N, N, N),
         'trh_fp_pcg': (N, 4, N, N, N, N, N, 1, 8, N, N, 1e-12, 0, 1, N, N, N, N),
+        'trh_fp_fitstats': (N, 1, N, N, N, N, N, N, N, 10, 1., 0., N, N),
+        'trh_fp_imstats': (N, 1, N, N, N, 0., N, N),
         'trh_fp_add_patches': (N, 1, N, N, N, N, 10, N, N),
         'trh_cnn_conv_fwd': (N, N, N, N, N, 1, 32, 32, 1, N),
         'trh_cnn_conv_bwd_data': (N, N, N, N, N, 1, 32, 32, N),

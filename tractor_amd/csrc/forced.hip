@@ -758,6 +758,108 @@ k_fp_jobs_bucketed(int n, const int64_t *__restrict__ border, const int32_t *__r
     dst[j] = J;
 }
 
+// Optimizer._get_fitstats (optimize.py:233-376) for one image, one unit-flux
+// model per source: a 256-thread workgroup per source over its window (the
+// window flattened over the threads), FP32 pixel products and FP64 sums.
+// out[10 s + ...]: prochi2, pronpix, promasked, the fracflux ratio
+// sum(|m/c - u| |u|) / sum(u^2), its denominator term sum(|u|) / |c|,
+// proflux, npix, sum |u| over the window (fracin), 1 if the source was
+// counted, and iv = sum (u scale inverr)^2 (_get_iv, optimize.py:378-423).
+// The sums run over nz = {u != 0, inverr > 0}; a source with counts 0 or an
+// empty nz gets zeros (iv aside).  chi = (data - (model + sky)) inverr;
+// model is the image model without sky.
+__global__ void __launch_bounds__(256)
+k_fp_fitstats(int n, const int32_t *__restrict__ win, const int64_t *__restrict__ poff,
+              const float *__restrict__ patches, const double *__restrict__ counts,
+              const float *__restrict__ model, const float *__restrict__ data,
+              const float *__restrict__ inverr, int img_w, double scale, double sky,
+              double *__restrict__ out) {
+    constexpr int NS = 10;
+    __shared__ double red[NS][4];
+    const int s = blockIdx.x;
+    const int x0 = win[4 * s], y0 = win[4 * s + 1], w = win[4 * s + 2], h = win[4 * s + 3];
+    const double c = counts[s];
+    const float fs = (float)scale, fsky = (float)sky;
+    double v[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) v[k] = 0.0;
+    if (w > 0 && h > 0) {
+        const float *u = patches + poff[s];
+        const int m = w * h;
+        FlatWalk fw(threadIdx.x, w, 256);
+        for (int i = threadIdx.x; i < m; i += 256, fw.step()) {
+            const int64_t ib = (int64_t)(y0 + fw.y) * img_w + x0 + fw.x;
+            const float ui = u[i], e = inverr[ib], md = model[ib];
+            const float t = ui * fs * e;
+            v[9] += (double)t * (double)t;
+            const float au = fabsf(ui);
+            v[7] += au;
+            if (e == 0.f) v[2] += au;
+            if (ui != 0.f && e > 0.f) {
+                const float chi = (data[ib] - (md + fsky)) * e;
+                v[0] += (double)(au * chi * chi);
+                v[1] += au;
+                v[3] += fabs((double)md / c - (double)ui) * au;
+                v[4] += (double)ui * (double)ui;
+                v[5] += au / fabs(c);
+                v[6] += fabs(((double)md - (double)ui * c) / scale) * au;
+                v[8] += 1.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const double t = wave_sum_d(v[k]);
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        const int k = threadIdx.x;
+        double t = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        const double npix = red[8][0] + red[8][1] + red[8][2] + red[8][3];
+        const double b2 = red[4][0] + red[4][1] + red[4][2] + red[4][3];
+        const bool counted = c != 0.0 && npix > 0.0;
+        double r;
+        if (k == 9) r = t;
+        else if (!counted) r = 0.0;
+        else if (k == 3) r = t / b2;
+        else if (k == 4) r = (red[5][0] + red[5][1] + red[5][2] + red[5][3]);
+        else if (k == 5) r = (red[6][0] + red[6][1] + red[6][2] + red[6][3]);
+        else if (k == 6) r = npix;
+        else if (k == 8) r = 1.0;
+        else r = t;   // 0 prochi2, 1 pronpix, 2 promasked, 7 fracin
+        out[NS * (int64_t)s + k] = r;
+    }
+}
+
+// Image chi^2 and the number of pixels with inverr > 0 (the per-image part
+// of _get_fitstats, optimize.py:244-250): one workgroup, fixed order.
+__global__ void __launch_bounds__(1024)
+k_fp_imstats(int64_t npix, const float *__restrict__ model, const float *__restrict__ data,
+             const float *__restrict__ inverr, double sky, double *__restrict__ out) {
+    __shared__ double red[2][16];
+    const float fsky = (float)sky;
+    double c2 = 0.0, np = 0.0;
+    for (int64_t i = threadIdx.x; i < npix; i += 1024) {
+        const float e = inverr[i];
+        const float chi = (data[i] - (model[i] + fsky)) * e;
+        c2 += (double)chi * (double)chi;
+        np += e > 0.f ? 1.0 : 0.0;
+    }
+    c2 = wave_sum_d(c2);
+    np = wave_sum_d(np);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = c2;
+        red[1][threadIdx.x >> 6] = np;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int k = 0; k < 16; ++k) t += red[threadIdx.x][k];
+        out[threadIdx.x] = t;
+    }
+}
+
 }  // namespace
 
 extern "C" int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
@@ -953,4 +1055,27 @@ extern "C" int trh_fp_jobs_bucketed(trh_handle *h, int n, const int64_t *border,
     hipLaunchKernelGGL(k_fp_jobs_bucketed, dim3((n + 255) / 256), dim3(256), 0,
                        (hipStream_t)stream, n, border, bucket, bstart, poff, src, dst);
     return hip_check(hipGetLastError(), "k_fp_jobs_bucketed");
+}
+
+extern "C" int trh_fp_fitstats(trh_handle *h, int n, const int32_t *win, const int64_t *poff,
+                               const float *patches, const double *counts, const float *model,
+                               const float *data, const float *inverr, int img_w, double scale,
+                               double sky, double *out, void *stream) {
+    if (!h || n < 0 ||
+        (n > 0 && (!win || !poff || !patches || !counts || !model || !data || !inverr || !out)) ||
+        img_w <= 0 || !(scale != 0.0))
+        return set_err(TRH_E_ARG, "trh_fp_fitstats: bad arguments");
+    if (n == 0) return TRH_OK;
+    hipLaunchKernelGGL(k_fp_fitstats, dim3(n), dim3(256), 0, (hipStream_t)stream, n, win, poff,
+                       patches, counts, model, data, inverr, img_w, scale, sky, out);
+    return hip_check(hipGetLastError(), "k_fp_fitstats");
+}
+
+extern "C" int trh_fp_imstats(trh_handle *h, int64_t npix, const float *model, const float *data,
+                              const float *inverr, double sky, double *out, void *stream) {
+    if (!h || npix < 0 || (npix > 0 && (!model || !data || !inverr)) || !out)
+        return set_err(TRH_E_ARG, "trh_fp_imstats: bad arguments");
+    hipLaunchKernelGGL(k_fp_imstats, dim3(1), dim3(1024), 0, (hipStream_t)stream, npix, model,
+                       data, inverr, sky, out);
+    return hip_check(hipGetLastError(), "k_fp_imstats");
 }

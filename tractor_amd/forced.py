@@ -206,6 +206,12 @@ def mixture_table():
     return t
 
 
+class FitStats(object):
+    """Fit statistics of a forced-photometry fit (the attributes of
+    Optimizer._get_fitstats's FitStats, optimize.py:246-300)."""
+    pass
+
+
 class BatchForcedPhotometry(object):
     """Forced photometry of n sources on one image (device tensors data,
     inverr of shape (H, W), float32).  psf_basis: (G, ph, pw) stamps of the
@@ -487,6 +493,38 @@ class BatchForcedPhotometry(object):
                                           ptr(self.patches), ptr(f), self.W, ptr(out), self.stream),
               'fp_add_patches')
         return out
+
+    def fitstats(self, flux=None):
+        """Optimizer._get_fitstats (optimize.py:233-376) of the fit on the
+        device (trh_fp_fitstats, trh_fp_imstats): per source (caller order)
+        prochi2, pronpix, promasked, profracflux, proflux, npix, fracin, and
+        the image's imchisq, imnpix and sky (0: this class fits no sky).
+        flux: counts per source in the caller's order (default: the solved
+        fluxes)."""
+        flux = self.flux if flux is None else np.asarray(flux, np.float64)
+        lib, h, st, n = self.lib, self.h.value, self.stream, self.n
+        model = self.model_image(flux)
+        counts = self._t(np.asarray(flux, np.float64))[self.d_order].contiguous()
+        out = torch.empty((n, 10), dtype=torch.float64, device=self.dev)
+        im = torch.empty(2, dtype=torch.float64, device=self.dev)
+        check(lib.trh_fp_fitstats(h, n, ptr(self.d_win), ptr(self.d_poff), ptr(self.patches),
+                                  ptr(counts), ptr(model), ptr(self.data), ptr(self.inverr), self.W,
+                                  1., 0., ptr(out), st), 'fp_fitstats')
+        check(lib.trh_fp_imstats(h, self.H * self.W, ptr(model), ptr(self.data), ptr(self.inverr),
+                                 0., ptr(im), st), 'fp_imstats')
+        res = torch.empty_like(out)
+        res[self.d_order] = out
+        a = res.cpu().numpy()
+        imv = im.cpu().numpy()
+        fs = FitStats()
+        fs.prochi2, fs.pronpix, fs.promasked = a[:, 0], a[:, 1], a[:, 2]
+        fs.profracflux = a[:, 3] / np.maximum(1., a[:, 4])
+        fs.proflux, fs.npix = a[:, 5], a[:, 6].astype(int)
+        fs.fracin = a[:, 7] / np.maximum(1., a[:, 8])
+        fs.imchisq = np.array([imv[0]])
+        fs.imnpix = np.array([int(imv[1])])
+        fs.sky = np.array([0.])
+        return fs
 
     def run(self, tol=1e-12, maxiter=1000):
         """plan -> render -> assemble -> solve; wall times per stage (host

@@ -129,11 +129,84 @@ class Optimizer(object):
                 break
         result.ims0 = ims0
         result.ims1 = imsBest
-        if variance:
-            result.IV = self._get_iv(imgs, umodels, scales, Nsourceparams)
-        if fitstats:
-            result.fitstats = None
+        if variance or fitstats:
+            ims = imsBest if imsBest is not None else ims0
+            fluxes = tractor.catalog.getParams()
+            stats = [self._unit_stats(img, umods, scale, fluxes, im[1])
+                     for img, umods, scale, im in zip(imgs, umodels, scales, ims)]
+            if variance:
+                result.IV = np.zeros(Nsourceparams)
+                for a, _ in stats:
+                    result.IV[:len(a)] += a[:, 9]
+            if fitstats:
+                result.fitstats = self._get_fitstats(imgs, stats)
         return result
+
+    def _unit_stats(self, img, umods, scale, fluxes, mod):
+        """Per-source sums of Optimizer._get_fitstats (optimize.py:233-376)
+        and _get_iv (378-423) for one image on the device (trh_fp_fitstats,
+        trh_fp_imstats): the unit-flux patches clipped to the image and packed
+        back to back, the model with its sky taken out (the chi keeps it).
+        Returns ((n, 10) array, (chisq, npix) of the image)."""
+        import torch
+        from . import _lib
+        from ._lib import check, ptr
+        lib = _lib.load()
+        h = _lib.handle(0)
+        dev = torch.device('cuda', 0)
+        H, W = img.shape
+        n = len(umods)
+        win = np.zeros((n, 4), np.int32)
+        poff = np.zeros(n, np.int64)
+        parts = []
+        off = 0
+        for i, um in enumerate(umods):
+            if um is None or um.patch is None:
+                continue
+            p = np.asarray(um.patch, np.float32)
+            ph, pw = p.shape
+            X0, Y0 = max(um.x0, 0), max(um.y0, 0)
+            X1, Y1 = min(um.x0 + pw, W), min(um.y0 + ph, H)
+            if X1 <= X0 or Y1 <= Y0:
+                continue
+            sub = p[Y0 - um.y0:Y1 - um.y0, X0 - um.x0:X1 - um.x0]
+            win[i] = (X0, Y0, X1 - X0, Y1 - Y0)
+            poff[i] = off
+            off += sub.size
+            parts.append(sub.ravel())
+        sky = float(getattr(img.getSky(), 'val', 0.))
+        # counts from the catalog's fluxes (already clamped at minFlux)
+        counts = np.asarray(fluxes[:n], np.float64) * scale
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        d_p = t(np.concatenate(parts) if parts else np.zeros(1, np.float32))
+        d_model = t(np.asarray(mod, np.float32) - np.float32(sky))
+        d_data = t(np.asarray(img.getImage(), np.float32))
+        d_ie = t(np.asarray(img.getInvError(), np.float32))
+        out = torch.zeros((n, 10), dtype=torch.float64, device=dev)
+        im = torch.zeros(2, dtype=torch.float64, device=dev)
+        d_win, d_poff, d_counts = t(win), t(poff), t(counts)
+        check(lib.trh_fp_fitstats(h.value, n, ptr(d_win), ptr(d_poff), ptr(d_p), ptr(d_counts),
+                                  ptr(d_model), ptr(d_data), ptr(d_ie), W, float(scale), sky,
+                                  ptr(out), None), 'fp_fitstats')
+        check(lib.trh_fp_imstats(h.value, H * W, ptr(d_model), ptr(d_data), ptr(d_ie), sky,
+                                 ptr(im), None), 'fp_imstats')
+        return out.cpu().numpy(), im.cpu().numpy()
+
+    def _get_fitstats(self, imgs, stats):
+        """FitStats from the per-image sums (optimize.py:233-376): sums add
+        over images; profracflux = sum(ratio) / max(1, sum(flux fraction
+        terms)), fracin = sum(window profile) / max(1, images counted)."""
+        from .forced import FitStats
+        fs = FitStats()
+        a = sum(st for st, _ in stats)
+        fs.imchisq = np.array([im[0] for _, im in stats])
+        fs.imnpix = np.array([int(im[1]) for _, im in stats])
+        fs.sky = np.array([float(getattr(img.getSky(), 'val', 0.)) for img in imgs])
+        fs.prochi2, fs.pronpix, fs.promasked = a[:, 0], a[:, 1], a[:, 2]
+        fs.profracflux = a[:, 3] / np.maximum(1., a[:, 4])
+        fs.proflux, fs.npix = a[:, 5], a[:, 6].astype(int)
+        fs.fracin = a[:, 7] / np.maximum(1., a[:, 8])
+        return fs
 
     def _get_umodels(self, tractor, srcs, imgs, minsb, **kwargs):
         """Unit-flux patches per (image, source brightness) (optimize.py:147-222)."""
@@ -189,16 +262,3 @@ class Optimizer(object):
             chisq += (chi.astype(np.float64) ** 2).sum()
         return lnp - 0.5 * chisq, chis, ims
 
-    def _get_iv(self, imgs, umodels, scales, Nsourceparams):
-        """Inverse variances sum((umodel*scale*inverr)^2) (optimize.py:378-423)."""
-        IV = np.zeros(Nsourceparams)
-        for img, umods, scale in zip(imgs, umodels, scales):
-            ie = img.getInvError()
-            H, W = img.shape
-            for ui, um in enumerate(umods):
-                if um is None or um.patch is None:
-                    continue
-                mm = np.zeros((H, W))
-                um.addTo(mm)
-                IV[ui] += np.sum((mm * scale * ie) ** 2)
-        return IV
