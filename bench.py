@@ -424,6 +424,14 @@ def main_c3(args, torch):
               'solve (block inverse + PCG)': (dm['solve'], ab['pcg_iteration'] * fp.iterations)}
     kname = max(stages, key=lambda k: stages[k][0])
     ms, nbytes = stages[kname]
+    # FitStats of the fit (optimize.py:233-376), outside the timed step:
+    # model image + per-source profile-weighted sums + image chi^2
+    fp.fitstats()
+    torch.cuda.synchronize()
+    tf = time.perf_counter()
+    fsx = fp.fitstats()
+    torch.cuda.synchronize()
+    fitstats_ms = (time.perf_counter() - tf) * 1e3
     achieved = nbytes / (ms / 1e3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
@@ -462,7 +470,10 @@ def main_c3(args, torch):
                    'pcg_iterations': fp.iterations, 'pcg_rel_residual': fp.resid,
                    'nnz': int(fp.nnz), 'tiles': int(len(fp.tile_start) - 1),
                    'solver_max_err_over_sigma_vs_fp64_direct': solver_err,
-                   'flux_pull_rms_vs_truth': float(np.sqrt(np.mean(ferr ** 2)))},
+                   'flux_pull_rms_vs_truth': float(np.sqrt(np.mean(ferr ** 2))),
+                   'fitstats_ms': round(fitstats_ms, 3),
+                   'fitstats_median_prochi2_over_pronpix': float(np.median(
+                       fsx.prochi2[fsx.pronpix > 0] / fsx.pronpix[fsx.pronpix > 0]))},
     }
     print(json.dumps(out), flush=True)
 
