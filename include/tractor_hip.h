@@ -234,10 +234,18 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  * step, tractor/lsqr_optimizer.py:11-251, on one image of many overlapping
  * sources).  Unit-flux patches u_s live in one float32 buffer: patch s is
  * win[s] = (x0, y0, w, h) (inside the image, row-major) at poff[s].
- *   psf_combine:   out[s] = sum_q weights[s][q] * basis[idx[s][q]], q < 4,
- *                  arrays of `len` floats (bilinear PSF interpolation of
- *                  stamps or of spectra; replaces PixelizedPsfEx's
- *                  per-position stamp, tractor/psfex.py:481-520); n <= 65535.
+ *   psf_combine:   out[s] = sum_{q < nq} weights[s][q] * basis[idx[s][q]]
+ *                  (1 <= nq <= 16), arrays of `len` floats: a spatially
+ *                  varying PSF as a weighted sum of basis images, for stamps
+ *                  and, by linearity, their spectra -- bilinear interpolation
+ *                  of a PSF grid (nq = 4) or PSFEx's polynomial over its
+ *                  eigen-PSFs (PixelizedPsfEx, tractor/psfex.py:326-346,
+ *                  481-520); n <= 65535.
+ *   psfex_terms:   weights[q][ii] = dx^j dy^k, idx[q][ii] = ii for source
+ *                  order[q] (order may be NULL: identity; src rows x, y,
+ *                  ... at stride 6), dx = (x - x0) / xscale, dy likewise,
+ *                  ii = j + (degree + 1) k - k (k - 1) / 2 over j + k <=
+ *                  degree (PsfEx.polynomials, tractor/psfex.py:294-325).
  *   fp_keys:       sort key of each source (src rows: x, y, kind, log r_e,
  *                  ee1, ee2): its tile cell (tile x tile pixels of the
  *                  clipped position, row-major) * 64 + the Morton index of
@@ -317,7 +325,11 @@ int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, double damp, in
  * fp_pcg: the reference calls scipy's LSQR, lsqr_optimizer.py:586.)
  */
 int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
-                    const float *weights, const int32_t *idx, float *out, int n, void *stream);
+                    const float *weights, const int32_t *idx, int nq, float *out, int n,
+                    void *stream);
+int trh_psfex_terms(trh_handle *h, int n, const int64_t *order, const double *src, int degree,
+                    double x0, double y0, double xscale, double yscale, float *weights,
+                    int32_t *idx, void *stream);
 int trh_fp_keys(trh_handle *h, int n, const double *src, int img_h, int img_w, int tile,
                 int32_t *key, void *stream);
 int trh_fp_plan(trh_handle *h, int n, const int64_t *order, const double *src, const int32_t *key,

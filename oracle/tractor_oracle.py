@@ -21,7 +21,7 @@ __all__ = [
     'lanczos_taps3', 'correlate7f', 'lanczos_shift_image',
     'render_galaxy_fft', 'galaxy_patch_halfsize', 'mog_grid', 'mog_masked',
     'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch', 'forced_unit_patch', 'forced_phot_lsq',
-    'forced_fitstats',
+    'forced_fitstats', 'psfex_terms', 'psfex_at',
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
@@ -1112,3 +1112,25 @@ def forced_fitstats(patches, counts, model, data, inverr, scale=1., sky=0.):
     fs['imchisq'] = float(np.sum(chi ** 2))
     fs['imnpix'] = int(np.sum(inverr > 0))
     return fs
+
+
+def psfex_terms(x, y, degree, x0, y0, xscale, yscale):
+    """PsfEx.polynomials (psfex.py:294-325): the term dx^j dy^k, dx = (x - x0)
+    / xscale, dy = (y - y0) / yscale, sits at index
+    j + (degree + 1) k - k (k - 1) / 2 for every j + k <= degree."""
+    dx = (x - x0) / xscale
+    dy = (y - y0) / yscale
+    nb = (degree + 1) * (degree + 2) // 2
+    terms = np.zeros(nb)
+    for d in range(degree + 1):
+        for j in range(d + 1):
+            k = d - j
+            terms[j + (degree + 1) * k - (k * (k - 1)) // 2] = dx ** j * dy ** k
+    return terms
+
+
+def psfex_at(bases, x, y, degree, x0, y0, xscale, yscale):
+    """PsfEx.at (psfex.py:326-346): sum of the eigen-PSFs weighted by the
+    polynomial terms at (x, y), FP64."""
+    t = psfex_terms(x, y, degree, x0, y0, xscale, yscale)
+    return np.tensordot(t, np.asarray(bases, np.float64), axes=(0, 0))

@@ -241,3 +241,37 @@ def test_oracle_fitstats_scaling_and_sign():
     c = O.forced_fitstats(patches, -counts, -model, -data, ie)
     for key in ('prochi2', 'pronpix', 'promasked', 'profracflux', 'proflux', 'fracin'):
         np.testing.assert_allclose(c[key], a[key], rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_psfex_terms_order_and_values():
+    """PSFEx's term order (x powers fastest within each y power; psfex.py:
+    304-318) written out for degrees 0..3, at a point with distinct dx, dy."""
+    x0, y0, xs, ys = 100., 50., 40., 20.
+    x, y = 120., 56.
+    dx, dy = (x - x0) / xs, (y - y0) / ys      # 0.5, 0.3
+    want = {0: [1.],
+            1: [1., dx, dy],
+            2: [1., dx, dx * dx, dy, dx * dy, dy * dy],
+            3: [1., dx, dx ** 2, dx ** 3, dy, dx * dy, dx ** 2 * dy, dy ** 2, dx * dy ** 2,
+                dy ** 3]}
+    for deg, w in want.items():
+        np.testing.assert_allclose(O.psfex_terms(x, y, deg, x0, y0, xs, ys), w, rtol=1e-15)
+
+
+def test_oracle_psfex_at_reproduces_a_polynomial_field():
+    """Bases chosen so that pixel (i, j) of sum_t terms_t basis_t is a known
+    polynomial of the position: the model evaluated at several positions
+    reproduces it exactly (each basis image a unit impulse per term)."""
+    deg = 2
+    nb = 6
+    bases = np.zeros((nb, 3, 3))
+    for t in range(nb):
+        bases[t].flat[t] = 1.
+    rng = np.random.default_rng(2)
+    for _ in range(5):
+        x, y = rng.uniform(0, 200, 2)
+        img = O.psfex_at(bases, x, y, deg, 100., 100., 50., 50.)
+        dx, dy = (x - 100.) / 50., (y - 100.) / 50.
+        np.testing.assert_allclose(img.flat[:nb], [1., dx, dx * dx, dy, dx * dy, dy * dy],
+                                   rtol=1e-14, atol=1e-15)
+        assert np.all(img.flat[nb:] == 0.)

@@ -264,3 +264,52 @@ def test_device_pcg_zero_rhs_and_iteration_cap():
     b = rng.normal(size=60)
     x, status, _ = _device_pcg(A, row_ptr, col, vals, ts, bo, b, 0.0, [2])
     assert status[0] == 0 and status[1] == 2
+
+
+def test_forced_photometry_psfex_matches_oracle():
+    """Forced photometry with a PSFEx-style spatially varying PSF (degree-2
+    polynomial over six eigen-PSFs; PixelizedPsfEx, psfex.py:294-346,
+    481-520): the device's polynomial terms against PsfEx.polynomials, and
+    the fluxes against the oracle rendering each source with psfex_at."""
+    import torch
+    import oracle as O
+    from tractor_amd import forced as F, workloads as W
+    size, n = 256, 120
+    rng = np.random.default_rng(21)
+    kind, pos, flux, shape = W.draw_forced_scene(n, size, rng=rng)
+    grid = W.psf_grid_stamps().astype(np.float64)          # 9 stamps, 33 x 33
+    mean = grid.mean(0)
+    bases = np.stack([mean, 0.5 * (grid[5] - grid[3]), 0.1 * (grid[4] - mean),
+                      0.5 * (grid[7] - grid[1]), 0.05 * (grid[8] - grid[0]),
+                      0.1 * (grid[6] - mean)]).astype(np.float32)
+    pe = dict(degree=2, x0=size / 2., y0=size / 2., xscale=size / 2., yscale=size / 2.)
+    print('oracle patches', flush=True)
+    patches = []
+    for s in range(n):
+        stamp = O.psfex_at(bases, pos[s, 0], pos[s, 1], 2, pe['x0'], pe['y0'], pe['xscale'],
+                           pe['yscale'])
+        patches.append(O.forced_unit_patch(int(kind[s]), pos[s, 0], pos[s, 1], shape[s],
+                                           stamp.astype(np.float32), (size, size)))
+    model = np.zeros((size, size))
+    for pt, f in zip(patches, flux):
+        if pt is not None:
+            x0, y0, p = pt
+            model[y0:y0 + p.shape[0], x0:x0 + p.shape[1]] += f * p
+    data = (model + rng.normal(size=model.shape)).astype(np.float32)
+    inverr = np.ones((size, size), np.float32)
+    ref, iv_ref = O.forced_phot_lsq(patches, data.astype(np.float64), inverr.astype(np.float64))
+    print('device forced photometry (PSFEx)', flush=True)
+    fp = F.BatchForcedPhotometry(torch.from_numpy(data).cuda(), torch.from_numpy(inverr).cuda(),
+                                 bases, None, None, kind, pos, shape, psfex=pe)
+    got = fp.run(tol=1e-13)
+    terms = np.empty((n, 6))
+    terms[fp.order] = fp.d_pw.cpu().numpy()
+    want = np.stack([O.psfex_terms(pos[s, 0], pos[s, 1], 2, pe['x0'], pe['y0'], pe['xscale'],
+                                   pe['yscale']) for s in range(n)])
+    np.testing.assert_allclose(terms, want, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(fp.iv, iv_ref, rtol=1e-4, atol=1e-12)
+    sig = 1. / np.sqrt(np.maximum(iv_ref, 1e-300))
+    err = np.abs(got - ref)
+    ok = (err <= 1e-4 * np.abs(ref)) | (err <= 0.01 * sig)
+    print('flux parity: %d/%d; max err/sigma %.3g' % (ok.sum(), n, np.max(err / sig)), flush=True)
+    assert ok.all()

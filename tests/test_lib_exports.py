@@ -64,7 +64,8 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_normal_eq_dense': (N, N, 10, 3, N, N),
         'trh_solve_normal_f64': (N, N, 3, 0., 0, N, N, N),
         'trh_set_fft_config': (N, 64, 0, 0),
-        'trh_psf_combine': (N, N, 9, 100, N, N, N, 1, N),
+        'trh_psf_combine': (N, N, 9, 100, N, N, 4, N, 1, N),
+        'trh_psfex_terms': (N, 1, N, N, 2, 0., 0., 1., 1., N, N, N),
         'trh_fp_keys': (N, 1, N, 10, 10, 4, N, N),
         'trh_fp_plan': (N, 1, N, N, N, 10, 10, 3, 3, N, N, N, N, N, N, N, N),
         'trh_fp_jobs_bucketed': (N, 1, N, N, N, N, N, N, N),

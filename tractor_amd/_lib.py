@@ -64,7 +64,8 @@ SIGNATURES = {
     'trh_render_mog_approx3_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     'trh_render_point_batch': (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
-    'trh_psf_combine': (_I, [_P, _P, _I, ctypes.c_int64, _P, _P, _P, _I, _P]),
+    'trh_psf_combine': (_I, [_P, _P, _I, ctypes.c_int64, _P, _P, _I, _P, _I, _P]),
+    'trh_psfex_terms': (_I, [_P, _I, _P, _P, _I, _D, _D, _D, _D, _P, _P, _P]),
     'trh_fp_keys': (_I, [_P, _I, _P, _I, _I, _I, _P, _P]),
     'trh_fp_plan': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_fp_jobs_bucketed': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),

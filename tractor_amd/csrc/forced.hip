@@ -22,25 +22,61 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
-// out[s][e] = sum_q w[s][q] * basis[idx[s][q]][e], q < 4 (bilinear PSF
-// interpolation; the same linear map applies to stamps and to spectra).
+// out[s][e] = sum_{q < nq} w[s][q] * basis[idx[s][q]][e] (nq <= 16): a
+// spatially varying PSF as a weighted sum of basis images -- bilinear
+// interpolation of a PSF grid (nq = 4) or PSFEx's polynomial over its
+// eigen-PSFs (psfex.py:326-346).  The same linear map applies to stamps and
+// to their spectra (PixelizedPsfEx.getFourierTransform, psfex.py:481-520).
+constexpr int PSF_MAXQ = 16;
 __global__ void k_psf_combine(const float *__restrict__ basis, int nbasis, int64_t len,
-                              const float *__restrict__ w, const int32_t *__restrict__ idx,
+                              const float *__restrict__ w, const int32_t *__restrict__ idx, int nq,
                               float *__restrict__ out, int n) {
     const int s = blockIdx.y;
     if (s >= n) return;
-    float ws[4];
-    const float *b[4];
-    for (int q = 0; q < 4; ++q) {
-        int k = idx[4 * s + q];
+    float ws[PSF_MAXQ];
+    const float *b[PSF_MAXQ];
+#pragma unroll
+    for (int q = 0; q < PSF_MAXQ; ++q) {
+        int k = q < nq ? idx[nq * s + q] : 0;
         k = k < 0 ? 0 : (k >= nbasis ? nbasis - 1 : k);
-        ws[q] = w[4 * s + q];
+        ws[q] = q < nq ? w[nq * s + q] : 0.f;
         b[q] = basis + (int64_t)k * len;
     }
     float *o = out + (int64_t)s * len;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < len;
-         e += (int64_t)gridDim.x * blockDim.x)
-        o[e] = ws[0] * b[0][e] + ws[1] * b[1][e] + ws[2] * b[2][e] + ws[3] * b[3][e];
+         e += (int64_t)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < PSF_MAXQ; ++q)
+            if (q < nq) acc += ws[q] * b[q][e];
+        o[e] = acc;
+    }
+}
+
+// PSFEx polynomial terms of source order[q] at its position (src rows x, y,
+// ...; stride 6): dx = (x - x0) / xscale, dy = (y - y0) / yscale, term
+// dx^j dy^k at index j + (degree + 1) k - k (k - 1) / 2 (PsfEx.polynomials,
+// psfex.py:294-325), written with the identity basis index for each term.
+__global__ void k_psfex_terms(int n, const int64_t *__restrict__ order,
+                              const double *__restrict__ src, int degree, double x0, double y0,
+                              double xscale, double yscale, float *__restrict__ w,
+                              int32_t *__restrict__ idx) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t s = order ? order[q] : q;
+    const double dx = (src[6 * s] - x0) / xscale, dy = (src[6 * s + 1] - y0) / yscale;
+    const int nq = (degree + 1) * (degree + 2) / 2;
+    double yk = 1.0;
+    for (int k = 0; k <= degree; ++k) {
+        double xj = 1.0;
+        for (int j = 0; j + k <= degree; ++j) {
+            const int ii = j + (degree + 1) * k - (k * (k - 1)) / 2;
+            w[(int64_t)nq * q + ii] = (float)(xj * yk);
+            idx[(int64_t)nq * q + ii] = ii;
+            xj *= dx;
+        }
+        yk *= dy;
+    }
 }
 
 // One wavefront per overlapping pair (s, t): A_st over the intersection of
@@ -863,16 +899,30 @@ k_fp_imstats(int64_t npix, const float *__restrict__ model, const float *__restr
 }  // namespace
 
 extern "C" int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, int64_t len,
-                               const float *weights, const int32_t *idx, float *out, int n,
+                               const float *weights, const int32_t *idx, int nq, float *out, int n,
                                void *stream) {
-    if (!h || !basis || !weights || !idx || !out || nbasis <= 0 || len <= 0 || n < 0 || n > 65535)
-        return set_err(TRH_E_ARG, "trh_psf_combine: bad arguments (n <= 65535)");
+    if (!h || !basis || !weights || !idx || !out || nbasis <= 0 || len <= 0 || n < 0 ||
+        n > 65535 || nq < 1 || nq > PSF_MAXQ)
+        return set_err(TRH_E_ARG, "trh_psf_combine: bad arguments (n <= 65535, 1 <= nq <= %d)",
+                       PSF_MAXQ);
     if (n == 0) return TRH_OK;
     long gx = (len + 255) / 256;
     if (gx > 64) gx = 64;
     hipLaunchKernelGGL(k_psf_combine, dim3((int)gx, n), dim3(256), 0, (hipStream_t)stream, basis,
-                       nbasis, len, weights, idx, out, n);
+                       nbasis, len, weights, idx, nq, out, n);
     return hip_check(hipGetLastError(), "k_psf_combine");
+}
+
+extern "C" int trh_psfex_terms(trh_handle *h, int n, const int64_t *order, const double *src,
+                               int degree, double x0, double y0, double xscale, double yscale,
+                               float *weights, int32_t *idx, void *stream) {
+    if (!h || n < 0 || (n > 0 && (!src || !weights || !idx)) || degree < 0 ||
+        (degree + 1) * (degree + 2) / 2 > PSF_MAXQ || !(xscale != 0.0) || !(yscale != 0.0))
+        return set_err(TRH_E_ARG, "trh_psfex_terms: bad arguments (degree %d)", degree);
+    if (n == 0) return TRH_OK;
+    hipLaunchKernelGGL(k_psfex_terms, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                       order, src, degree, x0, y0, xscale, yscale, weights, idx);
+    return hip_check(hipGetLastError(), "k_psfex_terms");
 }
 
 extern "C" int trh_fp_overlaps(trh_handle *h, int n, const int32_t *win, const int32_t *stile,

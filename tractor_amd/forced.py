@@ -6,9 +6,11 @@ tractor/optimize.py:16-145, with LsqrOptimizer's unscaled least-squares step,
 tractor/lsqr_optimizer.py:11-251) for thousands of overlapping sources with
 a spatially varying pixelized PSF:
 
-1. per-source PSFs by bilinear interpolation of a PSF grid, applied to the
-   stamps (point sources) and, since the Fourier transform is linear, to the
-   grid's spectra (galaxies) -- trh_psf_combine;
+1. per-source PSFs as weighted sums of basis images -- bilinear
+   interpolation of a PSF grid, or PSFEx's polynomial over its eigen-PSFs
+   (PixelizedPsfEx, psfex.py:294-346, 481-520; trh_psfex_terms) -- applied
+   to the stamps (point sources) and, since the Fourier transform is linear,
+   to the bases' spectra (galaxies) -- trh_psf_combine;
 2. unit-flux patches with the reference's no-modelMask extents
    (HoggGalaxy._getUnitFluxPatchSize, galaxy.py:678-691 and the FFT path,
    galaxy.py:336-508; PixelizedPSF.getPointSourcePatch, psf.py:137-207),
@@ -215,7 +217,10 @@ class FitStats(object):
 class BatchForcedPhotometry(object):
     """Forced photometry of n sources on one image (device tensors data,
     inverr of shape (H, W), float32).  psf_basis: (G, ph, pw) stamps of the
-    PSF grid; psf_idx / psf_w: (n, 4) bilinear weights of each source.
+    PSF grid; psf_idx / psf_w: (n, q) basis indices and weights of each
+    source (bilinear: q = 4).  Or psfex = dict(degree, x0, y0, xscale,
+    yscale): psf_basis holds PSFEx's eigen-PSFs and each source's weights are
+    its polynomial terms at its position (psf_idx, psf_w unused).
     kind: 0 point, 1 exp, 2 dev; pos: (n, 2) pixels; shape: (n, 3)
     EllipseESoft (log r_e, ee1, ee2) of the galaxies.  Sources are processed
     in tile order (tile x tile pixel cells; by default the tile holds about
@@ -227,7 +232,7 @@ class BatchForcedPhotometry(object):
     on first use."""
 
     def __init__(self, data, inverr, psf_basis, psf_idx, psf_w, kind, pos, shape, tile=None,
-                 max_block=64, device=0):
+                 max_block=64, device=0, psfex=None):
         self.lib = _lib.load()
         self.h = _lib.handle(device)
         self.dev = torch.device('cuda', device)
@@ -253,8 +258,15 @@ class BatchForcedPhotometry(object):
         src[:, 2] = kind
         src[:, 3:6] = shape
         self._src = src
-        self._pidx = np.ascontiguousarray(psf_idx, np.int32)
-        self._pw = np.ascontiguousarray(psf_w, np.float32)
+        self.psfex = psfex
+        if psfex is None:
+            self._pidx = np.ascontiguousarray(psf_idx, np.int32).reshape(n, -1)
+            self._pw = np.ascontiguousarray(psf_w, np.float32).reshape(n, -1)
+            self.nq = self._pw.shape[1]
+        else:
+            d = int(psfex['degree'])
+            self.nq = (d + 1) * (d + 2) // 2
+            assert self.nq == G, 'PSFEx degree %d needs %d bases, got %d' % (d, self.nq, G)
         self.stream = torch.cuda.current_stream(self.dev).cuda_stream
         self.timing = {}
         self._host = {}
@@ -320,8 +332,17 @@ class BatchForcedPhotometry(object):
         self.d_tend = (cfirst + ccount).to(i32)
         self.d_order, self.d_win, self.d_stile, self.d_bucket = order, win, stile, bucket
         self.d_border, self.d_jobs = border, jobs_b
-        self.d_pidx = self._t(self._pidx)[order]
-        self.d_pw = self._t(self._pw)[order]
+        if self.psfex is None:
+            self.d_pidx = self._t(self._pidx)[order]
+            self.d_pw = self._t(self._pw)[order]
+        else:
+            pe = self.psfex
+            self.d_pw = torch.empty((n, self.nq), dtype=torch.float32, device=dev)
+            self.d_pidx = torch.empty((n, self.nq), dtype=i32, device=dev)
+            check(lib.trh_psfex_terms(h, n, ptr(order), ptr(d_src), int(pe['degree']),
+                                      float(pe['x0']), float(pe['y0']), float(pe['xscale']),
+                                      float(pe['yscale']), ptr(self.d_pw), ptr(self.d_pidx), st),
+                  'psfex_terms')
         self._host = {}
         self.pattern()
         self.timing['plan_s'] = time.perf_counter() - t0
@@ -386,8 +407,9 @@ class BatchForcedPhotometry(object):
             for c0 in range(0, c, 65535):
                 c1 = min(c0 + 65535, c)
                 check(lib.trh_psf_combine(h, ptr(src), len(self.basis), length,
-                                          ptr(wq) + 16 * c0, ptr(iq) + 16 * c0,
-                                          ptr(out) + 4 * c0 * length, c1 - c0, st), 'psf_combine')
+                                          ptr(wq) + 4 * self.nq * c0, ptr(iq) + 4 * self.nq * c0,
+                                          self.nq, ptr(out) + 4 * c0 * length, c1 - c0, st),
+                      'psf_combine')
         # point sources: interpolated stamps, then one batched Lanczos render
         b0, c = int(self.bstart[B_POINT]), int(self.bcount[B_POINT])
         if c:
