@@ -448,6 +448,15 @@ def main_c3(args, torch):
         torch.cuda.synchronize()
         psfex_ms = (time.perf_counter() - tp) * 1e3
     del fpe
+    # the Gaussian-mixture approximations of the nine grid PSFs
+    # (HybridPixelizedPSF(gauss=None): EM, N = 2, 1000 steps), one batch
+    from tractor_amd import psf as TP
+    TP.fit_mog_to_stamps(list(basis), N=2, emsteps=1000)
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    TP.fit_mog_to_stamps(list(basis), N=2, emsteps=1000)
+    torch.cuda.synchronize()
+    em_ms = (time.perf_counter() - te) * 1e3
     achieved = nbytes / (ms / 1e3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
@@ -489,6 +498,7 @@ def main_c3(args, torch):
                    'flux_pull_rms_vs_truth': float(np.sqrt(np.mean(ferr ** 2))),
                    'fitstats_ms': round(fitstats_ms, 3),
                    'psfex_degree2_step_ms': round(psfex_ms, 3),
+                   'psf_em_fit_9_stamps_K2_1000_steps_ms': round(em_ms, 3),
                    'fitstats_median_prochi2_over_pronpix': float(np.median(
                        fsx.prochi2[fsx.pronpix > 0] / fsx.pronpix[fsx.pronpix > 0]))},
     }

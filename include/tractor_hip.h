@@ -374,6 +374,22 @@ int trh_fp_add_patches(trh_handle *h, int n, const int32_t *win, const int64_t *
                        void *stream);
 
 /*
+ * EM fit of a K-component (K <= 8) mixture of Gaussians to each of n PSF
+ * stamps (float64, ny x nx, pixel (ix, iy) at (x0 + ix, y0 + iy)): the
+ * Gaussian approximation of HybridPixelizedPSF(gauss=None) and
+ * GaussianMixturePSF.fromStamp (tractor/psf.py:537-597, 600-625), with
+ * em_fit_2d_reg (tractor/emfit.i:274): `steps` EM steps, terms with
+ * Mahalanobis distance^2 >= 100 dropped, inverse-Wishart prior alpha I on
+ * the variances, amp = the summed responsibilities.  amp [n][K], mean
+ * [n][K][2] and var [n][K][2][2] (FP64) hold the start and receive the fit;
+ * status[s] = -1 if a variance lost positive definiteness (the fit stops
+ * there), else 0.  One stamp's nx ny doubles must fit in LDS.
+ */
+int trh_em_fit_psf(trh_handle *h, int n, const double *stamps, int ny, int nx, int x0, int y0,
+                   int K, double alpha, int steps, double *amp, double *mean, double *var,
+                   int32_t *status, void *stream);
+
+/*
  * Surrogate CNN (no reference counterpart: the learned initialiser that
  * proposes a source's type and starting parameters for the batched fit).
  * Activations are NHWC bfloat16 stored as uint16; convolutions are 3x3,

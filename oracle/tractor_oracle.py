@@ -21,7 +21,7 @@ __all__ = [
     'lanczos_taps3', 'correlate7f', 'lanczos_shift_image',
     'render_galaxy_fft', 'galaxy_patch_halfsize', 'mog_grid', 'mog_masked',
     'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch', 'forced_unit_patch', 'forced_phot_lsq',
-    'forced_fitstats', 'psfex_terms', 'psfex_at',
+    'forced_fitstats', 'psfex_terms', 'psfex_at', 'em_fit_2d_reg', 'em_init_params',
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
@@ -1134,3 +1134,58 @@ def psfex_at(bases, x, y, degree, x0, y0, xscale, yscale):
     polynomial terms at (x, y), FP64."""
     t = psfex_terms(x, y, degree, x0, y0, xscale, yscale)
     return np.tensordot(t, np.asarray(bases, np.float64), axes=(0, 0))
+
+
+def em_fit_2d_reg(img, x0, y0, amp, mean, var, alpha, steps):
+    """em_fit_2d_reg (emfit.i:274-...), FP64: EM fit of a K-component 2-D
+    mixture of Gaussians to image weights, with an inverse-Wishart prior of
+    strength alpha on the variances (prior I).  Pixel (ix, iy) sits at
+    (x0 + ix, y0 + iy).  E step: z_k = amp_k N(x | mean_k, var_k), terms with
+    Mahalanobis distance^2 >= 100 set to 0, normalised over k (pixels with
+    all terms 0 are dropped); M step: wsum_k = sum img z_k, mean_k = the
+    img z_k weighted centroid, var_k = (alpha I + sum img z_k d d^T) /
+    (wsum_k + alpha) about the new mean, amp_k = wsum_k.  Returns (amp, mean,
+    var) (copies); raises on a non-positive determinant."""
+    img = np.asarray(img, np.float64)
+    amp = np.array(amp, np.float64)
+    mean = np.array(mean, np.float64).reshape(-1, 2)
+    var = np.array(var, np.float64).reshape(-1, 2, 2)
+    K = len(amp)
+    NY, NX = img.shape
+    yy, xx = np.mgrid[0:NY, 0:NX]
+    X = (x0 + xx).ravel().astype(np.float64)
+    Y = (y0 + yy).ravel().astype(np.float64)
+    w = img.ravel()
+    tpd = (2. * np.pi) ** 2
+    for _ in range(steps):
+        Z = np.zeros((K, X.size))
+        for k in range(K):
+            V = var[k]
+            det = V[0, 0] * V[1, 1] - V[0, 1] * V[1, 0]
+            if det <= 0.:
+                raise ValueError('non-positive determinant')
+            iv = np.array([V[1, 1], -V[0, 1], -V[1, 0], V[0, 0]]) / det
+            dx, dy = X - mean[k, 0], Y - mean[k, 1]
+            dsq = iv[0] * dx * dx + iv[1] * dx * dy + iv[2] * dx * dy + iv[3] * dy * dy
+            z = amp[k] / np.sqrt(tpd * det) * np.exp(-0.5 * dsq)
+            Z[k] = np.where(dsq >= 100., 0., z)
+        zs = Z.sum(0)
+        ok = zs != 0
+        Z[:, ok] /= zs[ok]
+        wz = w[None, :] * Z
+        wsum = wz.sum(1)
+        mean = np.stack([(wz * X).sum(1), (wz * Y).sum(1)], 1) / wsum[:, None]
+        for k in range(K):
+            dx, dy = X - mean[k, 0], Y - mean[k, 1]
+            S = np.array([[alpha + np.sum(wz[k] * dx * dx), np.sum(wz[k] * dx * dy)],
+                          [np.sum(wz[k] * dy * dx), alpha + np.sum(wz[k] * dy * dy)]])
+            var[k] = S / (wsum[k] + alpha)
+        amp = wsum.copy()
+    return amp, mean, var
+
+
+def em_init_params(K):
+    """fitpsf.em_init_params(K, None, None, None) (fitpsf.py:72-86): equal
+    weights, zero means, variances I (k + 1)."""
+    return (np.ones(K) / K, np.zeros((K, 2)),
+            np.array([np.eye(2) * (k + 1) for k in range(K)]))

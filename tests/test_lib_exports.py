@@ -80,6 +80,7 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_fp_fitstats': (N, 1, N, N, N, N, N, N, N, 10, 1., 0., N, N),
         'trh_fp_imstats': (N, 1, N, N, N, 0., N, N),
         'trh_fp_add_patches': (N, 1, N, N, N, N, 10, N, N),
+        'trh_em_fit_psf': (N, 1, N, 9, 9, -4, -4, 2, 0., 10, N, N, N, N, N),
         'trh_cnn_conv_fwd': (N, N, N, N, N, 1, 32, 32, 1, N),
         'trh_cnn_conv_bwd_data': (N, N, N, N, N, 1, 32, 32, N),
         'trh_cnn_conv_bwd_weight': (N, N, N, N, N, 1, 32, 32, 1, N),

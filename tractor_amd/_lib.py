@@ -80,6 +80,7 @@ SIGNATURES = {
     'trh_fp_fitstats': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _D, _P, _P]),
     'trh_fp_imstats': (_I, [_P, ctypes.c_int64, _P, _P, _P, _D, _P, _P]),
     'trh_fp_add_patches': (_I, [_P, _I, _P, _P, _P, _P, _I, _P, _P]),
+    'trh_em_fit_psf': (_I, [_P, _I, _P, _I, _I, _I, _I, _I, _D, _I, _P, _P, _P, _P, _P]),
     'trh_cnn_conv_fwd': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),
     'trh_cnn_conv_bwd_data': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
     'trh_cnn_conv_bwd_weight': (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _P]),

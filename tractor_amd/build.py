@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libtractor_hip.so')
-SOURCES = ['tractor_hip.hip', 'fitter.hip', 'mog_approx.hip', 'cnn.hip', 'forced.hip']
+SOURCES = ['tractor_hip.hip', 'fitter.hip', 'mog_approx.hip', 'cnn.hip', 'forced.hip',
+           'psf_fit.hip']
 HEADERS = ['fft_common.h', 'render_fft.h', 'internal.h']
 
 HIPCC_FLAGS = [

@@ -154,6 +154,19 @@ class GaussianMixturePSF(BaseParams):
     def copy(self):
         return GaussianMixturePSF(self.mog.amp.copy(), self.mog.mean.copy(), self.mog.var.copy())
 
+    @staticmethod
+    def fromStamp(stamp, N=3, P0=None, xy0=None, alpha=0., emsteps=1000, v2=False,
+                  approx=1e-30, v3=False):
+        """EM fit of an N-component mixture to a PSF stamp on the device
+        (GaussianMixturePSF.fromStamp, tractor/psf.py:537-597 with
+        em_fit_2d_reg, emfit.i:274; trh_em_fit_psf).  P0 = (w, mu, var) or
+        fitpsf.em_init_params's start; xy0 = the stamp's origin (default
+        minus half its size).  The stamp is normalised to sum 1 and clipped
+        at 0 first."""
+        if v2 or v3:
+            raise NotImplementedError('fromStamp: only the em_fit_2d_reg fit is built')
+        return GaussianMixturePSF(*fit_mog_to_stamps([stamp], N, P0, xy0, alpha, emsteps)[0])
+
     def hashkey(self):
         return ('GaussianMixturePSF', tuple(self.mog.amp), tuple(self.mog.mean.ravel()),
                 tuple(self.mog.var.ravel()))
@@ -307,12 +320,12 @@ class NCircularGaussianPSF(MultiParams):
 
 class HybridPixelizedPSF(HybridPSF):
     """PixelizedPSF + a Gaussian-mixture approximation (tractor/psf.py:600-655).
-    gauss=None would need an EM fit of the stamp (tractor/emfit.i), which is
-    not built: pass the MoG explicitly."""
+    gauss=None fits an N-component mixture to the stamp at (cx, cy) on the
+    device (GaussianMixturePSF.fromStamp)."""
 
     def __init__(self, pix, gauss=None, N=2, cx=0., cy=0.):
         if gauss is None:
-            raise NotImplementedError('HybridPixelizedPSF needs an explicit gauss= MoG PSF')
+            gauss = GaussianMixturePSF.fromStamp(pix.getImage(cx, cy), N=N)
         self.__dict__['pix'] = pix
         self.__dict__['gauss'] = gauss
 
@@ -341,3 +354,42 @@ class HybridPixelizedPSF(HybridPSF):
 
     def __setstate__(self, state):
         self.__dict__['gauss'], self.__dict__['pix'] = state
+
+
+def fit_mog_to_stamps(stamps, N=3, P0=None, xy0=None, alpha=0., emsteps=1000):
+    """Batched EM fit (trh_em_fit_psf) of an N-component mixture of Gaussians
+    to each stamp (all of one shape): the stamp normalised to sum 1 and
+    clipped at 0, start P0 = (w, mu, var) or equal weights, zero means and
+    variances I (k + 1) (fitpsf.em_init_params, fitpsf.py:72-86).  Returns
+    [(amp, mean, var)] per stamp."""
+    import torch
+    from . import _lib
+    from ._lib import check, ptr
+    st = np.stack([np.asarray(s, np.float64) for s in stamps])
+    n, NY, NX = st.shape
+    st = st / st.sum(axis=(1, 2), keepdims=True)
+    st = np.maximum(st, 0.)
+    if P0 is not None:
+        w, mu, var = (np.asarray(a, np.float64) for a in P0)
+    else:
+        w = np.ones(N) / N
+        mu = np.zeros((N, 2))
+        var = np.array([np.eye(2) * (k + 1) for k in range(N)])
+    K = len(w)
+    x0, y0 = (-(NX // 2), -(NY // 2)) if xy0 is None else xy0
+    dev = torch.device('cuda', 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    d_st = t(st)
+    d_amp = t(np.tile(w, (n, 1)))
+    d_mean = t(np.tile(mu.reshape(1, K, 2), (n, 1, 1)))
+    d_var = t(np.tile(var.reshape(1, K, 2, 2), (n, 1, 1, 1)))
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    h = _lib.handle(0)
+    check(lib.trh_em_fit_psf(h.value, n, ptr(d_st), NY, NX, int(x0), int(y0), K, float(alpha),
+                             int(emsteps), ptr(d_amp), ptr(d_mean), ptr(d_var), ptr(status),
+                             None), 'em_fit_psf')
+    if np.any(status.cpu().numpy() != 0):
+        raise ValueError('EM fit: a variance lost positive definiteness')
+    a, m, v = d_amp.cpu().numpy(), d_mean.cpu().numpy(), d_var.cpu().numpy()
+    return [(a[i], m[i], v[i]) for i in range(n)]
