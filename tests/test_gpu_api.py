@@ -94,11 +94,16 @@ def test_galaxy_sequence():
     mod7 = _add(p7)
     assert abs(mod7.sum() - 1.963) < 1e-3
 
-    # test_galaxy.py:361-379 wraps this PSF again with gauss=None, which
-    # EM-fits a MoG to the stamp (tractor/emfit.i): not built here, so the
-    # constructor refuses it instead of guessing a mixture.
-    with pytest.raises(NotImplementedError):
-        T.HybridPixelizedPSF(tim.psf)
+    # test_galaxy.py:361-379: wrap again with gauss=None, which EM-fits a
+    # 2-component mixture to the stamp; the rest of the sequence uses it.
+    tim.psf = T.HybridPixelizedPSF(tim.psf)
+    g = tim.psf.gauss.mog
+    cov = sum(g.amp[k] * (g.var[k] + np.outer(g.mean[k], g.mean[k])) for k in range(2))
+    assert abs(g.amp.sum() - 1.) < 1e-6
+    np.testing.assert_allclose(cov, [[6., -1.], [-1., 6.]], atol=0.1)
+    p8 = gal1.getModelPatch(tim, modelMask=mm)
+    assert (p8.x0, p8.y0, p8.shape) == (25, 25, (50, 50))
+    assert abs(_add(p8).sum() - 1.963) < 5e-2
 
     gal1.pos = T.PixPos(5., 5.)
     mod9 = _add(gal1.getModelPatch(tim))
@@ -374,3 +379,55 @@ def test_forced_phot_fitstats_and_variance():
     # the overlapping pair sees each other's flux, the isolated source none
     assert fs.profracflux[0] > 0.05 and fs.profracflux[1] > 0.05
     assert fs.profracflux[2] < 1e-6
+
+
+@pytest.mark.parametrize('n', [0.5, 2.5, 5.5])
+def test_sersic_patch_matches_oracle(n):
+    """SersicGalaxy (sersic.py:362-500) through the device FFT render against
+    the FP64 oracle render of the same sheared mixture (SersicMixture at n),
+    and its index derivative against the oracle's forward difference."""
+    from oracle import tractor_oracle as O
+    psf0 = T.GaussianMixturePSF(1., 0., 0., 6., 6., -1.)
+    psfpatch = psf0.getPointSourcePatch(24., 24., modelMask=ModelMask(0, 0, 50, 50))
+    pix = psfpatch.patch[:49, :49].astype(np.float32)
+    tim = _tim(T.PixelizedPSF(pix))
+    shape = T.EllipseESoft(np.log(2.2), 0.15, -0.1)
+    gal = T.SersicGalaxy(T.PixPos(49.3, 50.2), T.Flux(100.), shape, T.SersicIndex(n))
+    mm = ModelMask(25, 25, 50, 50)
+    p = gal.getModelPatch(tim, modelMask=mm)
+
+    def oracle_patch(nn):
+        mix = T.SersicMixture.getProfile(nn)
+        var = O.shear_mixture(mix.var, np.eye(2) * 3600.,
+                              O.ellipse_esoft_basis(np.log(2.2), 0.15, -0.1))
+        return O.render_galaxy_fft(100. * mix.amp, var, 49.3, 50.2, pix, mm=(25, 25, 50, 50))
+    x0, y0, ref = oracle_patch(n)
+    assert (p.x0, p.y0) == (x0, y0)
+    np.testing.assert_allclose(p.patch, ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+    gal.freezeAllBut('sersicindex')
+    d = gal.getParamDerivatives(tim, modelMask=mm)
+    assert len(d) == 1
+    _, _, ref1 = oracle_patch(n + 0.01)
+    dref = (ref1 - ref) / 0.01
+    np.testing.assert_allclose(d[0].patch, dref, rtol=0, atol=5e-3 * np.abs(dref).max())
+
+
+def test_optimize_loop_recovers_sersic_index():
+    """optimize_loop on a noiseless Sersic galaxy (n = 2.5) from n = 2.0 and a
+    perturbed shape and flux: every free parameter returns to the truth."""
+    psf = T.GaussianMixturePSF(1., 0., 0., 4., 4., 0.)
+    H = W = 64
+    truth = T.SersicGalaxy(T.PixPos(31.7, 32.4), T.Flux(500.),
+                           T.EllipseESoft(np.log(3.), 0.2, -0.1), T.SersicIndex(2.5))
+    tim = T.Image(data=np.zeros((H, W), np.float32), inverr=np.ones((H, W), np.float32) * 10.,
+                  psf=psf, photocal=T.LinearPhotoCal(1.))
+    tim.data = T.Tractor([tim], [truth]).getModelImage(0).astype(np.float32)
+    gal = T.SersicGalaxy(T.PixPos(31.7, 32.4), T.Flux(420.), T.EllipseESoft(np.log(2.6), 0.1, 0.),
+                         T.SersicIndex(2.0))
+    gal.freezeParam('pos')
+    tr = T.Tractor([tim], [gal], optimizer=T.GpuOptimizer())
+    tr.freezeParam('images')
+    lnp0 = tr.getLogProb()
+    r = tr.optimize_loop()
+    assert r['steps'] < 100 and tr.getLogProb() > lnp0
+    np.testing.assert_allclose(gal.getParams(), truth.getParams()[2:], atol=5e-3, rtol=5e-3)

@@ -1,0 +1,74 @@
+"""SersicMixture (tractor/sersic.py:25-341) and the Sersic parameters, on
+the host.  The table's nodes are reproduced exactly by the interpolation;
+at n = 1 and n = 4 the mixtures agree with the separately tabulated exp and
+dev profiles (mixture_profiles.py:17-38) to their fitting precision."""
+import numpy as np
+import pytest
+
+from tractor_amd import mixture_profiles as mp
+from tractor_amd.sersic import SersicGalaxy, SersicIndex, SersicMixture
+from tractor_amd.sersic_table import CORES, RANGES
+
+
+def _node(n):
+    for lo, hi, nodes in RANGES:
+        for s, a, v in nodes:
+            if s == n and lo <= n < hi:
+                return np.array(a), np.array(v)
+    raise KeyError(n)
+
+
+@pytest.mark.parametrize('n', [0.3, 0.45, 0.8, 2.0, 5.0])
+def test_profile_reproduces_table_nodes(n):
+    a, v = _node(n)
+    a = a / a.sum()
+    if n > 1.:
+        core = dict(CORES).get(n)
+        if core is None:
+            core = float(SersicMixture.getProfile(n).amp[0])   # spline between core nodes
+        a = np.append(a * (1. - core), core)
+        v = np.append(v, 0.)
+    o = np.argsort(v)
+    m = SersicMixture.getProfile(n)
+    np.testing.assert_allclose(m.amp, a[o], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(m.var[:, 0, 0], v[o], rtol=1e-12)
+    np.testing.assert_allclose(m.var[:, 1, 1], v[o], rtol=1e-12)
+    assert np.all(m.var[:, 0, 1] == 0.) and np.all(m.mean == 0.)
+
+
+@pytest.mark.parametrize('n,ref', [(1.0, mp.get_exp_mixture), (4.0, mp.get_dev_mixture)])
+def test_exp_and_dev_special_cases(n, ref):
+    m = SersicMixture.getProfile(n)
+    r = ref()
+    o = np.argsort(r.var[:, 0, 0])
+    assert m.K == r.K
+    np.testing.assert_allclose(m.amp, r.amp[o], rtol=1e-3, atol=2e-6)
+    np.testing.assert_allclose(m.var[:, 0, 0], r.var[o, 0, 0], rtol=2e-3)
+
+
+def test_normalised_sorted_and_clamped():
+    for n in np.linspace(0.1, 7.0, 57):
+        m = SersicMixture.getProfile(n)
+        assert m.amp.sum() == pytest.approx(1., abs=1e-12)
+        assert np.all(np.diff(m.var[:, 0, 0]) >= 0)
+        assert m.K <= 16
+    lo = SersicMixture.getProfile(0.1)
+    np.testing.assert_array_equal(lo.amp, SersicMixture.getProfile(0.29).amp)
+    hi = SersicMixture.getProfile(9.0)
+    np.testing.assert_array_equal(hi.amp, SersicMixture.getProfile(6.3).amp)
+
+
+def test_sersic_index_and_galaxy_params():
+    import tractor_amd as T
+    si = SersicIndex(2.5)
+    assert si.getStepSizes() == [0.01] and si.getMaxStep() == [0.1]
+    assert si.getLowerBounds() == [0.29] and si.getUpperBounds() == [6.3]
+    assert si.getLogPrior() == 0.
+    si.setValue(7.)
+    assert si.getLogPrior() == -np.inf
+    g = SersicGalaxy(T.PixPos(10., 12.), T.Flux(5.), T.EllipseESoft(0.5, 0.1, -0.2),
+                     SersicIndex(2.5))
+    assert g.numberOfParams() == 7
+    assert g.getParamNames()[-1] == 'sersicindex.SersicIndex'
+    c = g.copy()
+    assert c.getParams() == g.getParams() and c.getName() == 'SersicGalaxy'

@@ -18,6 +18,7 @@ from .psf import (PixelizedPSF, GaussianMixturePSF, NCircularGaussianPSF, Hybrid
 from .pointsource import PointSource, SingleProfileSource, BasicSource
 from .galaxy import (GalaxyShape, Galaxy, ProfileGalaxy, HoggGalaxy, GaussianGalaxy, ExpGalaxy,
                      DevGalaxy)
+from .sersic import SersicGalaxy, SersicIndex, SersicMixture
 from .mixture_profiles import MixtureOfGaussians, get_exp_mixture, get_dev_mixture
 from .engine import Tractor, Images, Catalog, OptResult
 from .optimize import Optimizer

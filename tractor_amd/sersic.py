@@ -1,0 +1,155 @@
+"""Sersic galaxies (tractor/sersic.py:25-500): the Sersic profile as a
+mixture of Gaussians interpolated in the Sersic index, a Sersic-index
+parameter, and the galaxy whose unit-flux patches go through the same device
+renders as the exp and dev galaxies (FFT path, or the pixel-space MoG path
+for the wide components of a hybrid PSF)."""
+import functools
+
+import numpy as np
+
+from . import mixture_profiles as mp
+from .galaxy import HoggGalaxy
+from .sersic_table import CORES, RANGES
+from .utils import ScalarParam
+
+
+class SersicMixture(object):
+    """SersicMixture (sersic.py:25-341).  Within each tabulated range of the
+    index, every component's amplitude and log variance is a spline through
+    the range's nodes (cubic; linear for three nodes or fewer).  Where two
+    ranges overlap their mixtures are concatenated with a linear ramp of
+    weights; amplitudes are normalised to 1; above n = 1 a zero-variance
+    core carries the central flux the softened profile leaves out (CORES,
+    cubic spline); components come sorted by variance.  Indices outside the
+    table clamp to its ends."""
+    _instance = None
+
+    @staticmethod
+    def getProfile(sindex):
+        if SersicMixture._instance is None:
+            SersicMixture._instance = SersicMixture()
+        return SersicMixture._instance._profile(float(sindex))
+
+    def __init__(self):
+        from scipy.interpolate import InterpolatedUnivariateSpline as Spline
+        self.core = Spline([c[0] for c in CORES], [c[1] for c in CORES], k=3)
+        self.ranges = []
+        for lo, hi, nodes in RANGES:
+            s = [nd[0] for nd in nodes]
+            k = 3 if len(nodes) > 3 else 1
+            K = len(nodes[0][1])
+            amp = [Spline(s, [nd[1][i] for nd in nodes], k=k) for i in range(K)]
+            lvar = [Spline(s, [np.log(nd[2][i]) for nd in nodes], k=k) for i in range(K)]
+            self.ranges.append((lo, hi, amp, lvar))
+        self.lowest = self.ranges[0][0]
+        self.highest = self.ranges[-1][1]
+
+    @staticmethod
+    def _eval(funcs, n):
+        return np.array([float(f(n)) for f in funcs])
+
+    @functools.lru_cache(maxsize=16)
+    def _profile(self, n):
+        if n <= self.lowest:
+            found, n = [self.ranges[0]], self.lowest
+        elif n >= self.highest:
+            found, n = [self.ranges[-1]], self.highest
+        else:
+            found = [r for r in self.ranges if r[0] <= n < r[1]]
+        if len(found) == 2:
+            (lo0, hi0, a0, v0), (lo1, hi1, a1, v1) = found
+            frac = (n - lo1) / (hi0 - lo1)
+            amp0 = self._eval(a0, n)
+            amp1 = self._eval(a1, n)
+            amp = np.append((1. - frac) * amp0 / amp0.sum(), frac * amp1 / amp1.sum())
+            var = np.exp(self._eval(v0 + v1, n))
+        else:
+            _, _, a, v = found[0]
+            amp = self._eval(a, n)
+            amp = amp / amp.sum()
+            var = np.exp(self._eval(v, n))
+        if n > 1.:
+            core = float(self.core(n))
+            amp = np.append(amp * (1. - core) / amp.sum(), core)
+            var = np.append(var, 0.)
+        order = np.argsort(var)
+        return mp.MixtureOfGaussians(amp[order], np.zeros((len(amp), 2)), var[order])
+
+
+class SersicIndex(ScalarParam):
+    """SersicIndex (sersic.py:344-359): step 0.01, bounds [0.29, 6.3], at
+    most 0.1 per step; zero prior inside the bounds, -inf outside."""
+    stepsize = 0.01
+
+    def __init__(self, val=0):
+        super(SersicIndex, self).__init__(val=val)
+        self.lower = 0.29
+        self.upper = 6.3
+        self.maxstep = 0.1
+
+    def isLegal(self):
+        return self.lower <= self.val <= self.upper
+
+    def getLogPrior(self):
+        return 0. if self.isLegal() else -np.inf
+
+    def copy(self):
+        return SersicIndex(self.val)
+
+
+class SersicGalaxy(HoggGalaxy):
+    """SersicGalaxy (sersic.py:362-500): position, brightness, shape and
+    Sersic index; half-size 8 r_e like the dev profile; the index derivative
+    by a forward difference of the unit-flux render (step 0.01, backwards
+    when the step would pass the upper bound)."""
+    nre = 8.
+
+    @staticmethod
+    def getNamedParams():
+        return dict(pos=0, brightness=1, shape=2, sersicindex=3)
+
+    def __init__(self, pos, brightness, shape, sersicindex, **kwargs):
+        self.nre = SersicGalaxy.nre
+        super(SersicGalaxy, self).__init__(pos, brightness, shape, sersicindex)
+
+    def getName(self):
+        return 'SersicGalaxy'
+
+    def __str__(self):
+        return super(SersicGalaxy, self).__str__() + ', Sersic index %.3f' % self.sersicindex.val
+
+    def getProfile(self):
+        return SersicMixture.getProfile(self.sersicindex.val)
+
+    def copy(self):
+        c = SersicGalaxy(self.pos.copy(), self.brightness.copy(), self.shape.copy(),
+                         self.sersicindex.copy())
+        c.liquid = list(self.liquid)
+        return c
+
+    def getParamDerivatives(self, img, modelMask=None, **kwargs):
+        derivs = super(SersicGalaxy, self).getParamDerivatives(img, modelMask=modelMask,
+                                                               get_patch0=True, **kwargs)
+        patch0 = derivs.pop(0)
+        if patch0 is None or self.isParamFrozen('sersicindex'):
+            return derivs
+        if modelMask is None:
+            from .patch import ModelMask
+            x0, x1, y0, y1 = patch0.getExtent()
+            modelMask = ModelMask.fromExtent(x0, x1, y0, y1)
+        px0, py0 = img.getWcs().positionToPixel(self.getPosition(), self)
+        counts = img.getPhotoCal().brightnessToCounts(self.brightness)
+        step = self.sersicindex.getStepSizes()[0]
+        n0 = self.sersicindex.getValue()
+        if n0 + step > self.sersicindex.upper:
+            step = -step
+        self.sersicindex.setValue(n0 + step)
+        patch = self.getUnitFluxModelPatch(img, px=px0, py=py0, modelMask=modelMask, **kwargs)
+        self.sersicindex.setValue(n0)
+        if patch is None or counts == 0:
+            derivs.append(None)
+            return derivs
+        d = (patch - patch0) * (counts / step)
+        d.setName('d(%s)/d(sersicindex)' % self.dname)
+        derivs.append(d)
+        return derivs
