@@ -446,7 +446,8 @@ typedef struct trh_fitter trh_fitter;
 
 /*
  * Create a fitter for S sources.  Host arrays:
- *   kind[S]      0 PointSource, 1 ExpGalaxy, 2 DevGalaxy
+ *   kind[S]      0 PointSource, 1 ExpGalaxy, 2 DevGalaxy, 3 SersicGalaxy
+ *                (tractor/sersic.py:362-487; needs trh_fitter_set_sersic)
  *   P[S]         square modelMask size (patch is (0,0,P,P) in patch pixels)
  *   psf_stamp    float32 [ph][pw] PixelizedPSF stamp shared by all sources
  *   cdi[4]       position-derivative matrix (ducks.py:413-423): row-major 2x2
@@ -459,10 +460,27 @@ int trh_fitter_create(trh_handle *h, trh_fitter **f, int S, const int32_t *kind,
                       const double *cdi);
 int trh_fitter_destroy(trh_fitter *f);
 
-/* Copy parameters in/out: host double [S][6] = (x, y, flux, logre, ee1, ee2);
- * point sources use the first 3. */
+/* Copy parameters in/out: host double [S][TRH_FIT_NPARAM] = (x, y, flux,
+ * logre, ee1, ee2, sersic index); point sources use the first 3, exp and dev
+ * galaxies the first 6. */
+#define TRH_FIT_NPARAM 7
 int trh_fitter_set_params(trh_fitter *f, const double *params, void *stream);
 int trh_fitter_get_params(trh_fitter *f, double *params, void *stream);
+
+/*
+ * The Sersic mixture for kind-3 sources (SersicMixture, tractor/sersic.py:
+ * 25-342: amplitudes and log-variances interpolated in the index), as
+ * piecewise cubics in a flat double array of ntab values:
+ *   R; then per index range [lo, hi), in increasing lo:
+ *     lo, hi, K, m, breakpoints b[0..m], then for each component k the m
+ *     amplitude pieces and the m log-variance pieces, each 4 coefficients
+ *     (highest power first) in powers of (n - b[i]), valid on [b[i], b[i+1]);
+ *   then the core fraction: m, b[0..m], m pieces of 4 coefficients.
+ * Indices outside [lo of the first range, hi of the last] clamp to it; a
+ * zero-variance core carries the core fraction above n = 1.  At most 8
+ * ranges.  tractor_amd.sersic.fitter_table() builds it.
+ */
+int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab);
 
 /*
  * One LM iteration over all still-active sources (Jacobian renders,
@@ -503,9 +521,10 @@ int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations, doubl
                       double *dlnp, int32_t *n_active, void *stream);
 
 /* Device pointer to the fitter's Jacobian renders (float32, per source
- * 1 + n_shape planes of P*P at render_off[s]) and normal equations (double
- * [S][32]: 21 upper-triangle entries, 6 gradient entries, chi^2).  For
- * testing. */
+ * 1 + n_shape (+1 Sersic index) planes of P*P at render_off[s]) and normal
+ * equations (double [S][40]: the packed upper triangle of the NC x NC matrix,
+ * NC = 6 or 7 for a Sersic galaxy, then NC gradient entries; chi^2 at 36).
+ * For testing. */
 int trh_fitter_debug(trh_fitter *f, const float **renders, const int64_t **render_off,
                      const double **normal_eq, const double **step_x);
 

@@ -25,7 +25,8 @@ __all__ = [
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
-    'source_chisq', 'fit_source', 'ALPHAS', 'round_half_even',
+    'source_chisq', 'fit_source', 'ALPHAS', 'round_half_even', 'sersic_mixture',
+    'sersic_fd_step',
 ]
 
 # ---------------------------------------------------------------------------
@@ -877,7 +878,8 @@ class SourceProblem(object):
     """A single-source fit on a P x P patch: modelMask = (0, 0, P, P),
     positions in patch pixels, NullPhotoCal counts = flux.
 
-    kind: 0 point source, 1 ExpGalaxy, 2 DevGalaxy.
+    kind: 0 point source, 1 ExpGalaxy, 2 DevGalaxy, 3 SersicGalaxy (params
+          (x, y, flux, logre, ee1, ee2, sersic index)).
     cdi:  2x2 position-derivative matrix (ducks.py:413-423): NullWCS gives
           3600/pixscale * I (the reference's NullWCS factor), a pixel-unit
           WCS gives I.
@@ -893,25 +895,113 @@ class SourceProblem(object):
 
     @property
     def nparams(self):
-        return 3 if self.kind == 0 else 6
+        return 3 if self.kind == 0 else (7 if self.kind == 3 else 6)
 
     def maxstep(self):
         # PixPos maxstep 1 px (wcs.py:384); Flux none; EllipseESoft inherits
-        # EllipseE maxstep [1, 0.1, 0.1] (ellipses.py:48, 190-197)
-        return [1., 1., None] + ([1., 0.1, 0.1] if self.kind else [])
+        # EllipseE maxstep [1, 0.1, 0.1] (ellipses.py:48, 190-197);
+        # SersicIndex 0.1 (sersic.py:344-352)
+        return [1., 1., None] + ([1., 0.1, 0.1] if self.kind else []) + (
+            [0.1] if self.kind == 3 else [])
 
     def bounds(self):
         n = self.nparams
+        if self.kind == 3:
+            # SersicIndex bounds [0.29, 6.3] (sersic.py:348-351)
+            return [None] * 6 + [0.29], [None] * 6 + [6.3]
         return [None] * n, [None] * n
 
 
-def _profile(kind):
+def _profile(kind, sindex=None):
+    if kind == 3:
+        amp, var = sersic_mixture(sindex)
+        return amp, _iso_var(var)
     return exp_mixture() if kind == 1 else dev_mixture()
 
 
-def source_render(prob, params, shape_step=None):
+# ---------------------------------------------------------------------------
+# Sersic profiles (tractor/sersic.py:25-342, 447-487).  The tabulated fits
+# are data, read from tractor_amd/sersic_table.py; the interpolation is
+# restated here with scipy's CubicSpline (not-a-knot), the same interpolant
+# as the reference's InterpolatedUnivariateSpline(k=3) through the nodes.
+# ---------------------------------------------------------------------------
+_SERSIC = None
+
+
+def _sersic_splines():
+    global _SERSIC
+    if _SERSIC is None:
+        from scipy.interpolate import CubicSpline, interp1d
+        from tractor_amd.sersic_table import CORES, RANGES
+
+        def spl(x, y):
+            if len(x) > 3:
+                return CubicSpline(x, y, bc_type='not-a-knot')
+            return interp1d(x, y, fill_value='extrapolate')
+        ranges = []
+        for lo, hi, nodes in RANGES:
+            x = [nd[0] for nd in nodes]
+            K = len(nodes[0][1])
+            amp = [spl(x, [nd[1][k] for nd in nodes]) for k in range(K)]
+            lv = [spl(x, [math.log(nd[2][k]) for nd in nodes]) for k in range(K)]
+            ranges.append((lo, hi, amp, lv))
+        core = spl([c[0] for c in CORES], [c[1] for c in CORES])
+        _SERSIC = (ranges, core)
+    return _SERSIC
+
+
+def sersic_mixture(n):
+    """SersicMixture._getProfile (sersic.py:286-342) -> (amp, var): indices
+    clamped to the table, amplitudes normalised per range (ramped where two
+    ranges overlap), variances exp(log-variance), a zero-variance core above
+    n = 1, sorted by variance."""
+    ranges, core = _sersic_splines()
+    if n <= ranges[0][0]:
+        found, n = [ranges[0]], ranges[0][0]
+    elif n >= ranges[-1][1]:
+        found, n = [ranges[-1]], ranges[-1][1]
+    else:
+        found = [r for r in ranges if r[0] <= n < r[1]]
+    if len(found) == 2:
+        (lo0, hi0, a0, v0), (lo1, hi1, a1, v1) = found
+        f = (n - lo1) / (hi0 - lo1)
+        x0 = np.array([float(a(n)) for a in a0])
+        x1 = np.array([float(a(n)) for a in a1])
+        amp = np.append((1. - f) * x0 / x0.sum(), f * x1 / x1.sum())
+        var = np.exp([float(v(n)) for v in v0 + v1])
+    else:
+        _, _, a, v = found[0]
+        amp = np.array([float(f(n)) for f in a])
+        amp = amp / amp.sum()
+        var = np.exp([float(f(n)) for f in v])
+    if n > 1.:
+        c = float(core(n))
+        amp = np.append(amp * (1. - c) / amp.sum(), c)
+        var = np.append(var, 0.)
+    o = np.argsort(var, kind='stable')
+    return amp[o], var[o]
+
+
+def sersic_fd_step(n0, step=0.01, lower=0.29, upper=6.3):
+    """Signed finite-difference step of the Sersic index for its Jacobian
+    column, as SersicGalaxy.getDerivativeShearedProfiles (sersic.py:447-487)
+    takes it: +step, or -step past the upper bound; where the stepped profile
+    changes the number of components, the opposite step if it stays inside
+    the bounds and keeps the count."""
+    if n0 + step > upper:
+        step = -step
+    K0 = len(sersic_mixture(n0)[0])
+    if len(sersic_mixture(n0 + step)[0]) != K0:
+        n2 = n0 - step
+        if lower < n2 < upper and len(sersic_mixture(n2)[0]) == K0:
+            step = -step
+    return step
+
+
+def source_render(prob, params, shape_step=None, index_step=0.):
     """Unit-flux render of the problem's source at ``params`` on its P x P
-    modelMask (galaxy.py:336-508 / psf.py:137-207)."""
+    modelMask (galaxy.py:336-508 / psf.py:137-207); a Sersic galaxy's index
+    offset by ``index_step``."""
     P = prob.P
     mm = (0, 0, P, P)
     x, y = params[0], params[1]
@@ -921,7 +1011,7 @@ def source_render(prob, params, shape_step=None):
     sp = list(params[3:6])
     if shape_step is not None:
         sp[shape_step] += 0.01
-    amp, var = _profile(prob.kind)
+    amp, var = _profile(prob.kind, params[6] + index_step if prob.kind == 3 else None)
     basis = ellipse_esoft_basis(*sp)
     cdinv = np.eye(2) * 3600.  # NullWCS pixscale 1 (wcs.py:24-28)
     vp = shear_mixture(var, cdinv, basis)
@@ -940,6 +1030,10 @@ def source_jacobian(prob, params):
     p0, d = galaxy_param_derivs(
         lambda k: source_render(prob, params, shape_step=k), counts, prob.cdi,
         [0.01, 0.01, 0.01])
+    if prob.kind == 3:
+        step = sersic_fd_step(params[6])
+        pk = source_render(prob, params, index_step=step)
+        d.append(((pk - p0) * (counts / step)).astype(np.float32))
     return p0, d
 
 

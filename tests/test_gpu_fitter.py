@@ -91,7 +91,7 @@ def test_one_iteration_normal_equations_and_step(problem):
         g = A.T @ b
         assert np.max(np.abs(dbg['neq'][s, 21:21 + n] - g) / (np.sqrt(np.diag(N)) * np.sqrt(b @ b))) < 3e-3
         chi2 = b @ b
-        assert abs(dbg['neq'][s, 27] - chi2) / chi2 < 1e-5   # renders agree to 1e-5 of peak
+        assert abs(dbg['neq'][s, 36] - chi2) / chi2 < 1e-5   # renders agree to 1e-5 of peak
         # update direction: column-scaled least squares
         X = O.column_scaled_lstsq(A, b)
         sig = 1. / np.sqrt(np.diag(N))

@@ -93,6 +93,7 @@ SIGNATURES = {
     'trh_fitter_destroy': (_I, [_P]),
     'trh_fitter_set_params': (_I, [_P, _P, _P]),
     'trh_fitter_get_params': (_I, [_P, _P, _P]),
+    'trh_fitter_set_sersic': (_I, [_P, _P, _I]),
     'trh_fitter_iterate': (_I, [_P, _D, _D, _P]),
     'trh_fitter_iterate_graph': (_I, [_P, _D, _D, _I, _P]),
     'trh_fitter_render_models': (_I, [_P, _P, _P]),

@@ -21,8 +21,9 @@ class BatchFitter(object):
     """S sources, each on its own P x P patch of `data` / `inverr`
     (device float32 tensors, patch s at pix_off[s]).
 
-    kind: 0 PointSource, 1 ExpGalaxy, 2 DevGalaxy;  params (S, 6):
-    x, y, flux, logre, ee1, ee2 (EllipseESoft) in patch pixels.
+    kind: 0 PointSource, 1 ExpGalaxy, 2 DevGalaxy, 3 SersicGalaxy;  params
+    (S, 6): x, y, flux, logre, ee1, ee2 (EllipseESoft) in patch pixels, and a
+    seventh column, the Sersic index, when there are Sersic galaxies.
     cdi: 2x2 position-derivative matrix (tractor/ducks.py:413-423).
     """
     def __init__(self, kind, P, pix_off, data, inverr, psf_stamp, cdi=PIXEL_CDI, device=0):
@@ -49,6 +50,11 @@ class BatchFitter(object):
                                          ptr(psf), psf.shape[0], psf.shape[1], ptr(cdi)),
               'trh_fitter_create')
         self.f = f
+        self.npar = 7 if np.any(self.kind == 3) else 6
+        if self.npar == 7:
+            from .sersic import fitter_table
+            tab = fitter_table()
+            check(self.lib.trh_fitter_set_sersic(f, ptr(tab), len(tab)), 'trh_fitter_set_sersic')
         self.stream = torch.cuda.current_stream().cuda_stream
 
     def close(self):
@@ -63,14 +69,17 @@ class BatchFitter(object):
             pass
 
     def set_params(self, params):
-        p = np.ascontiguousarray(params, np.float64)
-        assert p.shape == (self.S, 6)
-        check(self.lib.trh_fitter_set_params(self.f, ptr(p), self.stream), 'set_params')
+        p = np.asarray(params, np.float64)
+        if p.shape != (self.S, self.npar):
+            raise ValueError('params: shape %s, want (%d, %d)' % (p.shape, self.S, self.npar))
+        p7 = np.zeros((self.S, 7))
+        p7[:, :self.npar] = p
+        check(self.lib.trh_fitter_set_params(self.f, ptr(p7), self.stream), 'set_params')
 
     def get_params(self):
-        p = np.zeros((self.S, 6))
+        p = np.zeros((self.S, 7))
         check(self.lib.trh_fitter_get_params(self.f, ptr(p), self.stream), 'get_params')
-        return p
+        return p[:, :self.npar].copy()
 
     def render_models(self, out):
         check(self.lib.trh_fitter_render_models(self.f, ptr(out), self.stream), 'render_models')
@@ -142,10 +151,10 @@ class BatchFitter(object):
         x = ctypes.c_void_p()
         check(self.lib.trh_fitter_debug(self.f, ctypes.byref(r), ctypes.byref(ro), ctypes.byref(ne),
                                         ctypes.byref(x)), 'debug')
-        nr = np.where(self.kind > 0, 4, 1) * self.P.astype(np.int64) ** 2
+        nr = np.select([self.kind == 3, self.kind > 0], [5, 4], 1) * self.P.astype(np.int64) ** 2
         total = int(nr.sum())
         renders = np.zeros(total, np.float32)
-        neq = np.zeros((self.S, 32))
+        neq = np.zeros((self.S, 40))
         X = np.zeros((self.S, 8))
         roff = np.zeros(self.S, np.int64)
         for dst, src, nb in [(renders, r, renders.nbytes), (neq, ne, neq.nbytes), (X, x, X.nbytes),

@@ -40,11 +40,144 @@ using namespace trh;
 #define SL_PREP 14
 #define TRH_NSLOT 15
 
+// Normal equations of one source: neq[NEQ_W s + .] holds the packed upper
+// triangle of the NC x NC matrix J^T W J (NC = 6; 7 for a Sersic galaxy),
+// then the NC entries of J^T W r, chi^2 at NEQ_CHI2 and the zero-model chi^2
+// of the window (point sources) at NEQ_CHI0.
+#define NEQ_W 40
+#define NEQ_CHI2 36
+#define NEQ_CHI0 37
+
 struct FitProfiles {
     double amp[2][TRH_MAXK];
     double var[2][TRH_MAXK];
     int K[2];
 };
+
+// Sersic index bounds and steps (sersic.py:344-359)
+#define SER_LOWER 0.29
+#define SER_UPPER 6.3
+#define SER_STEP 0.01
+#define SER_MAXSTEP 0.1
+#define SER_MAXR 8
+#define SER_MAXD 8192
+
+// The Sersic mixture as piecewise cubics in the index (SersicMixture,
+// sersic.py:246-342): per index range [lo, hi), K components whose amplitude
+// and log-variance are cubic on each of nint pieces (breakpoints at
+// d[boff ..], coefficients of component k, f = 0 amplitude / 1 log-variance,
+// piece i at d[coff + ((2 k + f) nint + i) 4 ..], highest power first in
+// powers of (n - breakpoint)); the core fraction likewise.
+struct SersicTab {
+    int nr;
+    int K[SER_MAXR], nint[SER_MAXR], boff[SER_MAXR], coff[SER_MAXR];
+    int core_nint, core_boff, core_coff;
+    double lo[SER_MAXR], hi[SER_MAXR];
+    double d[SER_MAXD];
+};
+
+// One piecewise cubic at x: the piece whose breakpoint is the last one <= x
+// (the first piece below the first breakpoint, the last above the last).
+__device__ __forceinline__ double ser_piece(const double *br, const double *c, int nint,
+                                            double x) {
+    int i = 0;
+    while (i + 1 < nint && x >= br[i + 1]) ++i;
+    const double t = x - br[i];
+    const double *q = c + 4 * i;
+    return ((q[0] * t + q[1]) * t + q[2]) * t + q[3];
+}
+
+__device__ int ser_range(const SersicTab &T, int r, double n, double *amp, double *lvar) {
+    const int K = T.K[r], m = T.nint[r];
+    const double *br = T.d + T.boff[r];
+    const double *c = T.d + T.coff[r];
+    for (int k = 0; k < K; ++k) {
+        amp[k] = ser_piece(br, c + (size_t)(2 * k) * m * 4, m, n);
+        lvar[k] = ser_piece(br, c + (size_t)(2 * k + 1) * m * 4, m, n);
+    }
+    return K;
+}
+
+// SersicMixture._getProfile (sersic.py:286-342): amplitudes normalised per
+// range (a linear ramp of weights where two ranges overlap), variances
+// exp(log-variance), a zero-variance core above n = 1, sorted by variance
+// (stable).  amp, var hold TRH_MAXK + 1 entries.  Returns K.
+__device__ int sersic_profile(const SersicTab &T, double n, double *amp, double *var) {
+    int r0 = -1, r1 = -1;
+    if (n <= T.lo[0]) {
+        r0 = 0;
+        n = T.lo[0];
+    } else if (n >= T.hi[T.nr - 1]) {
+        r0 = T.nr - 1;
+        n = T.hi[T.nr - 1];
+    } else {
+        for (int r = 0; r < T.nr; ++r)
+            if (T.lo[r] <= n && n < T.hi[r]) {
+                if (r0 < 0)
+                    r0 = r;
+                else if (r1 < 0)
+                    r1 = r;
+            }
+    }
+    int K;
+    if (r1 >= 0) {
+        const double frac = (n - T.lo[r1]) / (T.hi[r0] - T.lo[r1]);
+        const int K0 = ser_range(T, r0, n, amp, var);
+        const int K1 = ser_range(T, r1, n, amp + K0, var + K0);
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = 0; k < K0; ++k) s0 += amp[k];
+        for (int k = 0; k < K1; ++k) s1 += amp[K0 + k];
+        for (int k = 0; k < K0; ++k) amp[k] = (1.0 - frac) * (amp[k] / s0);
+        for (int k = 0; k < K1; ++k) amp[K0 + k] = frac * (amp[K0 + k] / s1);
+        K = K0 + K1;
+    } else {
+        K = ser_range(T, r0, n, amp, var);
+        double s = 0.0;
+        for (int k = 0; k < K; ++k) s += amp[k];
+        for (int k = 0; k < K; ++k) amp[k] /= s;
+    }
+    for (int k = 0; k < K; ++k) var[k] = exp(var[k]);
+    if (n > 1.0) {
+        const double core = ser_piece(T.d + T.core_boff, T.d + T.core_coff, T.core_nint, n);
+        double s = 0.0;
+        for (int k = 0; k < K; ++k) s += amp[k];
+        const double f = (1.0 - core) / s;
+        for (int k = 0; k < K; ++k) amp[k] *= f;
+        amp[K] = core;
+        var[K] = 0.0;
+        ++K;
+    }
+    for (int i = 1; i < K; ++i) {
+        const double a = amp[i], v = var[i];
+        int j = i - 1;
+        while (j >= 0 && var[j] > v) {
+            amp[j + 1] = amp[j];
+            var[j + 1] = var[j];
+            --j;
+        }
+        amp[j + 1] = a;
+        var[j + 1] = v;
+    }
+    return K;
+}
+
+// Finite-difference step of the Sersic index for its Jacobian column, as
+// SersicGalaxy.getDerivativeShearedProfiles (sersic.py:447-487): +0.01, or
+// -0.01 where that would pass the upper bound; if the stepped profile has a
+// different number of components, the opposite step when it stays inside
+// the bounds and keeps the count.
+__device__ double sersic_fd_step(const SersicTab &T, double n0) {
+    double amp[TRH_MAXK + 1], var[TRH_MAXK + 1];
+    double step = SER_STEP;
+    if (n0 + step > SER_UPPER) step = -step;
+    const int K0 = sersic_profile(T, n0, amp, var);
+    if (sersic_profile(T, n0 + step, amp, var) != K0) {
+        const double n2 = n0 - step;
+        if (n2 < SER_UPPER && n2 > SER_LOWER && sersic_profile(T, n2, amp, var) == K0)
+            step = -step;
+    }
+    return step;
+}
 
 struct trh_fitter {
     trh_handle *h = nullptr;
@@ -63,6 +196,12 @@ struct trh_fitter {
     float *stamp = nullptr;
     float *spec[NFFT] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float *renders = nullptr;
+    // Sersic galaxies (kind 3): their mixture table (set by
+    // trh_fitter_set_sersic) and each one's index step of the last Jacobian
+    int n_ser = 0;
+    int nj = 4;   // Jacobian render slots per source: 5 when there are Sersic galaxies
+    SersicTab *ser = nullptr;
+    double *sstep = nullptr;
     double *params = nullptr, *trial = nullptr, *X = nullptr, *neq = nullptr, *chi2ls = nullptr;
     int32_t *nsteps = nullptr, *steplim = nullptr, *active = nullptr, *iters = nullptr;
     double *chisq = nullptr, *dlnp = nullptr;
@@ -108,13 +247,13 @@ __device__ __forceinline__ int sz_bucket(int sz) {
     return -1;
 }
 
-// Build the FFT render job of a galaxy at (x, y) with EllipseESoft shape
-// (logre, ee1, ee2) on the modelMask (0, 0, P, P) (galaxy.py:336-406,
-// 659-669; ellipses.py:126-155, 232-255; NullWCS pixscale 1).  Returns the
-// FFT size.
-__device__ int build_gal_job(trh_fft_job &J, const FitProfiles &pr, int kind, double x, double y,
-                             double logre, double ee1, double ee2, int P, int ph, int pw) {
-    const int pi = kind - 1;
+// Build the FFT render job of a galaxy with profile (amp, var)[K] at (x, y)
+// with EllipseESoft shape (logre, ee1, ee2) on the modelMask (0, 0, P, P)
+// (galaxy.py:336-406, 659-669; ellipses.py:126-155, 232-255; NullWCS
+// pixscale 1).  Returns the FFT size.
+__device__ int build_gal_job(trh_fft_job &J, const double *amp, const double *var, int K,
+                             double x, double y, double logre, double ee1, double ee2, int P,
+                             int ph, int pw) {
     const double re = exp(logre);
     const double ee = hypot(ee1, ee2);
     const double e = 1.0 - exp(-ee);
@@ -129,10 +268,10 @@ __device__ int build_gal_job(trh_fft_job &J, const FitProfiles &pr, int kind, do
     const double M00 = T00 * T00 + T01 * T01;
     const double M01 = T00 * T10 + T01 * T11;
     const double M11 = T10 * T10 + T11 * T11;
-    J.K = pr.K[pi];
-    for (int k = 0; k < pr.K[pi]; ++k) {
-        const double v = pr.var[pi][k];
-        J.amp[k] = (float)pr.amp[pi][k];
+    J.K = K;
+    for (int k = 0; k < K; ++k) {
+        const double v = var[k];
+        J.amp[k] = (float)amp[k];
         J.var[k][0] = (float)(v * M00);
         J.var[k][1] = (float)(v * M01);
         J.var[k][2] = (float)(v * M11);
@@ -267,30 +406,53 @@ __device__ __forceinline__ void copy_jobs_out(const trh_fft_job *sj, const int8_
         if (built[i / W]) d[i] = s[i];
 }
 
-// One thread per (source, render): render 0 of every active source and
-// renders 1..3 (EllipseE steps) of the galaxies.
-__global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, const int32_t *__restrict__ kind,
+// The profile of a galaxy of kind kd (1 exp, 2 dev: the fixed tables; 3:
+// the Sersic mixture at index n, evaluated into sa, sv).  Returns K.
+__device__ __forceinline__ int gal_profile(const FitProfiles &pr, const SersicTab *ser, int kd,
+                                           double n, const double *&amp, const double *&var,
+                                           double *sa, double *sv) {
+    if (kd == 3) {
+        amp = sa;
+        var = sv;
+        return sersic_profile(*ser, n, sa, sv);
+    }
+    amp = pr.amp[kd - 1];
+    var = pr.var[kd - 1];
+    return pr.K[kd - 1];
+}
+
+// Jacobian render planes of a source: the unit render, the three EllipseE
+// steps of a galaxy and the index step of a Sersic galaxy.
+__device__ __forceinline__ int jac_planes(int kd) { return kd == 0 ? 1 : (kd == 3 ? 5 : 4); }
+
+// One thread per (source, render), nj slots per source: render 0 of every
+// active source, renders 1..3 (EllipseE steps) of the galaxies and render 4
+// (index step, whose size goes to sstep[s]) of the Sersic galaxies.
+__global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const int32_t *__restrict__ kind,
                                const int32_t *__restrict__ Parr, const int64_t *__restrict__ roff,
                                const double *__restrict__ params,
                                const int32_t *__restrict__ active, const FitProfiles *__restrict__ pr,
+                               const SersicTab *__restrict__ ser, double *__restrict__ sstep,
                                int ph, int pw, trh_fft_job *__restrict__ jobs,
                                int8_t *__restrict__ bk, int32_t *__restrict__ bcnt,
                                int32_t *__restrict__ err, unsigned long long *__restrict__ acc) {
     __shared__ trh_fft_job sj[PREP_TB];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = t >> 2, k = t & 3;
+    const int s = t / nj, k = t - s * nj;
     int b = -1;
-    if (s < S && active[s] && (k == 0 || kind[s] != 0)) {
+    if (s < S && active[s] && k < jac_planes(kind[s])) {
         const int kd = kind[s];
         const int P = Parr[s];
         const double *p = params + 8 * s;
         const unsigned long long PP = (unsigned long long)P * P;
         if (acc && k == 0) {
-            // normal equations read N_r renders + data + inverr, write 28 doubles
+            // normal equations read the renders + data + inverr and write
+            // the packed sums; the solve reads them and writes X and trials
+            const unsigned long long nc = kd == 3 ? 7 : 6, nq = nc * (nc + 1) / 2 + nc + 2;
             atomicAdd(&acc[2 * SL_NEQ], 1ull);
-            atomicAdd(&acc[2 * SL_NEQ + 1], (kd ? 4ull * 4 + 8 : 4ull + 8) * PP + 28 * 8);
+            atomicAdd(&acc[2 * SL_NEQ + 1], (4ull * jac_planes(kd) + 8) * PP + nq * 8);
             atomicAdd(&acc[2 * SL_SOLVE], 1ull);
-            atomicAdd(&acc[2 * SL_SOLVE + 1], 28ull * 8 + 8ull * 8 * NALPHA + 8 * 8);
+            atomicAdd(&acc[2 * SL_SOLVE + 1], nq * 8 + 8ull * 8 * NALPHA + 8 * 8);
         }
         trh_fft_job &J = sj[threadIdx.x];
         if (kd == 0) {
@@ -299,8 +461,17 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, const int32_t *
             b = PTB;
         } else {
             double sh[3] = {p[3], p[4], p[5]};
-            if (k > 0) sh[k - 1] += 0.01;  // EllipseE stepsizes (ellipses.py:44)
-            const int sz = build_gal_job(J, *pr, kd, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
+            if (k > 0 && k < 4) sh[k - 1] += 0.01;  // EllipseE stepsizes (ellipses.py:44)
+            double n = p[6];
+            if (k == 4) {
+                const double st = sersic_fd_step(*ser, n);
+                sstep[s] = st;
+                n += st;
+            }
+            double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1];
+            const double *amp, *var;
+            const int K = gal_profile(*pr, ser, kd, n, amp, var, sa, sv);
+            const int sz = build_gal_job(J, amp, var, K, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
             J.out_offset = roff[s] + (int64_t)k * P * P;
             b = sz_bucket(sz);
             if (b < 0) atomicExch(err, sz);
@@ -316,19 +487,20 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, const int32_t *
     }
     __shared__ int8_t built[PREP_TB];
     built[threadIdx.x] = b >= 0;
-    block_bucket_counts(b, t, 4 * S, bk, bcnt);
+    block_bucket_counts(b, t, nj * S, bk, bcnt);
     __syncthreads();
     const int first = blockIdx.x * blockDim.x;
-    copy_jobs_out(sj, built, jobs + first, min((int)blockDim.x, 4 * S - first));
+    copy_jobs_out(sj, built, jobs + first, min((int)blockDim.x, nj * S - first));
 }
 
 // Model renders (counts x unit-flux patch at the current parameters) into an
 // image buffer laid out like `data` (patch s at pix_off[s]).
-__global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
+__global__ void k_fit_prep_model(int S, int nj, const int32_t *__restrict__ kind,
                                  const int32_t *__restrict__ Parr,
                                  const int64_t *__restrict__ pix_off,
                                  const double *__restrict__ params,
-                                 const FitProfiles *__restrict__ pr, int ph, int pw,
+                                 const FitProfiles *__restrict__ pr,
+                                 const SersicTab *__restrict__ ser, int ph, int pw,
                                  trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
                                  int cap, int32_t *__restrict__ cnt, int32_t *__restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,13 +508,16 @@ __global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
     const int kd = kind[s];
     const int P = Parr[s];
     const double *p = params + 8 * s;
-    trh_fft_job &J = jobs[4 * s];
+    trh_fft_job &J = jobs[(size_t)nj * s];
     int b;
     if (kd == 0) {
         build_point_job(J, p[0], p[1], P, ph, pw);
         b = PTB;
     } else {
-        const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
+        double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1];
+        const double *amp, *var;
+        const int K = gal_profile(*pr, ser, kd, p[6], amp, var, sa, sv);
+        const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
         b = sz_bucket(sz);
         if (b < 0) {
             atomicExch(err, sz);
@@ -354,7 +529,7 @@ __global__ void k_fit_prep_model(int S, const int32_t *__restrict__ kind,
     J.scale = (float)p[2];
     J.slot = 0;
     const int i = atomicAdd(&cnt[b], 1);
-    lists[b * cap + i] = 4 * s;
+    lists[b * cap + i] = nj * s;
 }
 
 // Sums over the 64 lanes of 32 per-lane values, scattered: afterwards lane l
@@ -389,26 +564,31 @@ __device__ __forceinline__ double wave_allsum_d(double v) {
     return v;
 }
 
-// One pixel's contribution to the 27 FP32 sums (21 of J^T W J, 6 of J^T W r)
-// and to chi^2 (FP64).  Columns (galaxy.py:137-223, pointsource.py:130-241,
-// lsqr_optimizer.py:386-431):
+// One pixel's contribution to the NQ = NC (NC + 1) / 2 + NC FP32 sums (the
+// packed J^T W J, then J^T W r; NC = 6, or 7 with a Sersic index column) and
+// to chi^2 (FP64).  Columns (galaxy.py:137-223, pointsource.py:130-241,
+// sersic.py:398-436, lsqr_optimizer.py:386-431):
 //   pos_i = ((dxp*cdi[0][i] + dyp*cdi[1][i]) * counts) * inverr  (central
 //           differences of the unit render, zero 1-px border)
 //   flux  = render * inverr
 //   shape = ((R_k - R_0) * (counts / 0.01)) * inverr
+//   index = ((R_4 - R_0) * (counts / step)) * inverr  (step: sstep[s])
 //   b     = (data - counts * R_0) * inverr
 // One pixel's inputs to the normal equations: the model render, its four
-// neighbours (central differences for the position columns), the three
+// neighbours (central differences for the position columns), the NC - 3
 // derivative renders of a galaxy, inverr and data.
+template <int NC>
 struct NeqIn {
-    float r0, dx, dy, g[3], e, d;
+    float r0, dx, dy, g[NC - 3], e, d;
 };
 
+template <int NC>
 struct NeqPix {
+    static constexpr int NT = NC * (NC + 1) / 2, NQ = NT + NC;
     const float *R0, *D, *E;
     int P, PP, kd;
-    float invP, counts, cstep, c00, c01, c10, c11;
-    __device__ __forceinline__ void load(int idx, NeqIn &in) const {
+    float invP, counts, cstep, istep, c00, c01, c10, c11;
+    __device__ __forceinline__ void load(int idx, NeqIn<NC> &in) const {
         // exact for PP < 2^24: (idx + 0.5) / P is within 0.5/P of y + 0.5
         const int y = (int)(((float)idx + 0.5f) * invP);
         const int x = idx - y * P;
@@ -419,27 +599,28 @@ struct NeqPix {
         in.dx = (l - r) / 2.f;
         in.dy = (u - w) / 2.f;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) in.g[k] = kd ? R0[(k + 1) * PP + idx] : 0.f;
+        for (int k = 0; k < NC - 3; ++k) in.g[k] = kd ? R0[(k + 1) * PP + idx] : 0.f;
         in.e = E[idx];
         in.d = D[idx];
     }
-    __device__ __forceinline__ void add(const NeqIn &in, float (&acc)[27], double &chi2,
+    __device__ __forceinline__ void add(const NeqIn<NC> &in, float (&acc)[NQ], double &chi2,
                                         double &chi0) const {
         const float e = in.e, r0 = in.r0;
-        float col[6];
+        float col[NC];
         col[0] = ((in.dx * c00 + in.dy * c10) * counts) * e;
         col[1] = ((in.dx * c01 + in.dy * c11) * counts) * e;
         col[2] = r0 * e;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) col[3 + k] = kd ? ((in.g[k] - r0) * cstep) * e : 0.f;
+        for (int k = 0; k < NC - 3; ++k)
+            col[3 + k] = kd ? ((in.g[k] - r0) * (k < 3 ? cstep : istep)) * e : 0.f;
         const float b = (in.d - r0 * counts) * e;
         int q = 0;
 #pragma unroll
-        for (int i = 0; i < 6; ++i)
+        for (int i = 0; i < NC; ++i)
 #pragma unroll
-            for (int j = i; j < 6; ++j) acc[q++] += col[i] * col[j];
+            for (int j = i; j < NC; ++j) acc[q++] += col[i] * col[j];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) acc[21 + i] += col[i] * b;
+        for (int i = 0; i < NC; ++i) acc[NT + i] += col[i] * b;
         chi2 += (double)b * (double)b;
         if (!kd) {
             const float c0 = in.d * e;   // the chi of a zero model
@@ -454,30 +635,37 @@ struct NeqPix {
 // thread takes four pixels per step and issues all their loads before any of
 // their arithmetic (written as four inlined per-pixel steps, the compiler
 // interleaved loads and arithmetic: 2.9 TB/s against 4.0 on one config-2
-// iteration); each wave reduce-scatters its 27 sums (wave_reduce_scatter32) and the waves
-// combine through LDS.
+// iteration); each wave reduce-scatters its first 32 sums
+// (wave_reduce_scatter32), adds up the rest lane-wide, and the waves combine
+// through LDS.  NC = 6 takes every source but the Sersic galaxies, NC = 7
+// only those.
 #define NEQ_TB 512   // threads per source in k_fit_normal_eq
 #define NEQ_U 4      // pixels per thread in flight
 
+template <int NC>
 __global__ void __launch_bounds__(NEQ_TB)
 k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
                 const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
                 const float *__restrict__ data, const float *__restrict__ inverr,
                 const float *__restrict__ renders, const double *__restrict__ params,
+                const double *__restrict__ sstep,
                 const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
                 double cdi11, double *__restrict__ neq, const int32_t *__restrict__ order) {
-    __shared__ double red[NEQ_TB / 64][29];
+    constexpr int NQ = NeqPix<NC>::NQ;
+    static_assert(NQ <= NEQ_CHI2, "packed sums overlap chi^2");
+    __shared__ double red[NEQ_TB / 64][NEQ_W];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = blockIdx.x; i < S; i += gridDim.x) {
         const int s = order[i];
-        if (!active[s]) continue;
-        NeqPix px;
+        if (!active[s] || ((kind[s] == 3) != (NC == 7))) continue;
+        NeqPix<NC> px;
         px.kd = kind[s];
         px.P = Parr[s];
         px.PP = px.P * px.P;
         px.invP = 1.0f / (float)px.P;
         px.counts = (float)params[8 * s + 2];
         px.cstep = (float)(params[8 * s + 2] / 0.01);
+        px.istep = NC == 7 ? (float)(params[8 * s + 2] / sstep[s]) : 0.f;
         px.c00 = (float)cdi00;
         px.c01 = (float)cdi01;
         px.c10 = (float)cdi10;
@@ -485,42 +673,51 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         px.R0 = renders + roff[s];
         px.D = data + pix_off[s];
         px.E = inverr + pix_off[s];
-        float acc[27];
+        float acc[NQ];
 #pragma unroll
-        for (int k = 0; k < 27; ++k) acc[k] = 0.f;
+        for (int k = 0; k < NQ; ++k) acc[k] = 0.f;
         double chi2 = 0.0, chi0 = 0.0;
         int idx = threadIdx.x;
         for (; idx + (NEQ_U - 1) * NEQ_TB < px.PP; idx += NEQ_U * NEQ_TB) {
             // all the pixels' loads issue before any of their arithmetic
-            NeqIn in[NEQ_U];
+            NeqIn<NC> in[NEQ_U];
 #pragma unroll
             for (int u = 0; u < NEQ_U; ++u) px.load(idx + u * NEQ_TB, in[u]);
 #pragma unroll
             for (int u = 0; u < NEQ_U; ++u) px.add(in[u], acc, chi2, chi0);
         }
         for (; idx < px.PP; idx += NEQ_TB) {
-            NeqIn in;
+            NeqIn<NC> in;
             px.load(idx, in);
             px.add(in, acc, chi2, chi0);
         }
         float v32[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) v32[k] = k < 27 ? acc[k] : 0.f;
+        for (int k = 0; k < 32; ++k) v32[k] = k < NQ ? acc[k] : 0.f;
         const float sum = wave_reduce_scatter32(v32);
         const int q = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 +
                       ((lane >> 2) & 1) * 2 + ((lane >> 1) & 1);
         chi2 = wave_allsum_d(chi2);
         chi0 = wave_allsum_d(chi0);
-        if (!(lane & 1) && q < 27) red[wid][q] = (double)sum;
+        if (!(lane & 1) && q < NQ) red[wid][q] = (double)sum;
+#pragma unroll
+        for (int k = 32; k < NQ; ++k) {
+            float v = acc[k];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[wid][k] = (double)v;
+        }
         if (lane == 0) {
-            red[wid][27] = chi2;
-            red[wid][28] = chi0;
+            red[wid][NEQ_CHI2] = chi2;
+            red[wid][NEQ_CHI0] = chi0;
         }
         __syncthreads();
-        if (threadIdx.x < 29) {
+        if (threadIdx.x < NEQ_W) {
+            const int k = threadIdx.x;
             double t = 0.0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
-            neq[32 * s + threadIdx.x] = t;
+            if (k < NQ || k == NEQ_CHI2 || k == NEQ_CHI0)
+                for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][k];
+            neq[(size_t)NEQ_W * s + k] = t;
         }
         __syncthreads();
     }
@@ -529,10 +726,11 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
 // Column-scaled damped Cholesky solve (FP64) + getParameterSteps
 // (constrained_optimizer.py:84-159).  One thread per source.
 // Column-scaled, damped FP64 Cholesky solve of one source's NP x NP normal
-// equations (q: the 21 upper-triangle entries of the 6 x 6 matrix, then the
-// 6 gradient entries), then the line-search steps (getParameterSteps).  NP is
-// a compile-time constant so every array stays in registers.
-template <int NP>
+// equations (q: the packed upper triangle of the NC x NC matrix, then the NC
+// gradient entries; NP <= NC), then the line-search steps
+// (getParameterSteps).  NP and NC are compile-time constants so every array
+// stays in registers.
+template <int NP, int NC>
 __device__ __forceinline__ void solve_source(int s, const double *__restrict__ q,
                                              const double *__restrict__ p0, double lambda,
                                              double *__restrict__ X, double *__restrict__ trial,
@@ -543,13 +741,13 @@ __device__ __forceinline__ void solve_source(int s, const double *__restrict__ q
     for (int i = 0; i < NP; ++i)
 #pragma unroll
         for (int j = i; j < NP; ++j) {
-            // upper-triangle index of (i, j) in the 6 x 6 packing
-            const double v = q[i * 6 - i * (i - 1) / 2 + (j - i)];
+            // upper-triangle index of (i, j) in the NC x NC packing
+            const double v = q[i * NC - i * (i - 1) / 2 + (j - i)];
             A[i][j] = v;
             A[j][i] = v;
         }
 #pragma unroll
-    for (int i = 0; i < NP; ++i) g[i] = q[21 + i];
+    for (int i = 0; i < NP; ++i) g[i] = q[NC * (NC + 1) / 2 + i];
     bool any = false;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
@@ -624,8 +822,9 @@ __device__ __forceinline__ void solve_source(int s, const double *__restrict__ q
     for (int i = 0; i < 8; ++i) X[8 * s + i] = i < NP ? x[i < NP ? i : 0] : 0.0;
 
     // getParameterSteps: maxstep PixPos [1,1] (wcs.py:384), Flux none,
-    // EllipseESoft [1, 0.1, 0.1] (ellipses.py:48); no bounds for ESoft.
-    const double maxstep[6] = {1.0, 1.0, -1.0, 1.0, 0.1, 0.1};
+    // EllipseESoft [1, 0.1, 0.1] (ellipses.py:48), SersicIndex 0.1; bounds
+    // only on the Sersic index, [0.29, 6.3] (sersic.py:344-359).
+    const double maxstep[7] = {1.0, 1.0, -1.0, 1.0, 0.1, 0.1, SER_MAXSTEP};
     double p[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) p[i] = p0[i];
@@ -644,10 +843,20 @@ __device__ __forceinline__ void solve_source(int s, const double *__restrict__ q
                 alpha = maxstep[i] / fabs(x[i]);
             }
         }
+        if (NP == 7) {
+            // reduce alpha to land on a bound the step would pass
+            double px = p[NP - 1] + alpha * x[NP - 1];
+            if (px < SER_LOWER) {
+                alpha = (SER_LOWER - p[NP - 1]) / x[NP - 1];
+                px = SER_LOWER;
+            }
+            if (px > SER_UPPER) alpha = (SER_UPPER - p[NP - 1]) / x[NP - 1];
+        }
         if (alpha < 1e-8) break;
         double *tp = trial + ((size_t)s * NALPHA + n) * 8;
 #pragma unroll
         for (int i = 0; i < 8; ++i) tp[i] = i < NP ? p[i < NP ? i : 0] + alpha * x[i < NP ? i : 0] : 0.0;
+        if (NP == 7) tp[NP - 1] = fmin(fmax(tp[NP - 1], SER_LOWER), SER_UPPER);
         ++n;
         if (do_break) break;
     }
@@ -666,10 +875,13 @@ __global__ void __launch_bounds__(64) k_fit_solve(int S, const int32_t *__restri
         nsteps[s] = 0;
         return;
     }
-    if (kind[s])
-        solve_source<6>(s, neq + 32 * s, params + 8 * s, lambda, X, trial, nsteps, steplim);
+    const double *q = neq + (size_t)NEQ_W * s;
+    if (kind[s] == 3)
+        solve_source<7, 7>(s, q, params + 8 * s, lambda, X, trial, nsteps, steplim);
+    else if (kind[s])
+        solve_source<6, 6>(s, q, params + 8 * s, lambda, X, trial, nsteps, steplim);
     else
-        solve_source<3>(s, neq + 32 * s, params + 8 * s, lambda, X, trial, nsteps, steplim);
+        solve_source<3, 6>(s, q, params + 8 * s, lambda, X, trial, nsteps, steplim);
 }
 
 // One thread per (source, trial step) of the line search.
@@ -679,7 +891,8 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
                               const double *__restrict__ trial,
                               const int32_t *__restrict__ nsteps,
                               const int32_t *__restrict__ active,
-                              const FitProfiles *__restrict__ pr, int ph, int pw,
+                              const FitProfiles *__restrict__ pr,
+                              const SersicTab *__restrict__ ser, int ph, int pw,
                               trh_fft_job *__restrict__ jobs, int8_t *__restrict__ bk,
                               int32_t *__restrict__ bcnt, int32_t *__restrict__ err,
                               unsigned long long *__restrict__ acc) {
@@ -694,10 +907,13 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
         trh_fft_job &J = sj[threadIdx.x];
         if (kd == 0) {
             build_point_job(J, p[0], p[1], P, ph, pw);
-            J.pad0 = 32 * s + 28;   // the window's zero-model chi^2 in neq
+            J.pad0 = NEQ_W * s + NEQ_CHI0;   // the window's zero-model chi^2 in neq
             b = PTB;
         } else {
-            const int sz = build_gal_job(J, *pr, kd, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
+            double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1];
+            const double *amp, *var;
+            const int K = gal_profile(*pr, ser, kd, p[6], amp, var, sa, sv);
+            const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
             b = sz_bucket(sz);
             if (b < 0) atomicExch(err, sz);
         }
@@ -739,7 +955,7 @@ __global__ void k_fit_accept(int S, const double *__restrict__ neq,
                              double dchisq, int max_iter) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S || !active[s]) return;
-    const double chi0 = neq[32 * s + 27];
+    const double chi0 = neq[(size_t)NEQ_W * s + NEQ_CHI2];
     const double lnp0 = -0.5 * chi0;
     double best = lnp0;
     int ib = -1;
@@ -860,15 +1076,16 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
     int64_t tot = 0;
     int maxP_sz[NFFT] = {0, 0, 0, 0, 0};
     for (int s = 0; s < S; ++s) {
-        if (kind[s] < 0 || kind[s] > 2 || P[s] < 3 || P[s] > 4096) {
+        if (kind[s] < 0 || kind[s] > 3 || P[s] < 3 || P[s] > 4096) {
             trh_fitter_destroy(f);
             return set_err(TRH_E_ARG, "trh_fitter_create: source %d has kind %d / P %d", s, kind[s],
                            P[s]);
         }
         roff[s] = tot;
-        tot += (int64_t)(kind[s] ? 4 : 1) * P[s] * P[s];
+        tot += (int64_t)(kind[s] == 3 ? 5 : (kind[s] ? 4 : 1)) * P[s] * P[s];
         if (kind[s]) {
             ++f->n_gal;
+            if (kind[s] == 3) ++f->n_ser;
             for (int b = 0; b < NFFT; ++b)
                 if (P[s] < kSizes[b] && P[s] > maxP_sz[b]) maxP_sz[b] = P[s];
         } else {
@@ -890,13 +1107,15 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         trh_fitter_destroy(f);
         return set_err(TRH_E_UNSUPPORTED, "point-source patch width %d too large for LDS", f->maxP_pt);
     }
-    const int cap_jac = 4 * S, cap_ls = NALPHA * S;
+    f->nj = f->n_ser ? 5 : 4;
+    const int cap_jac = f->nj * S, cap_ls = NALPHA * S;
     if ((rc = falloc(f, &f->kind, S)) || (rc = falloc(f, &f->P, S)) ||
+        (rc = falloc(f, &f->sstep, S)) ||
         (rc = falloc(f, &f->pix_off, S)) || (rc = falloc(f, &f->render_off, S)) ||
         (rc = falloc(f, &f->stamp, (size_t)ph * pw)) || (rc = falloc(f, &f->renders, tot)) ||
         (rc = falloc(f, &f->params, 8 * (size_t)S)) ||
         (rc = falloc(f, &f->trial, 8 * (size_t)S * NALPHA)) || (rc = falloc(f, &f->X, 8 * (size_t)S)) ||
-        (rc = falloc(f, &f->neq, 32 * (size_t)S)) || (rc = falloc(f, &f->chi2ls, (size_t)S * NALPHA)) ||
+        (rc = falloc(f, &f->neq, NEQ_W * (size_t)S)) || (rc = falloc(f, &f->chi2ls, (size_t)S * NALPHA)) ||
         (rc = falloc(f, &f->nsteps, S)) || (rc = falloc(f, &f->steplim, S)) ||
         (rc = falloc(f, &f->active, S)) || (rc = falloc(f, &f->iters, S)) ||
         (rc = falloc(f, &f->chisq, S)) || (rc = falloc(f, &f->dlnp, S)) ||
@@ -965,7 +1184,8 @@ extern "C" int trh_fitter_set_params(trh_fitter *f, const double *params, void *
     if (!f || !params) return set_err(TRH_E_ARG, "trh_fitter_set_params: null argument");
     std::vector<double> p8((size_t)8 * f->S, 0.0);
     for (int s = 0; s < f->S; ++s)
-        for (int i = 0; i < 6; ++i) p8[8 * (size_t)s + i] = params[6 * (size_t)s + i];
+        for (int i = 0; i < TRH_FIT_NPARAM; ++i)
+            p8[8 * (size_t)s + i] = params[TRH_FIT_NPARAM * (size_t)s + i];
     std::vector<int32_t> ones(f->S, 1), zeros(f->S, 0);
     int rc;
     if ((rc = hip_check(hipMemcpy(f->params, p8.data(), 8 * p8.size(), hipMemcpyHostToDevice), "set params")))
@@ -983,8 +1203,64 @@ extern "C" int trh_fitter_get_params(trh_fitter *f, double *params, void *stream
     if ((rc = hip_check(hipMemcpy(p8.data(), f->params, 8 * p8.size(), hipMemcpyDeviceToHost), "get params")))
         return rc;
     for (int s = 0; s < f->S; ++s)
-        for (int i = 0; i < 6; ++i) params[6 * (size_t)s + i] = p8[8 * (size_t)s + i];
+        for (int i = 0; i < TRH_FIT_NPARAM; ++i)
+            params[TRH_FIT_NPARAM * (size_t)s + i] = p8[8 * (size_t)s + i];
     return TRH_OK;
+}
+
+extern "C" int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab) {
+    if (!f || !tab || ntab < 1) return set_err(TRH_E_ARG, "trh_fitter_set_sersic: bad arguments");
+    SersicTab *T = new SersicTab();
+    int pos = 0, nd = 0;
+    // next value of the flat table, -1e300 past its end (rejected below)
+    auto next = [&]() { return pos < ntab ? tab[pos++] : (++pos, -1e300); };
+    // a count read from the table: an integer in [lo, hi]
+    auto count = [&](int lo, int hi, int &out) {
+        const double v = next();
+        if (!(v >= lo && v <= hi) || v != (double)(int)v) return false;
+        out = (int)v;
+        return true;
+    };
+    auto copy = [&](int n) {
+        if (nd + n > SER_MAXD || pos + n > ntab) return false;
+        for (int i = 0; i < n; ++i) T->d[nd++] = tab[pos++];
+        return true;
+    };
+    bool ok = count(1, SER_MAXR, T->nr);
+    int maxK = 0;
+    for (int r = 0; ok && r < T->nr; ++r) {
+        T->lo[r] = next();
+        T->hi[r] = next();
+        ok = T->lo[r] < T->hi[r] && (r == 0 || T->lo[r] > T->lo[r - 1]) &&
+             count(1, TRH_MAXK, T->K[r]) && count(1, 256, T->nint[r]);
+        if (!ok) break;
+        T->boff[r] = nd;
+        ok = copy(T->nint[r] + 1);
+        T->coff[r] = nd;
+        ok = ok && copy(8 * T->K[r] * T->nint[r]);
+        int k = T->K[r];
+        // a ramp between overlapping ranges carries both ranges' components
+        if (r > 0 && T->lo[r] < T->hi[r - 1]) k += T->K[r - 1];
+        if (k > maxK) maxK = k;
+    }
+    ok = ok && count(1, 256, T->core_nint);
+    if (ok) {
+        T->core_boff = nd;
+        ok = copy(T->core_nint + 1);
+        T->core_coff = nd;
+        ok = ok && copy(4 * T->core_nint) && pos == ntab;
+    }
+    if (ok && maxK + 1 > TRH_MAXK) ok = false;
+    int rc = TRH_OK;
+    if (!ok)
+        rc = set_err(TRH_E_ARG, "trh_fitter_set_sersic: malformed table (at value %d of %d)", pos,
+                     ntab);
+    else if (!f->ser)
+        rc = falloc(f, &f->ser, 1);
+    if (!rc)
+        rc = hip_check(hipMemcpy(f->ser, T, sizeof(SersicTab), hipMemcpyHostToDevice), "copy sersic table");
+    delete T;
+    return rc;
 }
 
 static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *list,
@@ -1019,8 +1295,10 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
 static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t st,
                         bool capture) {
     const int S = f->S;
-    const int cap_jac = 4 * S, cap_ls = NALPHA * S;
+    const int cap_jac = f->nj * S, cap_ls = NALPHA * S;
     const int tb = 256, nb = (S + tb - 1) / tb;
+    if (f->n_ser && !f->ser)
+        return set_err(TRH_E_ARG, "trh_fitter_iterate: Sersic galaxies need trh_fitter_set_sersic");
     const bool prof = f->profiling && !capture;
     unsigned long long *acc = prof ? f->acc : nullptr;
     int rc;
@@ -1028,8 +1306,8 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     SLOT_BEGIN(SL_PREP);
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
     const int nb_jac = (cap_jac + PREP_TB - 1) / PREP_TB, nb_ls = (cap_ls + PREP_TB - 1) / PREP_TB;
-    hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb_jac), dim3(PREP_TB), 0, st, S, f->kind, f->P,
-                       f->render_off, f->params, f->active, f->prof, f->ph, f->pw, f->jobs_jac,
+    hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb_jac), dim3(PREP_TB), 0, st, S, f->nj, f->kind, f->P,
+                       f->render_off, f->params, f->active, f->prof, f->ser, f->sstep, f->ph, f->pw, f->jobs_jac,
                        f->bk_jac, f->bcnt_jac, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_jac), dim3(PREP_TB), 0, st, cap_jac, f->bk_jac,
@@ -1040,7 +1318,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
         if (kSizes[b] < f->min_sz) continue;
         SLOT_BEGIN(b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
-                                 f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal, f->maxrows[b], f->spec[b], f->renders,
+                                 f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal + f->n_ser, f->maxrows[b], f->spec[b], f->renders,
                                  nullptr, nullptr, nullptr, st)))
             return rc;
         SLOT_END(b);
@@ -1054,11 +1332,21 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     SLOT_BEGIN(SL_NEQ);
     {
         const int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
-        hipLaunchKernelGGL(k_fit_normal_eq, dim3(grid), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
-                           f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
-                           f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq,
-                           f->neq_order);
-        if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq"))) return rc;
+        if (S > f->n_ser) {
+            hipLaunchKernelGGL(k_fit_normal_eq<6>, dim3(grid), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
+                               f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
+                               f->sstep, f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3],
+                               f->neq, f->neq_order);
+            if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq<6>"))) return rc;
+        }
+        if (f->n_ser) {
+            const int g7 = f->n_ser < f->h->n_cu * 8 ? f->n_ser : f->h->n_cu * 8;
+            hipLaunchKernelGGL(k_fit_normal_eq<7>, dim3(g7), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
+                               f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
+                               f->sstep, f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3],
+                               f->neq, f->neq_order);
+            if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq<7>"))) return rc;
+        }
     }
     SLOT_END(SL_NEQ);
     if (!capture) (void)hipEventRecord(f->ev[2], st);
@@ -1069,7 +1357,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     SLOT_END(SL_SOLVE);
     if (!capture) (void)hipEventRecord(f->ev[3], st);
     hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb_ls), dim3(PREP_TB), 0, st, S, f->kind, f->P,
-                       f->pix_off, f->trial, f->nsteps, f->active, f->prof, f->ph, f->pw,
+                       f->pix_off, f->trial, f->nsteps, f->active, f->prof, f->ser, f->ph, f->pw,
                        f->jobs_ls, f->bk_ls, f->bcnt_ls, f->err, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_ls), dim3(PREP_TB), 0, st, cap_ls, f->bk_ls,
@@ -1217,11 +1505,13 @@ extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, void *stream)
     if (!f || !out) return set_err(TRH_E_ARG, "trh_fitter_render_models: null argument");
     hipStream_t st = (hipStream_t)stream;
     const int S = f->S;
-    const int cap_jac = 4 * S;
+    const int cap_jac = f->nj * S;
     int rc;
+    if (f->n_ser && !f->ser)
+        return set_err(TRH_E_ARG, "trh_fitter_render_models: Sersic galaxies need trh_fitter_set_sersic");
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
-    hipLaunchKernelGGL(k_fit_prep_model, dim3((S + 255) / 256), dim3(256), 0, st, S, f->kind, f->P,
-                       f->pix_off, f->params, f->prof, f->ph, f->pw, f->jobs_jac, f->list_jac,
+    hipLaunchKernelGGL(k_fit_prep_model, dim3((S + 255) / 256), dim3(256), 0, st, S, f->nj, f->kind, f->P,
+                       f->pix_off, f->params, f->prof, f->ser, f->ph, f->pw, f->jobs_jac, f->list_jac,
                        cap_jac, f->cnt, f->err);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_model"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {

@@ -153,3 +153,45 @@ class SersicGalaxy(HoggGalaxy):
         d.setName('d(%s)/d(sersicindex)' % self.dname)
         derivs.append(d)
         return derivs
+
+
+def _pieces(nodes, vals, k):
+    """Breakpoints and per-piece cubic coefficients (highest power first, in
+    powers of n - breakpoint) of the interpolating spline of degree k through
+    (nodes, vals), with the zero-length pieces of repeated end knots dropped."""
+    from scipy.interpolate import PPoly, splrep
+    pp = PPoly.from_spline(splrep(nodes, vals, k=k, s=0))
+    keep = np.nonzero(np.diff(pp.x) > 0)[0]
+    br = np.append(pp.x[keep], pp.x[keep[-1] + 1])
+    c = np.zeros((len(keep), 4))
+    c[:, 3 - k:] = pp.c[:, keep].T
+    return br, c
+
+
+@functools.lru_cache(maxsize=1)
+def fitter_table():
+    """The Sersic mixture (SersicMixture above) as the flat table of piecewise
+    cubics that trh_fitter_set_sersic takes: R, then per range lo, hi, K, m,
+    the m + 1 breakpoints and, per component, m amplitude and m log-variance
+    pieces of 4 coefficients; then the core fraction's m, breakpoints and
+    pieces.  The splines are the same interpolants as SersicMixture's."""
+    out = [float(len(RANGES))]
+    for lo, hi, nodes in RANGES:
+        s = [nd[0] for nd in nodes]
+        k = 3 if len(nodes) > 3 else 1
+        K = len(nodes[0][1])
+        comps = []
+        br0 = None
+        for i in range(K):
+            for vals in ([nd[1][i] for nd in nodes], [np.log(nd[2][i]) for nd in nodes]):
+                br, c = _pieces(s, vals, k)
+                if br0 is None:
+                    br0 = br
+                assert np.array_equal(br, br0)
+                comps.append(c.ravel())
+        out += [lo, hi, float(K), float(len(br0) - 1)] + list(br0)
+        for c in comps:
+            out += list(c)
+    br, c = _pieces([c[0] for c in CORES], [c[1] for c in CORES], 3)
+    out += [float(len(br) - 1)] + list(br) + list(c.ravel())
+    return np.array(out, np.float64)
