@@ -101,6 +101,52 @@ def cpu_baseline(wl, budget_s=15.0, min_sources=4, dchisq=0.):
     return n, dt, results
 
 
+def sersic_leg(torch, n=4096, steps=3, n_check=8):
+    """Batched fit to convergence of n sources drawn as in the config-2
+    workload with every galaxy a SersicGalaxy (index U(0.5, 5); seven
+    parameters), from truth perturbed as config 4 plus U(-0.3, 0.3) in the
+    index; timed over `steps` fits after one warm-up, with the chi^2 of the
+    first n_check sources checked against the FP64 oracle's fit."""
+    import oracle as O
+    from tractor_amd import batch as B
+    from tractor_amd import workloads as W
+    kind, P, t6 = W.draw_sources(n, config=2)
+    kind = np.where(kind > 0, 3, 0).astype(np.int32)
+    rng = np.random.default_rng(W.seed(2, 9))
+    truth = np.zeros((n, 7))
+    truth[:, :6] = t6
+    ser = kind == 3
+    truth[ser, 6] = rng.uniform(0.5, 5.0, int(ser.sum()))
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    f, data, inverr, pix_off = B.make_problem(kind, P, truth, psf, W.seed(2, 10))
+    start = np.zeros((n, 7))
+    start[:, :6] = W.perturb(kind, t6, np.random.default_rng(W.seed(2, 11)))
+    start[ser, 6] = np.clip(truth[ser, 6] + rng.uniform(-0.3, 0.3, int(ser.sum())), 0.3, 6.2)
+    f.set_params(start)
+    f.fit(max_iter=100)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.set_params(start)
+        f.fit(max_iter=100)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = f.status()
+    dh, ih = data.cpu().numpy(), inverr.cpu().numpy()
+    errs = []
+    for s in np.nonzero(ser)[0][:n_check]:
+        Pz, o = int(P[s]), int(pix_off[s])
+        prob = O.SourceProblem(3, start[s], Pz, psf, data=dh[o:o + Pz * Pz].reshape(Pz, Pz),
+                               inverr=ih[o:o + Pz * Pz].reshape(Pz, Pz), cdi=B.PIXEL_CDI)
+        _, _, chi = O.fit_source(prob, start[s])
+        errs.append(abs(st['chisq'][s] - chi) / chi)
+    return {'workload': '%d sources (%d Sersic, index U(0.5, 5); the rest points), P 16..128, '
+                        'fit to convergence' % (n, int(ser.sum())),
+            'ms_per_fit': round(dt * 1e3, 3), 'sources_per_s': round(n / dt, 1),
+            'mean_iterations': round(float(st['iterations'].mean()), 2),
+            'max_rel_chisq_vs_fp64_first_%d' % len(errs): float(max(errs))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -114,6 +160,8 @@ def main():
                     help='only run three 256 MiB device-to-device copies (HBM counter '
                          'calibration under rocprofv3 --pmc)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-sersic', action='store_true',
+                    help='skip the Sersic-galaxy fit reported beside the config-4 line')
     ap.add_argument('--no-slot-profile', action='store_true',
                     help='skip the extra per-launch-timed step after the timed region (its '
                          'event timing and byte counters slow that step down; use this under '
@@ -225,6 +273,9 @@ def main():
     }
     if parity is not None:
         out['parity'] = parity
+    if wl['cfg'] == 4 and not args.no_sersic:
+        log('sersic leg')
+        out['sersic_fit'] = sersic_leg(torch)
     print(json.dumps(out), flush=True)
 
 
