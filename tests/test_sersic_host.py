@@ -150,3 +150,16 @@ def test_oracle_sersic_restatement():
     assert O.sersic_fd_step(1.0) == -0.01
     assert O.sersic_fd_step(6.295) == -0.01
     assert O.sersic_fd_step(0.3) == 0.01
+
+
+def test_table_matches_reference_source():
+    """sersic_table.py is what gen_sersic_table writes from the reference's
+    tractor/sersic.py (parsed as text; skipped where the reference tree is
+    not present)."""
+    import os
+    src = '/root/reference/tractor/sersic.py'
+    if not os.path.exists(src):
+        pytest.skip('reference source not present')
+    from tractor_amd import gen_sersic_table as G
+    import tractor_amd.sersic_table as ST
+    assert G.render(*G.read_tables(src)) == open(ST.__file__).read()
