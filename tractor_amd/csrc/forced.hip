@@ -28,24 +28,42 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // eigen-PSFs (psfex.py:326-346).  The same linear map applies to stamps and
 // to their spectra (PixelizedPsfEx.getFourierTransform, psfex.py:481-520).
 constexpr int PSF_MAXQ = 16;
-__global__ void k_psf_combine(const float *__restrict__ basis, int nbasis, int64_t len,
-                              const float *__restrict__ w, const int32_t *__restrict__ idx, int nq,
-                              float *__restrict__ out, int n) {
-    const int s = blockIdx.y;
+// One 256-thread block per source; VT = float4 where the length and the
+// pointers allow it (the spectra), float otherwise (odd-sized stamps).
+__device__ __forceinline__ float4 operator*(float a, float4 v) {
+    return make_float4(a * v.x, a * v.y, a * v.z, a * v.w);
+}
+__device__ __forceinline__ float4 &operator+=(float4 &a, float4 v) {
+    a.x += v.x;
+    a.y += v.y;
+    a.z += v.z;
+    a.w += v.w;
+    return a;
+}
+__device__ __forceinline__ void vzero(float &a) { a = 0.f; }
+__device__ __forceinline__ void vzero(float4 &a) { a = make_float4(0.f, 0.f, 0.f, 0.f); }
+template <typename VT>
+__global__ void __launch_bounds__(256)
+k_psf_combine(const float *__restrict__ basis, int nbasis, int64_t len,
+              const float *__restrict__ w, const int32_t *__restrict__ idx, int nq,
+              float *__restrict__ out, int n) {
+    const int s = blockIdx.x;
     if (s >= n) return;
+    constexpr int V = sizeof(VT) / sizeof(float);
     float ws[PSF_MAXQ];
-    const float *b[PSF_MAXQ];
+    const VT *b[PSF_MAXQ];
 #pragma unroll
     for (int q = 0; q < PSF_MAXQ; ++q) {
         int k = q < nq ? idx[nq * s + q] : 0;
         k = k < 0 ? 0 : (k >= nbasis ? nbasis - 1 : k);
         ws[q] = q < nq ? w[nq * s + q] : 0.f;
-        b[q] = basis + (int64_t)k * len;
+        b[q] = reinterpret_cast<const VT *>(basis + (int64_t)k * len);
     }
-    float *o = out + (int64_t)s * len;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < len;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        float acc = 0.f;
+    VT *o = reinterpret_cast<VT *>(out + (int64_t)s * len);
+    const int64_t nv = len / V;
+    for (int64_t e = threadIdx.x; e < nv; e += blockDim.x) {
+        VT acc;
+        vzero(acc);
 #pragma unroll
         for (int q = 0; q < PSF_MAXQ; ++q)
             if (q < nq) acc += ws[q] * b[q][e];
@@ -870,13 +888,14 @@ k_fp_fitstats(int n, const int32_t *__restrict__ win, const int64_t *__restrict_
 
 // Image chi^2 and the number of pixels with inverr > 0 (the per-image part
 // of _get_fitstats, optimize.py:244-250): one workgroup, fixed order.
-__global__ void __launch_bounds__(1024)
-k_fp_imstats(int64_t npix, const float *__restrict__ model, const float *__restrict__ data,
-             const float *__restrict__ inverr, double sky, double *__restrict__ out) {
-    __shared__ double red[2][16];
+__global__ void __launch_bounds__(256)
+k_fp_imstats_part(int64_t npix, const float *__restrict__ model, const float *__restrict__ data,
+                  const float *__restrict__ inverr, double sky, double *__restrict__ part) {
+    __shared__ double red[2][4];
     const float fsky = (float)sky;
     double c2 = 0.0, np = 0.0;
-    for (int64_t i = threadIdx.x; i < npix; i += 1024) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < npix;
+         i += (int64_t)gridDim.x * 256) {
         const float e = inverr[i];
         const float chi = (data[i] - (model[i] + fsky)) * e;
         c2 += (double)chi * (double)chi;
@@ -889,10 +908,25 @@ k_fp_imstats(int64_t npix, const float *__restrict__ model, const float *__restr
         red[1][threadIdx.x >> 6] = np;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        double t = 0.0;
-        for (int k = 0; k < 16; ++k) t += red[threadIdx.x][k];
-        out[threadIdx.x] = t;
+    if (threadIdx.x < 2)
+        part[2 * blockIdx.x + threadIdx.x] =
+            (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
+}
+
+// The per-block partials added up in block order (the same bits every run).
+__global__ void __launch_bounds__(64)
+k_fp_imstats_final(int nblk, const double *__restrict__ part, double *__restrict__ out) {
+    const int lane = threadIdx.x;
+    double c2 = 0.0, np = 0.0;
+    for (int b = lane; b < nblk; b += 64) {
+        c2 += part[2 * b];
+        np += part[2 * b + 1];
+    }
+    c2 = wave_sum_d(c2);
+    np = wave_sum_d(np);
+    if (lane == 0) {
+        out[0] = c2;
+        out[1] = np;
     }
 }
 
@@ -906,10 +940,13 @@ extern "C" int trh_psf_combine(trh_handle *h, const float *basis, int nbasis, in
         return set_err(TRH_E_ARG, "trh_psf_combine: bad arguments (n <= 65535, 1 <= nq <= %d)",
                        PSF_MAXQ);
     if (n == 0) return TRH_OK;
-    long gx = (len + 255) / 256;
-    if (gx > 64) gx = 64;
-    hipLaunchKernelGGL(k_psf_combine, dim3((int)gx, n), dim3(256), 0, (hipStream_t)stream, basis,
-                       nbasis, len, weights, idx, nq, out, n);
+    const bool vec = len % 4 == 0 && ((uintptr_t)basis % 16) == 0 && ((uintptr_t)out % 16) == 0;
+    if (vec)
+        hipLaunchKernelGGL(k_psf_combine<float4>, dim3(n), dim3(256), 0, (hipStream_t)stream, basis,
+                           nbasis, len, weights, idx, nq, out, n);
+    else
+        hipLaunchKernelGGL(k_psf_combine<float>, dim3(n), dim3(256), 0, (hipStream_t)stream, basis,
+                           nbasis, len, weights, idx, nq, out, n);
     return hip_check(hipGetLastError(), "k_psf_combine");
 }
 
@@ -1125,7 +1162,17 @@ extern "C" int trh_fp_imstats(trh_handle *h, int64_t npix, const float *model, c
                               const float *inverr, double sky, double *out, void *stream) {
     if (!h || npix < 0 || (npix > 0 && (!model || !data || !inverr)) || !out)
         return set_err(TRH_E_ARG, "trh_fp_imstats: bad arguments");
-    hipLaunchKernelGGL(k_fp_imstats, dim3(1), dim3(1024), 0, (hipStream_t)stream, npix, model,
-                       data, inverr, sky, out);
-    return hip_check(hipGetLastError(), "k_fp_imstats");
+    // about eight pixels per thread, at most 2048 blocks of 256
+    int64_t nblk = (npix + 2047) / 2048;
+    if (nblk < 1) nblk = 1;
+    if (nblk > 2048) nblk = 2048;
+    int rc;
+    if ((rc = ensure_scratch(h, 16 * (size_t)nblk))) return rc;
+    double *part = (double *)h->scratch;
+    hipLaunchKernelGGL(k_fp_imstats_part, dim3((int)nblk), dim3(256), 0, (hipStream_t)stream, npix,
+                       model, data, inverr, sky, part);
+    if ((rc = hip_check(hipGetLastError(), "k_fp_imstats_part"))) return rc;
+    hipLaunchKernelGGL(k_fp_imstats_final, dim3(1), dim3(64), 0, (hipStream_t)stream, (int)nblk,
+                       part, out);
+    return hip_check(hipGetLastError(), "k_fp_imstats_final");
 }
