@@ -171,7 +171,7 @@ def main():
                     help='optimize_loop stopping tolerance on dlnp (reference default 0)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU only: this build runs the batched fit on a single '
+        print('bench.py measures one GPU only: the batched fit runs on a single '
               'MI355X (multi-GPU sharding is designed, not built); use --gpus 1', file=sys.stderr)
         sys.exit(2)
 
