@@ -313,3 +313,28 @@ def test_forced_photometry_psfex_matches_oracle():
     ok = (err <= 1e-4 * np.abs(ref)) | (err <= 0.01 * sig)
     print('flux parity: %d/%d; max err/sigma %.3g' % (ok.sum(), n, np.max(err / sig)), flush=True)
     assert ok.all()
+
+
+def test_device_catalog_gives_the_same_fit():
+    """Sources and PSF weights passed as device tensors (source_catalog)
+    give bit for bit the fluxes of the host-array path."""
+    import torch
+    from tractor_amd import forced as F, workloads as W
+    size, n = 384, 600
+    rng = np.random.default_rng(11)
+    kind, pos, flux, shape = W.draw_forced_scene(n, size, rng=rng)
+    basis = W.psf_grid_stamps()
+    idx, w = W.bilinear_psf_weights(pos[:, 0], pos[:, 1], size, size)
+    inverr = torch.ones((size, size), dtype=torch.float32, device='cuda')
+    syn = F.BatchForcedPhotometry(torch.zeros_like(inverr), inverr, basis, idx, w, kind, pos,
+                                  shape).plan().render()
+    g = torch.Generator(device='cuda')
+    g.manual_seed(3)
+    data = syn.model_image(flux) + torch.randn((size, size), generator=g, device='cuda')
+    a = F.BatchForcedPhotometry(data, inverr, basis, idx, w, kind, pos, shape)
+    fa = a.run(tol=1e-12)
+    cat = F.source_catalog(kind, pos, shape)
+    b = F.BatchForcedPhotometry(data, inverr, basis, torch.from_numpy(idx).cuda(),
+                                torch.from_numpy(w).cuda(), None, None, None, catalog=cat)
+    fb = b.run(tol=1e-12)
+    assert np.array_equal(np.asarray(fa), np.asarray(fb))

@@ -196,6 +196,29 @@ FFT_SIZES = (32, 64, 128, 256, 512)
 B_POINT, B_EMPTY, B_BAD, NBUCKET = 0, 6, 7, 8   # trh_fp_plan's buckets; 1..5 FFT sizes
 
 
+_MIX_DEV = {}
+
+
+def _mixture_table_on(dev):
+    """mixture_table() on the device, uploaded once per device."""
+    t = _MIX_DEV.get(dev)
+    if t is None:
+        t = _MIX_DEV[dev] = torch.from_numpy(mixture_table()).to(dev)
+    return t
+
+
+def source_catalog(kind, pos, shape, device=0):
+    """The (n, 6) float64 device catalogue BatchForcedPhotometry reads (x, y,
+    kind, log r_e, ee1, ee2 per source): build it once and pass it as
+    `catalog` to keep the sources resident across fits."""
+    n = len(kind)
+    src = np.empty((n, 6))
+    src[:, 0:2] = pos
+    src[:, 2] = kind
+    src[:, 3:6] = shape
+    return torch.from_numpy(src).to(torch.device('cuda', device))
+
+
 def mixture_table():
     """The exp and dev profiles as trh_fp_plan's mix rows: K, amplitudes,
     variances (circular, in units of r_e^2)."""
@@ -232,7 +255,7 @@ class BatchForcedPhotometry(object):
     on first use."""
 
     def __init__(self, data, inverr, psf_basis, psf_idx, psf_w, kind, pos, shape, tile=None,
-                 max_block=64, device=0, psfex=None):
+                 max_block=64, device=0, psfex=None, catalog=None):
         self.lib = _lib.load()
         self.h = _lib.handle(device)
         self.dev = torch.device('cuda', device)
@@ -244,7 +267,9 @@ class BatchForcedPhotometry(object):
         self.basis = np.ascontiguousarray(psf_basis, np.float32)
         G, self.ph, self.pw = self.basis.shape
         assert self.ph % 2 == 1 and self.pw % 2 == 1
-        n = len(kind)
+        # the sources: host arrays, or a device catalogue (source_catalog);
+        # psf_idx / psf_w may be device tensors too
+        n = len(kind) if catalog is None else int(catalog.shape[0])
         assert n > 0
         self.n = n
         if tile is None:
@@ -253,16 +278,19 @@ class BatchForcedPhotometry(object):
         self.tile = tile
         self.ntx, self.nty = (self.W + tile - 1) // tile, (self.H + tile - 1) // tile
         self.max_block = max_block
-        src = np.empty((n, 6))
-        src[:, 0:2] = pos
-        src[:, 2] = kind
-        src[:, 3:6] = shape
-        self._src = src
+        if catalog is None:
+            catalog = source_catalog(kind, pos, shape, device)
+        assert catalog.dtype == torch.float64 and catalog.shape == (n, 6)
+        self._d_src = catalog.to(self.dev).contiguous()
         self.psfex = psfex
         if psfex is None:
-            self._pidx = np.ascontiguousarray(psf_idx, np.int32).reshape(n, -1)
-            self._pw = np.ascontiguousarray(psf_w, np.float32).reshape(n, -1)
-            self.nq = self._pw.shape[1]
+            def dev_t(a, dt, npdt):
+                if isinstance(a, torch.Tensor):
+                    return a.to(self.dev, dt).reshape(n, -1).contiguous()
+                return torch.from_numpy(np.ascontiguousarray(a, npdt).reshape(n, -1)).to(self.dev)
+            self._d_pidx0 = dev_t(psf_idx, torch.int32, np.int32)
+            self._d_pw0 = dev_t(psf_w, torch.float32, np.float32)
+            self.nq = self._d_pw0.shape[1]
         else:
             d = int(psfex['degree'])
             self.nq = (d + 1) * (d + 2) // 2
@@ -280,7 +308,7 @@ class BatchForcedPhotometry(object):
         t0 = time.perf_counter()
         lib, h, st, dev, n = self.lib, self.h.value, self.stream, self.dev, self.n
         i32, i64, f64 = torch.int32, torch.int64, torch.float64
-        d_src = self._t(self._src)
+        d_src = self._d_src
         key = torch.empty(n, dtype=i32, device=dev)
         check(lib.trh_fp_keys(h, n, ptr(d_src), self.H, self.W, self.tile, ptr(key), st),
               'fp_keys')
@@ -291,7 +319,7 @@ class BatchForcedPhotometry(object):
         area = torch.empty(n, dtype=i64, device=dev)
         ext = torch.empty(n, dtype=f64, device=dev)
         jobs = torch.empty((n, FFT_JOB.itemsize), dtype=torch.uint8, device=dev)
-        mix = self._t(mixture_table())
+        mix = _mixture_table_on(dev)
         check(lib.trh_fp_plan(h, n, ptr(order), ptr(d_src), ptr(key), self.H, self.W, self.ph,
                               self.pw, ptr(mix), ptr(win), ptr(bucket), ptr(stile), ptr(area),
                               ptr(ext), ptr(jobs), st), 'fp_plan')
@@ -333,8 +361,8 @@ class BatchForcedPhotometry(object):
         self.d_order, self.d_win, self.d_stile, self.d_bucket = order, win, stile, bucket
         self.d_border, self.d_jobs = border, jobs_b
         if self.psfex is None:
-            self.d_pidx = self._t(self._pidx)[order]
-            self.d_pw = self._t(self._pw)[order]
+            self.d_pidx = self._d_pidx0[order]
+            self.d_pw = self._d_pw0[order]
         else:
             pe = self.psfex
             self.d_pw = torch.empty((n, self.nq), dtype=torch.float32, device=dev)
