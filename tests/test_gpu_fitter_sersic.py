@@ -158,3 +158,21 @@ def test_sersic_graph_replay_matches_eager(problem):
     f.iterate_graph(3)
     graph = f.get_params()
     assert np.array_equal(eager, graph)
+
+
+def test_sersic_table_is_validated(problem):
+    """trh_fitter_set_sersic rejects a truncated table and one whose ranges
+    leave a gap, and accepts the real one again."""
+    from tractor_amd._lib import ptr
+    from tractor_amd.sersic import fitter_table
+    f = problem['fitter']
+    tab = fitter_table().copy()
+    assert f.lib.trh_fitter_set_sersic(f.f, ptr(tab), len(tab) - 1) != 0
+    gap = tab.copy()
+    # range 1 starts after range 0 ends: lo of range 1 sits right after range
+    # 0's record (R, then lo, hi, K, m, m + 1 breakpoints, 8 K m coefficients)
+    K0, m0 = int(tab[3]), int(tab[4])
+    r1 = 1 + 4 + (m0 + 1) + 8 * K0 * m0
+    gap[r1] = tab[2] + 0.01
+    assert f.lib.trh_fitter_set_sersic(f.f, ptr(gap), len(gap)) != 0
+    assert f.lib.trh_fitter_set_sersic(f.f, ptr(tab), len(tab)) == 0

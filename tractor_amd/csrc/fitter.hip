@@ -104,7 +104,7 @@ __device__ int ser_range(const SersicTab &T, int r, double n, double *amp, doubl
 // (stable).  amp, var hold TRH_MAXK + 1 entries.  Returns K.
 __device__ int sersic_profile(const SersicTab &T, double n, double *amp, double *var) {
     int r0 = -1, r1 = -1;
-    if (n <= T.lo[0]) {
+    if (!(n > T.lo[0])) {   // also a NaN index: the lowest range
         r0 = 0;
         n = T.lo[0];
     } else if (n >= T.hi[T.nr - 1]) {
@@ -118,6 +118,11 @@ __device__ int sersic_profile(const SersicTab &T, double n, double *amp, double 
                 else if (r1 < 0)
                     r1 = r;
             }
+        // trh_fitter_set_sersic admits no gaps between ranges; keep every
+        // index inside the table regardless
+        if (r0 < 0)
+            for (int r = 0; r < T.nr; ++r)
+                if (T.lo[r] <= n) r0 = r;
     }
     int K;
     if (r1 >= 0) {
@@ -1231,7 +1236,12 @@ extern "C" int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab)
     for (int r = 0; ok && r < T->nr; ++r) {
         T->lo[r] = next();
         T->hi[r] = next();
-        ok = T->lo[r] < T->hi[r] && (r == 0 || T->lo[r] > T->lo[r - 1]) &&
+        // ranges in increasing lo, each starting at or before the previous
+        // one's end (no gaps), at most two covering any index
+        ok = T->lo[r] < T->hi[r] &&
+             (r == 0 || (T->lo[r] > T->lo[r - 1] && T->lo[r] <= T->hi[r - 1] &&
+                         T->hi[r] > T->hi[r - 1])) &&
+             (r < 2 || T->lo[r] >= T->hi[r - 2]) &&
              count(1, TRH_MAXK, T->K[r]) && count(1, 256, T->nint[r]);
         if (!ok) break;
         T->boff[r] = nd;
