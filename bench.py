@@ -15,6 +15,9 @@ normal equations, FP64 solve, line search); value = source-iterations/s.
 spatially varying PSF; one step = plan + render + normal equations + solve;
 value = sources/s.
 
+--workload sersic: the Sersic-galaxy fit that the config-4 line also reports
+(4096 sources, the galaxies Sersic with seven parameters).
+
 --workload c5: the config-4 sources fit from CNN proposals (the CNN is first
 trained on the GPU on rendered cutouts, outside the timed region); one step =
 cutouts + CNN inference + batched fit + fallback; the plain fit of the same
@@ -152,7 +155,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', default='c4', choices=['c4', 'c2', 'c3', 'c5'])
+    ap.add_argument('--workload', default='c4', choices=['c4', 'c2', 'c3', 'c5', 'sersic'])
     ap.add_argument('--cnn-train', type=int, default=1 << 20,
                     help='c5: number of rendered training cutouts')
     ap.add_argument('--cnn-epochs', type=int, default=4)
@@ -189,6 +192,16 @@ def main():
         return main_c5(args, torch)
     if args.workload == 'c3':
         return main_c3(args, torch)
+    if args.workload == 'sersic':
+        leg = sersic_leg(torch, steps=args.steps)
+        print(json.dumps({
+            'metric': 'Sersic-galaxy batched fit (side measurement)', 'value': leg['sources_per_s'],
+            'unit': 'sources/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': 1,
+            'ms_per_step': leg['ms_per_fit'], 'higher_is_better': True, 'scaling': 'weak',
+            'vs_baseline': None, 'dtype': 'f32 (renders, J^T W J) + f64 (Cholesky, chi2)',
+            'data': 'synthetic', 'config': {'workload': leg['workload']}, 'sersic_fit': leg}),
+            flush=True)
+        return
     log('building workload', args.workload)
     wl = build_workload(args.workload, torch)
     f = wl['fitter']

@@ -87,23 +87,28 @@ __device__ __forceinline__ double ser_piece(const double *br, const double *c, i
     return ((q[0] * t + q[1]) * t + q[2]) * t + q[3];
 }
 
+// Every component of range r at n: the components share the range's
+// breakpoints, so the piece is found once.
 __device__ int ser_range(const SersicTab &T, int r, double n, double *amp, double *lvar) {
     const int K = T.K[r], m = T.nint[r];
     const double *br = T.d + T.boff[r];
-    const double *c = T.d + T.coff[r];
+    int i = 0;
+    while (i + 1 < m && n >= br[i + 1]) ++i;
+    const double t = n - br[i];
+    const double *c = T.d + T.coff[r] + 4 * i;
     for (int k = 0; k < K; ++k) {
-        amp[k] = ser_piece(br, c + (size_t)(2 * k) * m * 4, m, n);
-        lvar[k] = ser_piece(br, c + (size_t)(2 * k + 1) * m * 4, m, n);
+        const double *qa = c + (size_t)(2 * k) * m * 4, *ql = qa + (size_t)m * 4;
+        amp[k] = ((qa[0] * t + qa[1]) * t + qa[2]) * t + qa[3];
+        lvar[k] = ((ql[0] * t + ql[1]) * t + ql[2]) * t + ql[3];
     }
     return K;
 }
 
-// SersicMixture._getProfile (sersic.py:286-342): amplitudes normalised per
-// range (a linear ramp of weights where two ranges overlap), variances
-// exp(log-variance), a zero-variance core above n = 1, sorted by variance
-// (stable).  amp, var hold TRH_MAXK + 1 entries.  Returns K.
-__device__ int sersic_profile(const SersicTab &T, double n, double *amp, double *var) {
-    int r0 = -1, r1 = -1;
+// The range(s) holding index n (r1 >= 0 where two overlap), with n clamped
+// to the table.
+__device__ __forceinline__ void ser_ranges(const SersicTab &T, double &n, int &r0, int &r1) {
+    r0 = -1;
+    r1 = -1;
     if (!(n > T.lo[0])) {   // also a NaN index: the lowest range
         r0 = 0;
         n = T.lo[0];
@@ -124,6 +129,22 @@ __device__ int sersic_profile(const SersicTab &T, double n, double *amp, double 
             for (int r = 0; r < T.nr; ++r)
                 if (T.lo[r] <= n) r0 = r;
     }
+}
+
+// Number of components of the mixture at n (no spline evaluated).
+__device__ int sersic_K(const SersicTab &T, double n) {
+    int r0, r1;
+    ser_ranges(T, n, r0, r1);
+    return T.K[r0] + (r1 >= 0 ? T.K[r1] : 0) + (n > 1.0 ? 1 : 0);
+}
+
+// SersicMixture._getProfile (sersic.py:286-342): amplitudes normalised per
+// range (a linear ramp of weights where two ranges overlap), variances
+// exp(log-variance), a zero-variance core above n = 1, sorted by variance
+// (stable).  amp, var hold TRH_MAXK + 1 entries.  Returns K.
+__device__ int sersic_profile(const SersicTab &T, double n, double *amp, double *var) {
+    int r0, r1;
+    ser_ranges(T, n, r0, r1);
     int K;
     if (r1 >= 0) {
         const double frac = (n - T.lo[r1]) / (T.hi[r0] - T.lo[r1]);
@@ -172,14 +193,12 @@ __device__ int sersic_profile(const SersicTab &T, double n, double *amp, double 
 // different number of components, the opposite step when it stays inside
 // the bounds and keeps the count.
 __device__ double sersic_fd_step(const SersicTab &T, double n0) {
-    double amp[TRH_MAXK + 1], var[TRH_MAXK + 1];
     double step = SER_STEP;
     if (n0 + step > SER_UPPER) step = -step;
-    const int K0 = sersic_profile(T, n0, amp, var);
-    if (sersic_profile(T, n0 + step, amp, var) != K0) {
+    const int K0 = sersic_K(T, n0);
+    if (sersic_K(T, n0 + step) != K0) {
         const double n2 = n0 - step;
-        if (n2 < SER_UPPER && n2 > SER_LOWER && sersic_profile(T, n2, amp, var) == K0)
-            step = -step;
+        if (n2 < SER_UPPER && n2 > SER_LOWER && sersic_K(T, n2) == K0) step = -step;
     }
     return step;
 }
