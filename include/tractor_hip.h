@@ -478,7 +478,10 @@ int trh_fitter_get_params(trh_fitter *f, double *params, void *stream);
  *   then the core fraction: m, b[0..m], m pieces of 4 coefficients.
  * Indices outside [lo of the first range, hi of the last] clamp to it; a
  * zero-variance core carries the core fraction above n = 1.  At most 8
- * ranges.  tractor_amd.sersic.fitter_table() builds it.
+ * ranges, in increasing lo, each starting at or before the previous one's
+ * end (no gaps) and at most two covering any index; a table that breaks
+ * these rules, or whose length does not match its counts, is rejected
+ * (TRH_E_ARG).  tractor_amd.sersic.fitter_table() builds it.
  */
 int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab);
 
