@@ -103,7 +103,16 @@ typedef struct trh_mog_approx_job {
     double  minval;          /* growth stops below this value                  */
 } trh_mog_approx_job;
 
-/* Library handle: per-device twiddle tables and scratch. */
+/*
+ * Library handle: per-device twiddle tables and scratch.  The scratch and
+ * the row slots of the sz >= 256 FFT renders belong to the handle: the entry
+ * points that use them (trh_render_fft_batch at sz >= 256,
+ * trh_lanczos_shift_batch, trh_solve_normal_f64, trh_fp_imstats) order a
+ * call on one stream after the previous call on another, so they may be
+ * mixed across streams; a stream being captured into a graph is not
+ * ordered, so a graph that uses them must be the handle's only user while it
+ * runs.  Fitters (trh_fitter_*) keep row slots of their own.
+ */
 typedef struct trh_handle trh_handle;
 
 const char *trh_last_error(void);
@@ -200,10 +209,11 @@ int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n, i
  * source in the reference).  Job fields as trh_fft_job with: (sx, sy) = window
  * coordinates of the stamp's pixel (0,0); (mux, muy) = the sub-pixel shift
  * px - round(px); psf selects the stamp at stamp + psf * ph * pw; K, amp,
- * var unused.  stamp: float32 [nstamp][ph][pw]; maxw >= every job's mw.
+ * var unused.  stamp: float32 [nstamp][ph][pw]; windows of any size (the
+ * LDS holds the stamp and pw + 6 filtered columns).
  * With out NULL and chi2, data, inverr given, every job is a TRH_CHI2 job.
  */
-int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, int maxw,
+int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs,
                            const float *stamp, int ph, int pw, float *out, const float *data,
                            const float *inverr, double *chi2, void *stream);
 
@@ -451,13 +461,19 @@ typedef struct trh_fitter trh_fitter;
  *   P[S]         square modelMask size (patch is (0,0,P,P) in patch pixels)
  *   psf_stamp    float32 [ph][pw] PixelizedPSF stamp shared by all sources
  *   cdi[4]       position-derivative matrix (ducks.py:413-423): row-major 2x2
+ *   row_stride[S] row stride of each source's window in data / inverr (>= P;
+ *                NULL: P, the patches back to back)
  * Device arrays (caller-owned, must outlive the fitter):
- *   data, inverr float32, patch s at pix_off[s] (host int64 array), P*P row-major
+ *   data, inverr float32; source s reads the P x P window at pix_off[s]
+ *                (host int64 array) with row stride row_stride[s], so
+ *                sources may share one image (overlapping windows, the
+ *                modelMask of each source in a tile, tractor/engine.py:
+ *                341-370) or own contiguous patches
  */
 int trh_fitter_create(trh_handle *h, trh_fitter **f, int S, const int32_t *kind,
-                      const int32_t *P, const int64_t *pix_off, const float *data,
-                      const float *inverr, const float *psf_stamp, int ph, int pw,
-                      const double *cdi);
+                      const int32_t *P, const int64_t *pix_off, const int32_t *row_stride,
+                      const float *data, const float *inverr, const float *psf_stamp, int ph,
+                      int pw, const double *cdi);
 int trh_fitter_destroy(trh_fitter *f);
 
 /* Copy parameters in/out: host double [S][TRH_FIT_NPARAM] = (x, y, flux,
@@ -491,28 +507,34 @@ int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab);
  * `lambda` on the column-scaled normal equations (lambda=0: Gauss-Newton, the
  * reference's direction), batched line search over the reference's alphas
  * with its acceptance rule).  Sources whose step satisfies the optimize_loop
- * stopping rule (constrained_optimizer.py:12-34) with `dchisq` become
+ * stopping rule (constrained_optimizer.py:12-34) with `dchisq`, or that
+ * have taken max_iter iterations (optimize_loop's `steps`), become
  * inactive.  Enqueued on `stream`.
  */
-int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream);
+int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, int max_iter,
+                       void *stream);
 
 /*
  * `n_iter` LM iterations replayed from a hipGraph that captures one
- * iteration's launches (captured on first use, and again when lambda or
- * dchisq change), on a library-owned stream ordered after and before
- * `stream`.  Same results as calling trh_fitter_iterate n_iter times; not
- * available while per-launch profiling is on.  No reference counterpart
+ * iteration's launches (captured on first use, and again when lambda,
+ * dchisq or max_iter change), on a library-owned stream ordered after and
+ * before `stream`.  Same results as calling trh_fitter_iterate n_iter times;
+ * not available while per-launch profiling is on.  No reference counterpart
  * (launch scheduling).
  */
-int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int n_iter,
-                             void *stream);
+int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int max_iter,
+                             int n_iter, void *stream);
 
 /*
  * Render every source's model (counts x unit-flux patch at the current
  * parameters, as getModelPatch, tractor/pointsource.py:36-60) into `out`,
- * laid out like `data` (patch s at pix_off[s], P*P).  Enqueued on `stream`.
+ * laid out like `data` (window s at pix_off[s], row stride row_stride[s]):
+ * stored, or with `accumulate` added (atomically, so overlapping windows
+ * sum: Tractor.getModelImage, tractor/engine.py:424-445).  A source whose
+ * model needs an FFT size outside 32..512 is not written and gets
+ * TRH_FIT_F_MODEL_SIZE.  Enqueued on `stream`.
  */
-int trh_fitter_render_models(trh_fitter *f, float *out, void *stream);
+int trh_fitter_render_models(trh_fitter *f, float *out, int accumulate, void *stream);
 
 /*
  * Per-source status, host arrays (NULL to skip): active[S] (int32),
@@ -522,6 +544,26 @@ int trh_fitter_render_models(trh_fitter *f, float *out, void *stream);
  */
 int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations, double *chisq,
                       double *dlnp, int32_t *n_active, void *stream);
+
+/*
+ * Per-source flags (host int32 [S]; synchronises `stream`), reset by
+ * trh_fitter_set_params:
+ *   TRH_FIT_F_JAC_SIZE    a Jacobian render needed an FFT size outside
+ *                         32..512: the source stopped at its parameters then
+ *   TRH_FIT_F_LS_SIZE     a line-search trial did: it counted as a
+ *                         non-finite log-probability (the walk over the
+ *                         steps ends there, tryUpdates' rule)
+ *   TRH_FIT_F_MAXITER     stopped at max_iter without meeting the stopping rule
+ *   TRH_FIT_F_LIMITED     the last iteration was step-limited (its stop test
+ *                         used dchisq_limited = 1e-6)
+ *   TRH_FIT_F_MODEL_SIZE  trh_fitter_render_models could not render it
+ */
+#define TRH_FIT_F_JAC_SIZE    1
+#define TRH_FIT_F_LS_SIZE     2
+#define TRH_FIT_F_MAXITER     4
+#define TRH_FIT_F_LIMITED     8
+#define TRH_FIT_F_MODEL_SIZE 16
+int trh_fitter_flags(trh_fitter *f, int32_t *flags, void *stream);
 
 /* Device pointer to the fitter's Jacobian renders (float32, per source
  * 1 + n_shape (+1 Sersic index) planes of P*P at render_off[s]) and normal

@@ -23,7 +23,7 @@ __all__ = [
     'mog_convolve', 'render_galaxy_mog', 'mog_grid_approx3', 'mog_psf_point_patch', 'forced_unit_patch', 'forced_phot_lsq',
     'forced_fitstats', 'psfex_terms', 'psfex_at', 'em_fit_2d_reg', 'em_init_params',
     'pixpsf_point_patch', 'galaxy_param_derivs', 'pointsource_param_derivs',
-    'column_scaled_lstsq', 'parameter_steps', 'accept_line_search',
+    'column_scaled_lstsq', 'parameter_steps', 'accept_line_search', 'loop_should_stop',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
     'source_chisq', 'fit_source', 'ALPHAS', 'round_half_even', 'sersic_mixture',
     'sersic_fd_step',
@@ -689,9 +689,14 @@ def render_galaxy_fft(amp, var_pix, px, py, psf_img, mm=None, nomask_extent=None
         pamp, pmean, pvar = hybrid_mog
         gh, gw = G.shape
         mamp, mvar = mogmix
-        G = G + render_galaxy_mog(mamp, mvar, px, py, pamp, pmean, pvar,
-                                  ix0, ix0 + gw, iy0, iy0 + gh,
-                                  mask=np.ones((gh, gw), bool))
+        # run_mog on the modelMask, or on ModelMask(ix0, iy0, gw, gh) without
+        # one (galaxy.py:510-516): the masked FP32 evaluator only when the
+        # mask has a pixel map, else the FP64 grid (galaxy.py:326-332); the
+        # sum lands in the float32 G (G += patch)
+        m = render_galaxy_mog(mamp, mvar, px, py, pamp, pmean, pvar,
+                              ix0, ix0 + gw, iy0, iy0 + gh,
+                              mask=mask if mm is not None else None)
+        G = (G + m).astype(np.float32)
     return (ix0, iy0, G)
 
 
@@ -804,9 +809,13 @@ def column_scaled_lstsq(A, b, use_lsqr=False, damp=0.):
 ALPHAS = np.append(2. ** np.arange(-10, 1), [np.sqrt(2.), 2.])
 
 
-def parameter_steps(p0, X, maxstep, lowers, uppers, alphas=None, tiny_alpha=1e-8):
+def parameter_steps(p0, X, maxstep, lowers, uppers, alphas=None, tiny_alpha=1e-8,
+                    flags=None):
     """ConstrainedOptimizer.getParameterSteps (constrained_optimizer.py:84-159).
-    Returns [(alpha, p, step_limit, hit_limit), ...]."""
+    Returns [(alpha, p, step_limit, hit_limit), ...].  ``flags`` (a dict, if
+    given) gets 'step_limited' = True wherever the reference sets
+    self.stepLimited (line 121: any max-step clamp, including one whose alpha
+    then falls below tiny_alpha and is not returned)."""
     if alphas is None:
         alphas = ALPHAS
     X = np.asarray(X, np.float64)
@@ -822,6 +831,8 @@ def parameter_steps(p0, X, maxstep, lowers, uppers, alphas=None, tiny_alpha=1e-8
                 step_limit = True
                 do_break = True
                 alpha = mx / abs(X[i])
+                if flags is not None:
+                    flags['step_limited'] = True
         hit_limit = False
         for i, (ps, l, u) in enumerate(zip(p0, lowers, uppers)):
             px = ps + alpha * X[i]
@@ -867,6 +878,16 @@ def accept_line_search(lnp0, steps, lnps):
     if ibest is None:
         return None, 0., 0., False
     return ibest, best - lnp0, steps[ibest][0], steps[ibest][3]
+
+
+def loop_should_stop(step_limited, dlnp, dchisq=0., dchisq_limited=1e-6):
+    """The stopping rule of ConstrainedOptimizer.optimize_loop
+    (constrained_optimizer.py:28-31): a step-limited iteration (a max-step
+    clamp, or no step to try at all, line 52-55) stops only when it gained at
+    most dchisq_limited; any other stops when it gained at most dchisq."""
+    if step_limited:
+        return dlnp <= dchisq_limited
+    return dlnp <= dchisq
 
 
 # ---------------------------------------------------------------------------
@@ -1069,8 +1090,11 @@ def fit_source(prob, params0, dchisq=0., steps=100, dchisq_limited=1e-6, alphas=
         if X is None:
             break
         lnp0 = -0.5 * source_chisq(prob, params, p0patch)
-        stps = parameter_steps(params, X, prob.maxstep(), lowers, uppers, alphas)
-        step_limited = any(s[2] for s in stps) or len(stps) == 0
+        flags = {'step_limited': False}
+        stps = parameter_steps(params, X, prob.maxstep(), lowers, uppers, alphas, flags=flags)
+        # tryUpdates marks an iteration with nothing to try as step-limited
+        # (constrained_optimizer.py:52-55)
+        step_limited = flags['step_limited'] or len(stps) == 0
         lnps = []
         # the reference evaluates sequentially and stops early; evaluating
         # every step and applying the same rule gives the same choice.
@@ -1079,9 +1103,7 @@ def fit_source(prob, params0, dchisq=0., steps=100, dchisq_limited=1e-6, alphas=
         ib, dlnp, alpha, hl = accept_line_search(lnp0, stps, lnps)
         if ib is not None:
             params = np.array(stps[ib][1], np.float64)
-        if not step_limited and dlnp <= dchisq:
-            break
-        if step_limited and dlnp <= dchisq_limited:
+        if loop_should_stop(step_limited, dlnp, dchisq, dchisq_limited):
             break
     return params, it + 1, source_chisq(prob, params)
 

@@ -60,7 +60,7 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_render_mog_batch': (N, N, 1, 1) + (N,) * 11,
         'trh_render_mog_approx3_batch': (N, N, 1, 10) + (N,) * 8,
         'trh_lanczos_shift_batch': (N, N, N, 1, 4, 4, N, N, N),
-        'trh_render_point_batch': (N, N, 1, 4, N, 3, 3, N, N, N, N, N),
+        'trh_render_point_batch': (N, N, 1, N, 3, 3, N, N, N, N, N),
         'trh_normal_eq_dense': (N, N, 10, 3, N, N),
         'trh_solve_normal_f64': (N, N, 3, 0., 0, N, N, N),
         'trh_set_fft_config': (N, 64, 0, 0),
@@ -88,8 +88,9 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_cnn_adam': (N, N, N, N, N, 10, 1e-3, 0.9, 0.999, 1e-8, 0., N),
         'trh_cnn_pack': (N, N, 1, N, N, N),
         'trh_cnn_cutouts': (N, N, N, N, 1, N, N, N),
-        'trh_fitter_iterate': (N, 0., 0., N),
-        'trh_fitter_iterate_graph': (N, 0., 0., 1, N),
+        'trh_fitter_iterate': (N, 0., 0., 100, N),
+        'trh_fitter_iterate_graph': (N, 0., 0., 100, 1, N),
+        'trh_fitter_flags': (N, N, N),
     }
     assert set(calls) <= set(_lib.SIGNATURES)
     for name, args in calls.items():

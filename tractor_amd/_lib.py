@@ -14,6 +14,9 @@ LIB_PATH = os.path.join(HERE, 'libtractor_hip.so')
 
 TRH_STORE, TRH_ADD, TRH_CHI2 = 0, 1, 2
 TRH_MAXK = 16
+# per-source flags of the batched fitter (trh_fitter_flags)
+TRH_FIT_F_JAC_SIZE, TRH_FIT_F_LS_SIZE, TRH_FIT_F_MAXITER = 1, 2, 4
+TRH_FIT_F_LIMITED, TRH_FIT_F_MODEL_SIZE = 8, 16
 
 # struct trh_fft_job (include/tractor_hip.h)
 FFT_JOB = np.dtype([
@@ -63,7 +66,7 @@ SIGNATURES = {
     'trh_render_mog_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_render_mog_approx3_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
-    'trh_render_point_batch': (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
+    'trh_render_point_batch': (_I, [_P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     'trh_psf_combine': (_I, [_P, _P, _I, ctypes.c_int64, _P, _P, _I, _P, _I, _P]),
     'trh_psfex_terms': (_I, [_P, _I, _P, _P, _I, _D, _D, _D, _D, _P, _P, _P]),
     'trh_fp_keys': (_I, [_P, _I, _P, _I, _I, _I, _P, _P]),
@@ -89,15 +92,16 @@ SIGNATURES = {
                           ctypes.c_float, ctypes.c_float, _P]),
     'trh_cnn_pack': (_I, [_P, _P, _I, _P, _P, _P]),
     'trh_cnn_cutouts': (_I, [_P, _P, _P, _P, _I, _P, _P, _P]),
-    'trh_fitter_create': (_I, [_P, _PP, _I, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
+    'trh_fitter_create': (_I, [_P, _PP, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P]),
     'trh_fitter_destroy': (_I, [_P]),
     'trh_fitter_set_params': (_I, [_P, _P, _P]),
     'trh_fitter_get_params': (_I, [_P, _P, _P]),
     'trh_fitter_set_sersic': (_I, [_P, _P, _I]),
-    'trh_fitter_iterate': (_I, [_P, _D, _D, _P]),
-    'trh_fitter_iterate_graph': (_I, [_P, _D, _D, _I, _P]),
-    'trh_fitter_render_models': (_I, [_P, _P, _P]),
+    'trh_fitter_iterate': (_I, [_P, _D, _D, _I, _P]),
+    'trh_fitter_iterate_graph': (_I, [_P, _D, _D, _I, _I, _P]),
+    'trh_fitter_render_models': (_I, [_P, _P, _I, _P]),
     'trh_fitter_status': (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    'trh_fitter_flags': (_I, [_P, _P, _P]),
     'trh_fitter_debug': (_I, [_P, _PP, _PP, _PP, _PP]),
     'trh_fitter_timing': (_I, [_P, _P]),
     'trh_fitter_profile': (_I, [_P, _I]),

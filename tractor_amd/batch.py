@@ -18,21 +18,29 @@ PIXEL_CDI = np.eye(2)             # position parameters in pixels
 
 
 class BatchFitter(object):
-    """S sources, each on its own P x P patch of `data` / `inverr`
-    (device float32 tensors, patch s at pix_off[s]).
+    """S sources, each fit on its own P x P window of `data` / `inverr`
+    (device float32 tensors): window s starts at pix_off[s] with row stride
+    row_stride[s] (default P: patches back to back; an image's width: the
+    sources' modelMasks in one shared image).
 
     kind: 0 PointSource, 1 ExpGalaxy, 2 DevGalaxy, 3 SersicGalaxy;  params
     (S, 6): x, y, flux, logre, ee1, ee2 (EllipseESoft) in patch pixels, and a
     seventh column, the Sersic index, when there are Sersic galaxies.
     cdi: 2x2 position-derivative matrix (tractor/ducks.py:413-423).
     """
-    def __init__(self, kind, P, pix_off, data, inverr, psf_stamp, cdi=PIXEL_CDI, device=0):
+    def __init__(self, kind, P, pix_off, data, inverr, psf_stamp, cdi=PIXEL_CDI, device=0,
+                 row_stride=None):
         self.lib = _lib.load()
         self.h = _lib.handle(device)
         self.S = len(kind)
         self.kind = np.ascontiguousarray(kind, np.int32)
         self.P = np.ascontiguousarray(P, np.int32)
         self.pix_off = np.ascontiguousarray(pix_off, np.int64)
+        if row_stride is None:
+            row_stride = self.P
+        self.row_stride = np.ascontiguousarray(np.broadcast_to(row_stride, self.P.shape), np.int32)
+        if np.any(self.row_stride < self.P):
+            raise ValueError('row_stride must be >= P')
         self.data = data
         self.inverr = inverr
         psf = np.ascontiguousarray(psf_stamp, np.float32)
@@ -40,13 +48,15 @@ class BatchFitter(object):
         cdi = np.ascontiguousarray(np.asarray(cdi, np.float64).ravel())
         assert data.dtype == torch.float32 and inverr.dtype == torch.float32
         assert data.is_cuda and inverr.is_cuda
-        need = int(np.max(self.pix_off + self.P.astype(np.int64) ** 2))
+        P64 = self.P.astype(np.int64)
+        need = int(np.max(self.pix_off + (P64 - 1) * self.row_stride + P64))
         if data.numel() < need or inverr.numel() < need:
             raise ValueError('data/inverr hold %d/%d pixels, patches need %d' % (
                 data.numel(), inverr.numel(), need))
         f = ctypes.c_void_p()
         check(self.lib.trh_fitter_create(self.h.value, ctypes.byref(f), self.S, ptr(self.kind),
-                                         ptr(self.P), ptr(self.pix_off), ptr(data), ptr(inverr),
+                                         ptr(self.P), ptr(self.pix_off), ptr(self.row_stride),
+                                         ptr(data), ptr(inverr),
                                          ptr(psf), psf.shape[0], psf.shape[1], ptr(cdi)),
               'trh_fitter_create')
         self.f = f
@@ -81,17 +91,27 @@ class BatchFitter(object):
         check(self.lib.trh_fitter_get_params(self.f, ptr(p), self.stream), 'get_params')
         return p[:, :self.npar].copy()
 
-    def render_models(self, out):
-        check(self.lib.trh_fitter_render_models(self.f, ptr(out), self.stream), 'render_models')
+    def render_models(self, out, accumulate=False):
+        """Models at the current parameters into `out` (laid out like data):
+        stored, or added (overlapping windows sum)."""
+        check(self.lib.trh_fitter_render_models(self.f, ptr(out), 1 if accumulate else 0,
+                                                self.stream), 'render_models')
 
-    def iterate(self, lam=0., dchisq=0.):
-        check(self.lib.trh_fitter_iterate(self.f, float(lam), float(dchisq), self.stream),
-              'iterate')
+    def iterate(self, lam=0., dchisq=0., max_iter=100):
+        check(self.lib.trh_fitter_iterate(self.f, float(lam), float(dchisq), int(max_iter),
+                                          self.stream), 'iterate')
 
-    def iterate_graph(self, n, lam=0., dchisq=0.):
+    def iterate_graph(self, n, lam=0., dchisq=0., max_iter=100):
         """n iterations replayed from a captured hipGraph of one iteration."""
-        check(self.lib.trh_fitter_iterate_graph(self.f, float(lam), float(dchisq), int(n),
-                                                self.stream), 'iterate_graph')
+        check(self.lib.trh_fitter_iterate_graph(self.f, float(lam), float(dchisq),
+                                                int(max_iter), int(n), self.stream),
+              'iterate_graph')
+
+    def flags(self):
+        """Per-source TRH_FIT_F_* bits (see include/tractor_hip.h)."""
+        fl = np.zeros(self.S, np.int32)
+        check(self.lib.trh_fitter_flags(self.f, ptr(fl), self.stream), 'flags')
+        return fl
 
     def status(self):
         active = np.zeros(self.S, np.int32)
@@ -130,14 +150,16 @@ class BatchFitter(object):
         time; graph=False launches eagerly (default check every iteration)."""
         if check_every is None:
             check_every = 4 if graph else 1
+        if max_iter < 1:
+            raise ValueError('max_iter must be >= 1')
         it = 0
         while it < max_iter:
             k = min(check_every, max_iter - it)
             if graph:
-                self.iterate_graph(k, lam, dchisq)
+                self.iterate_graph(k, lam, dchisq, max_iter)
             else:
                 for _ in range(k):
-                    self.iterate(lam, dchisq)
+                    self.iterate(lam, dchisq, max_iter)
             it += k
             if self.n_active() == 0:
                 break

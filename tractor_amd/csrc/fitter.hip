@@ -208,13 +208,13 @@ struct trh_fitter {
     int S = 0, n_gal = 0, n_pt = 0, ph = 0, pw = 0;
     double cdi[4] = {1, 0, 0, 1};
     int maxrows[NFFT] = {8, 8, 8, 8, 8};
-    int maxP_pt = 1;
     // smallest FFT size any galaxy render can use: halfsize >= max(pw, ph) / 2
     // (galaxy.py:337-351), so the buckets below it never hold a job
     int min_sz = 32;
     int cx[NFFT] = {0, 0, 0, 0, 0}, cy[NFFT] = {0, 0, 0, 0, 0};
     // device
     int32_t *kind = nullptr, *P = nullptr;
+    int32_t *stride = nullptr;   // row stride of each source's data / inverr window
     int64_t *pix_off = nullptr, *render_off = nullptr;
     const float *data = nullptr, *inverr = nullptr;
     float *stamp = nullptr;
@@ -228,8 +228,8 @@ struct trh_fitter {
     double *sstep = nullptr;
     double *params = nullptr, *trial = nullptr, *X = nullptr, *neq = nullptr, *chi2ls = nullptr;
     int32_t *nsteps = nullptr, *steplim = nullptr, *active = nullptr, *iters = nullptr;
+    int32_t *flags = nullptr;    // TRH_FIT_F_* per source
     double *chisq = nullptr, *dlnp = nullptr;
-    int32_t *err = nullptr;
     FitProfiles *prof = nullptr;
     trh_fft_job *jobs_jac = nullptr, *jobs_ls = nullptr;
     int32_t *list_jac = nullptr, *list_ls = nullptr, *cnt = nullptr;  // cnt[0..4] jac, [8..12] ls
@@ -255,6 +255,10 @@ struct trh_fitter {
     hipEvent_t gev[2];
     bool have_gev = false;
     double g_lambda = 0.0, g_dchisq = 0.0;
+    int g_max_iter = 0;
+    // the fitter's own row slots of the SZ >= 256 renders (sizes 256, 512),
+    // so that fitters on different streams never share them
+    float2 *hbuf[2] = {nullptr, nullptr};
     std::vector<void *> allocs;
 };
 
@@ -455,11 +459,12 @@ __device__ __forceinline__ int jac_planes(int kd) { return kd == 0 ? 1 : (kd == 
 __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const int32_t *__restrict__ kind,
                                const int32_t *__restrict__ Parr, const int64_t *__restrict__ roff,
                                const double *__restrict__ params,
-                               const int32_t *__restrict__ active, const FitProfiles *__restrict__ pr,
+                               const int32_t *active, const FitProfiles *__restrict__ pr,
                                const SersicTab *__restrict__ ser, double *__restrict__ sstep,
                                int ph, int pw, trh_fft_job *__restrict__ jobs,
                                int8_t *__restrict__ bk, int32_t *__restrict__ bcnt,
-                               int32_t *__restrict__ err, unsigned long long *__restrict__ acc) {
+                               int32_t *__restrict__ flags, int32_t *__restrict__ act_out,
+                               unsigned long long *__restrict__ acc) {
     __shared__ trh_fft_job sj[PREP_TB];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = t / nj, k = t - s * nj;
@@ -498,7 +503,12 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
             const int sz = build_gal_job(J, amp, var, K, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
             J.out_offset = roff[s] + (int64_t)k * P * P;
             b = sz_bucket(sz);
-            if (b < 0) atomicExch(err, sz);
+            if (b < 0) {
+                // no Jacobian for this source: it stops at its current
+                // parameters, flagged (the other sources go on)
+                atomicOr(&flags[s], TRH_FIT_F_JAC_SIZE);
+                act_out[s] = 0;
+            }
         }
         J.mode = TRH_STORE;
         J.scale = 1.f;
@@ -521,12 +531,14 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
 // image buffer laid out like `data` (patch s at pix_off[s]).
 __global__ void k_fit_prep_model(int S, int nj, const int32_t *__restrict__ kind,
                                  const int32_t *__restrict__ Parr,
+                                 const int32_t *__restrict__ stride, int mode,
                                  const int64_t *__restrict__ pix_off,
                                  const double *__restrict__ params,
                                  const FitProfiles *__restrict__ pr,
                                  const SersicTab *__restrict__ ser, int ph, int pw,
                                  trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
-                                 int cap, int32_t *__restrict__ cnt, int32_t *__restrict__ err) {
+                                 int cap, int32_t *__restrict__ cnt,
+                                 int32_t *__restrict__ flags) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const int kd = kind[s];
@@ -544,12 +556,13 @@ __global__ void k_fit_prep_model(int S, int nj, const int32_t *__restrict__ kind
         const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
         b = sz_bucket(sz);
         if (b < 0) {
-            atomicExch(err, sz);
+            atomicOr(&flags[s], TRH_FIT_F_MODEL_SIZE);
             return;
         }
     }
     J.out_offset = pix_off[s];
-    J.mode = TRH_STORE;
+    J.out_stride = stride[s];
+    J.mode = mode;
     J.scale = (float)p[2];
     J.slot = 0;
     const int i = atomicAdd(&cnt[b], 1);
@@ -610,7 +623,7 @@ template <int NC>
 struct NeqPix {
     static constexpr int NT = NC * (NC + 1) / 2, NQ = NT + NC;
     const float *R0, *D, *E;
-    int P, PP, kd;
+    int P, PP, kd, ds;   // ds: row stride of the data / inverr window
     float invP, counts, cstep, istep, c00, c01, c10, c11;
     __device__ __forceinline__ void load(int idx, NeqIn<NC> &in) const {
         // exact for PP < 2^24: (idx + 0.5) / P is within 0.5/P of y + 0.5
@@ -624,8 +637,9 @@ struct NeqPix {
         in.dy = (u - w) / 2.f;
 #pragma unroll
         for (int k = 0; k < NC - 3; ++k) in.g[k] = kd ? R0[(k + 1) * PP + idx] : 0.f;
-        in.e = E[idx];
-        in.d = D[idx];
+        const int di = y * ds + x;
+        in.e = E[di];
+        in.d = D[di];
     }
     __device__ __forceinline__ void add(const NeqIn<NC> &in, float (&acc)[NQ], double &chi2,
                                         double &chi0) const {
@@ -669,6 +683,7 @@ struct NeqPix {
 template <int NC>
 __global__ void __launch_bounds__(NEQ_TB)
 k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restrict__ Parr,
+                const int32_t *__restrict__ stride,
                 const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
                 const float *__restrict__ data, const float *__restrict__ inverr,
                 const float *__restrict__ renders, const double *__restrict__ params,
@@ -686,6 +701,7 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
         px.kd = kind[s];
         px.P = Parr[s];
         px.PP = px.P * px.P;
+        px.ds = stride[s];
         px.invP = 1.0f / (float)px.P;
         px.counts = (float)params[8 * s + 2];
         px.cstep = (float)(params[8 * s + 2] / 0.01);
@@ -911,6 +927,7 @@ __global__ void __launch_bounds__(64) k_fit_solve(int S, const int32_t *__restri
 // One thread per (source, trial step) of the line search.
 __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *__restrict__ kind,
                               const int32_t *__restrict__ Parr,
+                              const int32_t *__restrict__ stride,
                               const int64_t *__restrict__ pix_off,
                               const double *__restrict__ trial,
                               const int32_t *__restrict__ nsteps,
@@ -918,7 +935,8 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
                               const FitProfiles *__restrict__ pr,
                               const SersicTab *__restrict__ ser, int ph, int pw,
                               trh_fft_job *__restrict__ jobs, int8_t *__restrict__ bk,
-                              int32_t *__restrict__ bcnt, int32_t *__restrict__ err,
+                              int32_t *__restrict__ bcnt, int32_t *__restrict__ flags,
+                              double *__restrict__ chi2ls,
                               unsigned long long *__restrict__ acc) {
     __shared__ trh_fft_job sj[PREP_TB];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -939,9 +957,15 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
             const int K = gal_profile(*pr, ser, kd, p[6], amp, var, sa, sv);
             const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
             b = sz_bucket(sz);
-            if (b < 0) atomicExch(err, sz);
+            if (b < 0) {
+                // not rendered: the trial counts as non-finite, which ends
+                // the walk over the steps there (tryUpdates' isfinite break)
+                chi2ls[slot] = INFINITY;
+                atomicOr(&flags[s], TRH_FIT_F_LS_SIZE);
+            }
         }
         J.out_offset = pix_off[s];
+        J.out_stride = stride[s];
         J.mode = TRH_CHI2;
         J.scale = (float)p[2];
         J.slot = slot;
@@ -976,7 +1000,7 @@ __global__ void k_fit_accept(int S, const double *__restrict__ neq,
                              const double *__restrict__ chi2ls, double *__restrict__ params,
                              int32_t *__restrict__ active, int32_t *__restrict__ iters,
                              double *__restrict__ chisq, double *__restrict__ dlnp_out,
-                             double dchisq, int max_iter) {
+                             int32_t *__restrict__ flags, double dchisq, int max_iter) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S || !active[s]) return;
     const double chi0 = neq[(size_t)NEQ_W * s + NEQ_CHI2];
@@ -1006,8 +1030,12 @@ __global__ void k_fit_accept(int S, const double *__restrict__ neq,
     const int it = iters[s] + 1;
     iters[s] = it;
     const bool limited = steplim[s] || n == 0;
-    const bool done = (!limited && dl <= dchisq) || (limited && dl <= 1e-6) || it >= max_iter;
-    if (done) active[s] = 0;
+    const bool rule = (!limited && dl <= dchisq) || (limited && dl <= 1e-6);
+    int fl = flags[s] & ~(TRH_FIT_F_LIMITED | TRH_FIT_F_MAXITER);
+    if (limited) fl |= TRH_FIT_F_LIMITED;
+    if (!rule && it >= max_iter) fl |= TRH_FIT_F_MAXITER;
+    flags[s] = fl;
+    if (rule || it >= max_iter) active[s] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1078,7 +1106,8 @@ extern "C" int trh_fitter_destroy(trh_fitter *f) {
 }
 
 extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const int32_t *kind,
-                                 const int32_t *P, const int64_t *pix_off, const float *data,
+                                 const int32_t *P, const int64_t *pix_off,
+                                 const int32_t *row_stride, const float *data,
                                  const float *inverr, const float *psf_stamp, int ph, int pw,
                                  const double *cdi) {
     if (!h || !fp || S <= 0 || !kind || !P || !pix_off || !data || !inverr || !psf_stamp)
@@ -1100,10 +1129,12 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
     int64_t tot = 0;
     int maxP_sz[NFFT] = {0, 0, 0, 0, 0};
     for (int s = 0; s < S; ++s) {
-        if (kind[s] < 0 || kind[s] > 3 || P[s] < 3 || P[s] > 4096) {
+        if (kind[s] < 0 || kind[s] > 3 || P[s] < 3 || P[s] > 4096 ||
+            (row_stride && row_stride[s] < P[s]) || pix_off[s] < 0) {
             trh_fitter_destroy(f);
-            return set_err(TRH_E_ARG, "trh_fitter_create: source %d has kind %d / P %d", s, kind[s],
-                           P[s]);
+            return set_err(TRH_E_ARG,
+                           "trh_fitter_create: source %d has kind %d / P %d / row stride %d", s,
+                           kind[s], P[s], row_stride ? row_stride[s] : P[s]);
         }
         roff[s] = tot;
         tot += (int64_t)(kind[s] == 3 ? 5 : (kind[s] ? 4 : 1)) * P[s] * P[s];
@@ -1114,7 +1145,6 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
                 if (P[s] < kSizes[b] && P[s] > maxP_sz[b]) maxP_sz[b] = P[s];
         } else {
             ++f->n_pt;
-            if (P[s] > f->maxP_pt) f->maxP_pt = P[s];
         }
     }
     {
@@ -1127,13 +1157,14 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         if (mr < 8) mr = 8;
         f->maxrows[b] = mr;
     }
-    if (f->n_pt && point_lds_bytes(ph, pw, f->maxP_pt) + sizeof(RenderScratch) > (size_t)h->lds_max) {
+    if (f->n_pt && point_lds_bytes(ph, pw) + sizeof(RenderScratch) > (size_t)h->lds_max) {
         trh_fitter_destroy(f);
-        return set_err(TRH_E_UNSUPPORTED, "point-source patch width %d too large for LDS", f->maxP_pt);
+        return set_err(TRH_E_UNSUPPORTED, "point-source PSF stamp %dx%d too large for LDS", ph, pw);
     }
     f->nj = f->n_ser ? 5 : 4;
     const int cap_jac = f->nj * S, cap_ls = NALPHA * S;
     if ((rc = falloc(f, &f->kind, S)) || (rc = falloc(f, &f->P, S)) ||
+        (rc = falloc(f, &f->stride, S)) || (rc = falloc(f, &f->flags, S)) ||
         (rc = falloc(f, &f->sstep, S)) ||
         (rc = falloc(f, &f->pix_off, S)) || (rc = falloc(f, &f->render_off, S)) ||
         (rc = falloc(f, &f->stamp, (size_t)ph * pw)) || (rc = falloc(f, &f->renders, tot)) ||
@@ -1143,7 +1174,7 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         (rc = falloc(f, &f->nsteps, S)) || (rc = falloc(f, &f->steplim, S)) ||
         (rc = falloc(f, &f->active, S)) || (rc = falloc(f, &f->iters, S)) ||
         (rc = falloc(f, &f->chisq, S)) || (rc = falloc(f, &f->dlnp, S)) ||
-        (rc = falloc(f, &f->err, 1)) || (rc = falloc(f, &f->prof, 1)) ||
+        (rc = falloc(f, &f->prof, 1)) ||
         (rc = falloc(f, &f->jobs_jac, (size_t)cap_jac)) || (rc = falloc(f, &f->jobs_ls, (size_t)cap_ls)) ||
         (rc = falloc(f, &f->list_jac, (size_t)NBUCKET * cap_jac)) ||
         (rc = falloc(f, &f->list_ls, (size_t)NBUCKET * cap_ls)) || (rc = falloc(f, &f->cnt, 16)) ||
@@ -1161,13 +1192,29 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
             return rc;
         }
     }
+    if (f->n_gal) {
+        // own row slots of the 256 and 512 renders (every galaxy bucket is
+        // launched from a captured graph, so both must exist)
+        for (int i = 0; i < 2; ++i) {
+            const int sz = i ? 512 : 256;
+            const size_t n = fft_hbuf_slot_floats(sz) * (size_t)fft_hbuf_slots(h, sz) / 2;
+            if ((rc = falloc(f, &f->hbuf[i], n))) {
+                trh_fitter_destroy(f);
+                return rc;
+            }
+        }
+    }
     FitProfiles pr;
     host_profiles(pr);
-    std::vector<int32_t> ones(S, 1), zeros(S, 0), order(S);
+    std::vector<int32_t> ones(S, 1), zeros(S, 0), order(S), strides(S);
+    for (int s = 0; s < S; ++s) strides[s] = row_stride ? row_stride[s] : P[s];
     for (int s = 0; s < S; ++s) order[s] = s;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P[a] > P[b]; });
-    const int32_t zero = 0;
     if ((rc = hip_check(hipMemcpy(f->kind, kind, 4 * (size_t)S, hipMemcpyHostToDevice), "copy kind")) ||
+        (rc = hip_check(hipMemcpy(f->stride, strides.data(), 4 * (size_t)S, hipMemcpyHostToDevice),
+                        "copy strides")) ||
+        (rc = hip_check(hipMemcpy(f->flags, zeros.data(), 4 * (size_t)S, hipMemcpyHostToDevice),
+                        "copy flags")) ||
         (rc = hip_check(hipMemcpy(f->P, P, 4 * (size_t)S, hipMemcpyHostToDevice), "copy P")) ||
         (rc = hip_check(hipMemcpy(f->neq_order, order.data(), 4 * (size_t)S, hipMemcpyHostToDevice),
                         "copy order")) ||
@@ -1176,8 +1223,7 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
         (rc = hip_check(hipMemcpy(f->stamp, psf_stamp, 4 * (size_t)ph * pw, hipMemcpyHostToDevice), "copy stamp")) ||
         (rc = hip_check(hipMemcpy(f->prof, &pr, sizeof(pr), hipMemcpyHostToDevice), "copy profiles")) ||
         (rc = hip_check(hipMemcpy(f->active, ones.data(), 4 * (size_t)S, hipMemcpyHostToDevice), "copy active")) ||
-        (rc = hip_check(hipMemcpy(f->iters, zeros.data(), 4 * (size_t)S, hipMemcpyHostToDevice), "copy iters")) ||
-        (rc = hip_check(hipMemcpy(f->err, &zero, 4, hipMemcpyHostToDevice), "copy err"))) {
+        (rc = hip_check(hipMemcpy(f->iters, zeros.data(), 4 * (size_t)S, hipMemcpyHostToDevice), "copy iters"))) {
         trh_fitter_destroy(f);
         return rc;
     }
@@ -1211,12 +1257,23 @@ extern "C" int trh_fitter_set_params(trh_fitter *f, const double *params, void *
         for (int i = 0; i < TRH_FIT_NPARAM; ++i)
             p8[8 * (size_t)s + i] = params[TRH_FIT_NPARAM * (size_t)s + i];
     std::vector<int32_t> ones(f->S, 1), zeros(f->S, 0);
+    hipStream_t st = (hipStream_t)stream;
     int rc;
-    if ((rc = hip_check(hipMemcpy(f->params, p8.data(), 8 * p8.size(), hipMemcpyHostToDevice), "set params")))
+    // after every graph replay issued so far (they run on the fitter's own
+    // stream); the copies are ordered on `stream`, and the host buffers are
+    // released only once they are done
+    if (f->have_gev && (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "order after replays")))
         return rc;
-    if ((rc = hip_check(hipMemcpy(f->active, ones.data(), 4 * (size_t)f->S, hipMemcpyHostToDevice), "reset active")))
+    if ((rc = hip_check(hipMemcpyAsync(f->params, p8.data(), 8 * p8.size(), hipMemcpyHostToDevice, st),
+                        "set params")) ||
+        (rc = hip_check(hipMemcpyAsync(f->active, ones.data(), 4 * (size_t)f->S,
+                                       hipMemcpyHostToDevice, st), "reset active")) ||
+        (rc = hip_check(hipMemcpyAsync(f->iters, zeros.data(), 4 * (size_t)f->S,
+                                       hipMemcpyHostToDevice, st), "reset iters")) ||
+        (rc = hip_check(hipMemcpyAsync(f->flags, zeros.data(), 4 * (size_t)f->S,
+                                       hipMemcpyHostToDevice, st), "reset flags")))
         return rc;
-    return hip_check(hipMemcpy(f->iters, zeros.data(), 4 * (size_t)f->S, hipMemcpyHostToDevice), "reset iters");
+    return hip_check(hipStreamSynchronize(st), "set params sync");
 }
 
 extern "C" int trh_fitter_get_params(trh_fitter *f, double *params, void *stream) {
@@ -1294,9 +1351,9 @@ extern "C" int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab)
 
 static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *list,
                         const int32_t *count, int bound, float *out, double *chi2,
-                        hipStream_t st) {
+                        hipStream_t st, int mode = TRH_STORE) {
     if (bound <= 0) return TRH_OK;
-    const size_t lds = point_lds_bytes(f->ph, f->pw, f->maxP_pt);
+    const size_t lds = point_lds_bytes(f->ph, f->pw);
     int wg = (int)((160 * 1024) / (lds + sizeof(RenderScratch)));
     if (wg < 1) wg = 1;
     if (wg > 8) wg = 8;
@@ -1306,12 +1363,16 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
         // the window's zero-model chi^2 comes from the normal equations of
         // the same iteration (neq[32 s + 28], indexed by the job's pad0)
         hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds, st, jobs, list,
-                           count, bound, f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data,
-                           f->inverr, chi2, f->neq);
-    else
+                           count, bound, f->stamp, f->ph, f->pw, out, f->data, f->inverr, chi2,
+                           f->neq);
+    else if (mode == TRH_STORE)
         hipLaunchKernelGGL(k_render_point<TRH_STORE>, dim3(grid), dim3(256), lds, st, jobs, list,
-                           count, bound, f->stamp, f->ph, f->pw, f->maxP_pt, out, f->data,
-                           f->inverr, chi2, nullptr);
+                           count, bound, f->stamp, f->ph, f->pw, out, f->data, f->inverr, chi2,
+                           nullptr);
+    else
+        hipLaunchKernelGGL(k_render_point<-1>, dim3(grid), dim3(256), lds, st, jobs, list,
+                           count, bound, f->stamp, f->ph, f->pw, out, f->data, f->inverr, chi2,
+                           nullptr);
     return hip_check(hipGetLastError(), "k_render_point");
 }
 
@@ -1321,7 +1382,11 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
 
 // The launches of one LM iteration on `st`.  `capture`: the stream is being
 // captured into a graph, so no events and no per-launch profiling.
-static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t st,
+static float2 *fitter_hbuf(trh_fitter *f, int sz) {
+    return sz == 256 ? f->hbuf[0] : (sz == 512 ? f->hbuf[1] : nullptr);
+}
+
+static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_iter, hipStream_t st,
                         bool capture) {
     const int S = f->S;
     const int cap_jac = f->nj * S, cap_ls = NALPHA * S;
@@ -1337,7 +1402,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     const int nb_jac = (cap_jac + PREP_TB - 1) / PREP_TB, nb_ls = (cap_ls + PREP_TB - 1) / PREP_TB;
     hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb_jac), dim3(PREP_TB), 0, st, S, f->nj, f->kind, f->P,
                        f->render_off, f->params, f->active, f->prof, f->ser, f->sstep, f->ph, f->pw, f->jobs_jac,
-                       f->bk_jac, f->bcnt_jac, f->err, acc);
+                       f->bk_jac, f->bcnt_jac, f->flags, f->active, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_jac), dim3(PREP_TB), 0, st, cap_jac, f->bk_jac,
                        f->bcnt_jac, f->list_jac, cap_jac, f->cnt);
@@ -1348,7 +1413,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
         SLOT_BEGIN(b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
                                  f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal + f->n_ser, f->maxrows[b], f->spec[b], f->renders,
-                                 nullptr, nullptr, nullptr, st)))
+                                 nullptr, nullptr, nullptr, st, fitter_hbuf(f, kSizes[b]))))
             return rc;
         SLOT_END(b);
     }
@@ -1363,7 +1428,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
         const int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
         if (S > f->n_ser) {
             hipLaunchKernelGGL(k_fit_normal_eq<6>, dim3(grid), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
-                               f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
+                               f->stride, f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
                                f->sstep, f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3],
                                f->neq, f->neq_order);
             if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq<6>"))) return rc;
@@ -1371,7 +1436,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
         if (f->n_ser) {
             const int g7 = f->n_ser < f->h->n_cu * 8 ? f->n_ser : f->h->n_cu * 8;
             hipLaunchKernelGGL(k_fit_normal_eq<7>, dim3(g7), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
-                               f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
+                               f->stride, f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
                                f->sstep, f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3],
                                f->neq, f->neq_order);
             if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq<7>"))) return rc;
@@ -1386,8 +1451,8 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     SLOT_END(SL_SOLVE);
     if (!capture) (void)hipEventRecord(f->ev[3], st);
     hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb_ls), dim3(PREP_TB), 0, st, S, f->kind, f->P,
-                       f->pix_off, f->trial, f->nsteps, f->active, f->prof, f->ser, f->ph, f->pw,
-                       f->jobs_ls, f->bk_ls, f->bcnt_ls, f->err, acc);
+                       f->stride, f->pix_off, f->trial, f->nsteps, f->active, f->prof, f->ser,
+                       f->ph, f->pw, f->jobs_ls, f->bk_ls, f->bcnt_ls, f->flags, f->chi2ls, acc);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_ls), dim3(PREP_TB), 0, st, cap_ls, f->bk_ls,
                        f->bcnt_ls, f->list_ls, cap_ls, f->cnt + LS_CNT);
@@ -1397,7 +1462,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
         SLOT_BEGIN(SL_LS + b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
                                  f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
-                                 nullptr, f->data, f->inverr, f->chi2ls, st)))
+                                 nullptr, f->data, f->inverr, f->chi2ls, st, fitter_hbuf(f, kSizes[b]))))
             return rc;
         SLOT_END(SL_LS + b);
     }
@@ -1409,7 +1474,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     if (!capture) (void)hipEventRecord(f->ev[4], st);
     hipLaunchKernelGGL(k_fit_accept, dim3(nb), dim3(tb), 0, st, S, f->neq, f->trial, f->nsteps,
                        f->steplim, f->chi2ls, f->params, f->active, f->iters, f->chisq, f->dlnp,
-                       dchisq, 100);
+                       f->flags, dchisq, max_iter);
     if ((rc = hip_check(hipGetLastError(), "k_fit_accept"))) return rc;
     if (!capture) (void)hipEventRecord(f->ev[5], st);
     if (prof) {
@@ -1441,14 +1506,16 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, hipStream_t
     return TRH_OK;
 }
 
-extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, void *stream) {
-    if (!f) return set_err(TRH_E_ARG, "trh_fitter_iterate: null fitter");
-    return iterate_body(f, lambda, dchisq, (hipStream_t)stream, false);
+extern "C" int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, int max_iter,
+                                  void *stream) {
+    if (!f || max_iter < 1) return set_err(TRH_E_ARG, "trh_fitter_iterate: bad arguments");
+    return iterate_body(f, lambda, dchisq, max_iter, (hipStream_t)stream, false);
 }
 
-extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int n_iter,
-                                        void *stream) {
-    if (!f || n_iter < 0) return set_err(TRH_E_ARG, "trh_fitter_iterate_graph: bad arguments");
+extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int max_iter,
+                                        int n_iter, void *stream) {
+    if (!f || n_iter < 0 || max_iter < 1)
+        return set_err(TRH_E_ARG, "trh_fitter_iterate_graph: bad arguments");
     if (f->profiling)
         return set_err(TRH_E_ARG, "trh_fitter_iterate_graph: per-launch profiling needs "
                                   "trh_fitter_iterate");
@@ -1463,19 +1530,17 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
                 return rc;
         f->have_gev = true;
     }
-    if (!f->gexec || lambda != f->g_lambda || dchisq != f->g_dchisq) {
+    if (!f->gexec || lambda != f->g_lambda || dchisq != f->g_dchisq || max_iter != f->g_max_iter) {
         if (f->gexec) {
             (void)hipGraphExecDestroy(f->gexec);
             f->gexec = nullptr;
         }
-        // every galaxy bucket is launched (device-side counts), so the global
-        // row slots of the SZ >= 256 renders must exist before capture
-        for (int b = 0; b < NFFT && f->n_gal; ++b)
-            if ((rc = reserve_fft_hbuf(f->h, kSizes[b]))) return rc;
+        // (the row slots of the SZ >= 256 renders are the fitter's own,
+        // allocated at creation)
         if ((rc = hip_check(hipStreamBeginCapture(f->gstream, hipStreamCaptureModeThreadLocal),
                             "begin capture")))
             return rc;
-        const int brc = iterate_body(f, lambda, dchisq, f->gstream, true);
+        const int brc = iterate_body(f, lambda, dchisq, max_iter, f->gstream, true);
         hipGraph_t g = nullptr;
         rc = hip_check(hipStreamEndCapture(f->gstream, &g), "end capture");
         if (brc) {
@@ -1488,6 +1553,7 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
         if (rc) return rc;
         f->g_lambda = lambda;
         f->g_dchisq = dchisq;
+        f->g_max_iter = max_iter;
     }
     hipStream_t st = (hipStream_t)stream;
     if ((rc = hip_check(hipEventRecord(f->gev[0], st), "graph order in")) ||
@@ -1530,7 +1596,8 @@ extern "C" int trh_fitter_profile_read(trh_fitter *f, double *out) {
     return TRH_OK;
 }
 
-extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, void *stream) {
+extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, int accumulate,
+                                        void *stream) {
     if (!f || !out) return set_err(TRH_E_ARG, "trh_fitter_render_models: null argument");
     hipStream_t st = (hipStream_t)stream;
     const int S = f->S;
@@ -1539,18 +1606,19 @@ extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, void *stream)
     if (f->n_ser && !f->ser)
         return set_err(TRH_E_ARG, "trh_fitter_render_models: Sersic galaxies need trh_fitter_set_sersic");
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
+    const int mode = accumulate ? TRH_ADD : TRH_STORE;
     hipLaunchKernelGGL(k_fit_prep_model, dim3((S + 255) / 256), dim3(256), 0, st, S, f->nj, f->kind, f->P,
-                       f->pix_off, f->params, f->prof, f->ser, f->ph, f->pw, f->jobs_jac, f->list_jac,
-                       cap_jac, f->cnt, f->err);
+                       f->stride, mode, f->pix_off, f->params, f->prof, f->ser, f->ph, f->pw,
+                       f->jobs_jac, f->list_jac, cap_jac, f->cnt, f->flags);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_model"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
-        if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
+        if ((rc = launch_fft_any(f->h, kSizes[b], accumulate ? -1 : TRH_STORE, f->jobs_jac,
                                  f->list_jac + (size_t)b * cap_jac, f->cnt + b, f->n_gal, f->maxrows[b], f->spec[b], out, nullptr,
-                                 nullptr, nullptr, st)))
+                                 nullptr, nullptr, st, fitter_hbuf(f, kSizes[b]))))
             return rc;
     }
     return launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt, out,
-                        nullptr, st);
+                        nullptr, st, mode);
 }
 
 extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations,
@@ -1558,9 +1626,6 @@ extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterat
     if (!f) return set_err(TRH_E_ARG, "trh_fitter_status: null fitter");
     int rc;
     if ((rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "sync"))) return rc;
-    int32_t err = 0;
-    if ((rc = hip_check(hipMemcpy(&err, f->err, 4, hipMemcpyDeviceToHost), "read err"))) return rc;
-    if (err) return set_err(TRH_E_UNSUPPORTED, "a render needed FFT size %d (supported: 32..512)", err);
     std::vector<int32_t> a(f->S);
     if ((rc = hip_check(hipMemcpy(a.data(), f->active, 4 * (size_t)f->S, hipMemcpyDeviceToHost), "read active")))
         return rc;
@@ -1578,6 +1643,14 @@ extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterat
     if (dlnp && (rc = hip_check(hipMemcpy(dlnp, f->dlnp, 8 * (size_t)f->S, hipMemcpyDeviceToHost), "read dlnp")))
         return rc;
     return TRH_OK;
+}
+
+extern "C" int trh_fitter_flags(trh_fitter *f, int32_t *flags, void *stream) {
+    if (!f || !flags) return set_err(TRH_E_ARG, "trh_fitter_flags: null argument");
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "sync"))) return rc;
+    return hip_check(hipMemcpy(flags, f->flags, 4 * (size_t)f->S, hipMemcpyDeviceToHost),
+                     "read flags");
 }
 
 extern "C" int trh_fitter_debug(trh_fitter *f, const float **renders, const int64_t **render_off,

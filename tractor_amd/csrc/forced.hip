@@ -1167,12 +1167,14 @@ extern "C" int trh_fp_imstats(trh_handle *h, int64_t npix, const float *model, c
     if (nblk < 1) nblk = 1;
     if (nblk > 2048) nblk = 2048;
     int rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = shared_begin(h, st))) return rc;
     if ((rc = ensure_scratch(h, 16 * (size_t)nblk))) return rc;
     double *part = (double *)h->scratch;
-    hipLaunchKernelGGL(k_fp_imstats_part, dim3((int)nblk), dim3(256), 0, (hipStream_t)stream, npix,
-                       model, data, inverr, sky, part);
+    hipLaunchKernelGGL(k_fp_imstats_part, dim3((int)nblk), dim3(256), 0, st, npix, model, data,
+                       inverr, sky, part);
     if ((rc = hip_check(hipGetLastError(), "k_fp_imstats_part"))) return rc;
-    hipLaunchKernelGGL(k_fp_imstats_final, dim3(1), dim3(64), 0, (hipStream_t)stream, (int)nblk,
-                       part, out);
-    return hip_check(hipGetLastError(), "k_fp_imstats_final");
+    hipLaunchKernelGGL(k_fp_imstats_final, dim3(1), dim3(64), 0, st, (int)nblk, part, out);
+    if ((rc = hip_check(hipGetLastError(), "k_fp_imstats_final"))) return rc;
+    return shared_end(h, st);
 }

@@ -17,6 +17,12 @@ struct trh_handle {
     // per FFT size: global half-spectrum slots of the SZ >= 256 renders,
     // allocated once (a captured graph keeps the address)
     float2 *hbuf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // ordering of the shared buffers above (scratch, hbuf) across streams:
+    // every entry point that uses them records order_ev on its stream at the
+    // end; a later call on another stream waits for it first
+    hipEvent_t order_ev = nullptr;
+    hipStream_t order_stream = nullptr;
+    bool order_recorded = false;
 };
 
 namespace trh {
@@ -30,8 +36,16 @@ void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *row
                       int *grid, int njobs);
 int fft_hbuf_slots(const trh_handle *h, int sz);
 int reserve_fft_hbuf(trh_handle *h, int sz);
+// `hbuf`: the row slots of the SZ >= 256 renders (fft_hbuf_slots(h, sz)
+// slots of fft_hbuf_slot_floats(sz) floats), or NULL for the handle's own
 int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
                    float *out, const float *data, const float *inverr, double *chi2,
-                   hipStream_t st);
+                   hipStream_t st, float2 *hbuf = nullptr);
+// Bracket every use of the handle's shared buffers (scratch, hbuf) on `st`:
+// shared_begin orders `st` after the last use on another stream, shared_end
+// records this use.  Streams being captured into a graph are not ordered
+// (a graph that uses the shared buffers must own the handle's stream).
+int shared_begin(trh_handle *h, hipStream_t st);
+int shared_end(trh_handle *h, hipStream_t st);
 }  // namespace trh

@@ -657,7 +657,8 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 // Pixelized-PSF point-source render (PixelizedPSF.getPointSourcePatch with a
 // modelMask, tractor/psf.py:137-190): the stamp, its (0,0) pixel placed at
 // window coordinates (sx, sy), Lanczos-3 shifted by (mux, muy); stamp, tmp
-// in LDS: tmp holds ph x mw floats.
+// in LDS: tmp holds the x-filtered stamp rows over the window columns the
+// shifted stamp reaches (at most pw + 6 of them, whatever the window width).
 template <int MODE>
 __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
                                  const float *__restrict__ stamp, bool load_stamp, int ph, int pw,
@@ -684,8 +685,8 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     const int ry0 = max(0, sy - 3), ry1 = min(mh, sy + ph + 3);
     const int cw = cx1 - cx0;
     // x pass over the stamp rows the y taps reach, [rlo, rhi), and the covered
-    // columns: tmp[ox][r] = sum_u fx[u] stamp[r][ox-sx-3+u], stored by
-    // column (row stride ts odd).  A thread takes one stamp row and PB
+    // columns: tmp[ox - cx0][r] = sum_u fx[u] stamp[r][ox-sx-3+u], stored by
+    // column (row stride ts odd; cw <= pw + 6 columns).  A thread takes one stamp row and PB
     // consecutive columns (PB + 6 stamp reads for PB outputs); threads run
     // along r, which the stamp's odd row stride spreads over the LDS banks.
     constexpr int PB = 4;
@@ -707,7 +708,7 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
             float a = 0.f;
 #pragma unroll
             for (int u = 0; u < 7; ++u) a += fx[u] * v[j + u];
-            if (ox0 + j < cx1) tmp[(ox0 + j) * ts + r] = a;
+            if (ox0 + j < cx1) tmp[(ox0 + j - cx0) * ts + r] = a;
         }
     }
     __syncthreads();
@@ -720,7 +721,7 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 #pragma unroll
         for (int t = 0; t < 7; ++t) {
             const int r = oy - sy - 3 + t;
-            if (r >= 0 && r < ph) a += fy[t] * tmp[ox * ts + r];
+            if (r >= 0 && r < ph) a += fy[t] * tmp[(ox - cx0) * ts + r];
         }
         return a;
     };
@@ -746,7 +747,7 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 #pragma unroll
             for (int k = 0; k < PB + 6; ++k) {
                 const int r = oy0 - sy - 3 + k;
-                v[k] = (r >= rlo && r < rhi) ? tmp[ox * ts + r] : 0.f;
+                v[k] = (r >= rlo && r < rhi) ? tmp[(ox - cx0) * ts + r] : 0.f;
             }
 #pragma unroll
             for (int j = 0; j < PB; ++j) {
@@ -845,11 +846,12 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
 // MODE: TRH_STORE / TRH_CHI2 fixed at compile time, or -1 (read from each job).
 // chi0 (CHI2 only, may be NULL): per-window chi^2 of a zero model, at
 // chi0[job.pad0]; with it a chi^2 render visits only the stamp's support.
+// Dynamic LDS: point_lds_bytes(ph, pw), for any window size.
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
                const int *__restrict__ count, int njobs, const float *__restrict__ stamp,
-               int ph, int pw, int maxw, float *__restrict__ out,
+               int ph, int pw, float *__restrict__ out,
                const float *__restrict__ data, const float *__restrict__ inverr,
                double *__restrict__ chi2, const double *__restrict__ chi0) {
     extern __shared__ float2 smem[];
@@ -860,7 +862,6 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
     int cur = -1;   // stamp index held in LDS
     for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
         const int jid = list ? list[jj] : jj;
-        if (jobs[jid].mw > maxw) continue;  // host guarantees maxw >= every mw
         // the job's psf field selects one of the stamps at stamp + psf * ph * pw
         const int psf = jobs[jid].psf;
         render_point_one<MODE>(jobs + jid, stamp + (size_t)psf * ph * pw, psf != cur, ph, pw,
@@ -869,8 +870,8 @@ k_render_point(const trh_fft_job *__restrict__ jobs, const int *__restrict__ lis
     }
 }
 
-__host__ __device__ inline size_t point_lds_bytes(int ph, int pw, int maxw) {
-    return sizeof(float) * ((((size_t)ph * pw + 3) & ~(size_t)3) + (size_t)(ph | 1) * maxw);
+__host__ __device__ inline size_t point_lds_bytes(int ph, int pw) {
+    return sizeof(float) * ((((size_t)ph * pw + 3) & ~(size_t)3) + (size_t)(ph | 1) * (pw + 6));
 }
 
 }  // namespace trh

@@ -15,6 +15,7 @@ using namespace trh;
 
 static thread_local std::string g_err;
 
+
 namespace trh {
 int set_err(int code, const char *fmt, ...) {
     char buf[512];
@@ -50,10 +51,29 @@ int pad_center(int sz, int p) {
 }
 
 int cu_count(trh_handle *h) { return h->n_cu; }
+
+static bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int shared_begin(trh_handle *h, hipStream_t st) {
+    if (!h->order_recorded || st == h->order_stream || capturing(st)) return TRH_OK;
+    return hip_check(hipStreamWaitEvent(st, h->order_ev, 0), "order shared buffers");
+}
+
+int shared_end(trh_handle *h, hipStream_t st) {
+    if (capturing(st)) return TRH_OK;
+    int rc = hip_check(hipEventRecord(h->order_ev, st), "record shared-buffer use");
+    if (rc) return rc;
+    h->order_recorded = true;
+    h->order_stream = st;
+    return TRH_OK;
+}
 }  // namespace trh
 
 extern "C" const char *trh_last_error(void) { return g_err.c_str(); }
-extern "C" int trh_version(void) { return 1; }
+extern "C" int trh_version(void) { return 2; }
 
 extern "C" int trh_struct_sizes(int *fft_job, int *mog_job, int *mog_approx_job) {
     if (fft_job) *fft_job = (int)sizeof(trh_fft_job);
@@ -102,6 +122,12 @@ extern "C" int trh_create(trh_handle **hp, int device) {
         delete h;
         return rc;
     }
+    if ((rc = hip_check(hipEventCreateWithFlags(&h->order_ev, hipEventDisableTiming),
+                        "order event"))) {
+        (void)hipFree(h->tw);
+        delete h;
+        return rc;
+    }
     hipLaunchKernelGGL(k_twiddles, dim3(1), dim3(256), 0, 0, h->tw);
     if ((rc = hip_check(hipGetLastError(), "k_twiddles"))) return rc;
     if ((rc = hip_check(hipDeviceSynchronize(), "twiddle sync"))) return rc;
@@ -113,6 +139,7 @@ extern "C" int trh_destroy(trh_handle *h) {
     if (!h) return TRH_OK;
     (void)hipSetDevice(h->device);
     if (h->tw) (void)hipFree(h->tw);
+    if (h->order_ev) (void)hipEventDestroy(h->order_ev);
     if (h->scratch) (void)hipFree(h->scratch);
     for (int i = 0; i < 5; ++i)
         if (h->hbuf[i]) (void)hipFree(h->hbuf[i]);
@@ -323,17 +350,18 @@ int max_rows_for(int sz, int lds_max) {
 template <int SZ>
 int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list,
                const int *count, int njobs_bound, int maxrows, const float *psf_spec, float *out,
-               const float *data, const float *inverr, double *chi2, hipStream_t st) {
+               const float *data, const float *inverr, double *chi2, hipStream_t st,
+               float2 *hbuf) {
     int nt, rows, grid;
     size_t lds;
     fft_launch_shape(h, SZ, maxrows, &nt, &rows, &lds, &grid, njobs_bound);
     if (lds + sizeof(RenderScratch) > (size_t)h->lds_max)
         return set_err(TRH_E_UNSUPPORTED, "k_render_fft<%d>: LDS %zu exceeds %d", SZ, lds,
                        h->lds_max);
-    float2 *hb = nullptr;
+    float2 *hb = hbuf;
     if constexpr (WaveGeom<SZ>::GH) {
         const int si = sz_index(SZ);
-        if (!h->hbuf[si]) {
+        if (!hb && !h->hbuf[si]) {
             // the slots are allocated once, outside any stream capture
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
@@ -342,7 +370,7 @@ int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list
             const int rc = reserve_fft_hbuf(h, SZ);
             if (rc) return rc;
         }
-        hb = h->hbuf[si];
+        if (!hb) hb = h->hbuf[si];
     }
     const float2 *tw = h->tw + tw_offset(SZ);
     const float2 *ps = (const float2 *)psf_spec;
@@ -366,23 +394,23 @@ int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list
 int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
                    float *out, const float *data, const float *inverr, double *chi2,
-                   hipStream_t st) {
+                   hipStream_t st, float2 *hbuf) {
     switch (sz) {
     case 32:
         return launch_fft<32>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                              data, inverr, chi2, st);
+                              data, inverr, chi2, st, hbuf);
     case 64:
         return launch_fft<64>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                              data, inverr, chi2, st);
+                              data, inverr, chi2, st, hbuf);
     case 128:
         return launch_fft<128>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                               data, inverr, chi2, st);
+                               data, inverr, chi2, st, hbuf);
     case 256:
         return launch_fft<256>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                               data, inverr, chi2, st);
+                               data, inverr, chi2, st, hbuf);
     case 512:
         return launch_fft<512>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                               data, inverr, chi2, st);
+                               data, inverr, chi2, st, hbuf);
     }
     return set_err(TRH_E_UNSUPPORTED, "FFT render size %d not in {32,64,128,256,512}", sz);
 }
@@ -410,8 +438,13 @@ extern "C" int trh_render_fft_batch(trh_handle *h, int sz, const trh_fft_job *jo
     // no output image but chi^2 slots: every job is a chi^2 job (a STORE or
     // ADD job would need `out`), so the chi^2-specialised kernel runs
     const int mode = (!out && chi2 && data && inverr) ? TRH_CHI2 : -1;
-    return launch_fft_any(h, sz, mode, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out, data,
-                          inverr, chi2, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (sz >= 256 && (rc = shared_begin(h, st))) return rc;
+    if ((rc = launch_fft_any(h, sz, mode, jobs, nullptr, nullptr, njobs, maxrows, psf_spec, out,
+                             data, inverr, chi2, st)))
+        return rc;
+    return sz >= 256 ? shared_end(h, st) : TRH_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -554,46 +587,56 @@ k_lanczos_pass(const float *__restrict__ in, float *__restrict__ out, int height
     }
 }
 
+// x pass into tmp (n * height * width floats), y pass into out
+static int lanczos_passes(trh_handle *h, const float *in, float *tmp, float *out, int n,
+                          int height, int width, const double *dx, const double *dy,
+                          hipStream_t st) {
+    (void)h;
+    const int gx = (height * width + 255) / 256 < 1024 ? (height * width + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_lanczos_pass, dim3(gx, n), dim3(256), 0, st, in, tmp, height, width, dx, 0);
+    int rc = hip_check(hipGetLastError(), "k_lanczos_pass x");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_lanczos_pass, dim3(gx, n), dim3(256), 0, st, tmp, out, height, width, dy, 1);
+    return hip_check(hipGetLastError(), "k_lanczos_pass y");
+}
+
 extern "C" int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n,
                                        int height, int width, const double *dx, const double *dy,
                                        void *stream) {
     if (!h || !in || !out || !dx || !dy || n <= 0 || height <= 0 || width <= 0)
         return set_err(TRH_E_ARG, "trh_lanczos_shift_batch: bad arguments");
     const size_t bytes = sizeof(float) * (size_t)n * height * width;
-    int rc = ensure_scratch(h, bytes);
-    if (rc) return rc;
-    float *tmp = (float *)h->scratch;
-    const int gx = (height * width + 255) / 256 < 1024 ? (height * width + 255) / 256 : 1024;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_lanczos_pass, dim3(gx, n), dim3(256), 0, st, in, tmp, height, width, dx, 0);
-    if ((rc = hip_check(hipGetLastError(), "k_lanczos_pass x"))) return rc;
-    hipLaunchKernelGGL(k_lanczos_pass, dim3(gx, n), dim3(256), 0, st, tmp, out, height, width, dy, 1);
-    return hip_check(hipGetLastError(), "k_lanczos_pass y");
+    int rc = shared_begin(h, st);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, bytes))) return rc;
+    if ((rc = lanczos_passes(h, in, (float *)h->scratch, out, n, height, width, dx, dy, st)))
+        return rc;
+    return shared_end(h, st);
 }
 
 // ---------------------------------------------------------------------------
 // Batched pixelized-PSF point renders (generic API path)
 // ---------------------------------------------------------------------------
-extern "C" int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs, int maxw,
+extern "C" int trh_render_point_batch(trh_handle *h, const trh_fft_job *jobs, int njobs,
                                       const float *stamp, int ph, int pw, float *out,
                                       const float *data, const float *inverr, double *chi2,
                                       void *stream) {
-    if (!h || !jobs || !stamp || njobs < 0 || maxw <= 0)
+    if (!h || !jobs || !stamp || njobs < 0 || ph <= 0 || pw <= 0)
         return set_err(TRH_E_ARG, "trh_render_point_batch: bad arguments");
     if (njobs == 0) return TRH_OK;
-    const size_t lds = point_lds_bytes(ph, pw, maxw);
+    const size_t lds = point_lds_bytes(ph, pw);
     if (lds + sizeof(RenderScratch) > (size_t)h->lds_max)
-        return set_err(TRH_E_UNSUPPORTED, "point render: stamp %dx%d with window width %d exceeds LDS",
-                       ph, pw, maxw);
+        return set_err(TRH_E_UNSUPPORTED, "point render: stamp %dx%d exceeds LDS", ph, pw);
     int grid = njobs < h->n_cu * 4 ? njobs : h->n_cu * 4;
     // no output image but chi^2 slots: every job is a chi^2 job
     if (!out && chi2 && data && inverr)
         hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds,
-                           (hipStream_t)stream, jobs, nullptr, nullptr, njobs, stamp, ph, pw, maxw,
+                           (hipStream_t)stream, jobs, nullptr, nullptr, njobs, stamp, ph, pw,
                            out, data, inverr, chi2, nullptr);
     else
         hipLaunchKernelGGL(k_render_point<-1>, dim3(grid), dim3(256), lds, (hipStream_t)stream,
-                           jobs, nullptr, nullptr, njobs, stamp, ph, pw, maxw, out, data, inverr,
+                           jobs, nullptr, nullptr, njobs, stamp, ph, pw, out, data, inverr,
                            chi2, nullptr);
     return hip_check(hipGetLastError(), "k_render_point");
 }
@@ -706,14 +749,16 @@ extern "C" int trh_solve_normal_f64(trh_handle *h, const double *Nb, int n, doub
                                     int scale_columns, double *x, int *info_host, void *stream) {
     if (!h || !Nb || !x || n <= 0 || n > 512)
         return set_err(TRH_E_ARG, "trh_solve_normal_f64: bad arguments (1 <= n <= 512)");
-    int rc = ensure_scratch(h, sizeof(double) * ((size_t)n * n + n) + 64);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = shared_begin(h, st);
     if (rc) return rc;
+    if ((rc = ensure_scratch(h, sizeof(double) * ((size_t)n * n + n) + 64))) return rc;
     double *L = (double *)h->scratch;
     int *info = (int *)(L + (size_t)n * n + n);
-    hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_solve_normal_f64, dim3(1), dim3(64), 0, st, Nb, n, damp, scale_columns, L,
                        x, info);
     if ((rc = hip_check(hipGetLastError(), "k_solve_normal_f64"))) return rc;
+    if ((rc = shared_end(h, st))) return rc;
     if (info_host) {
         if ((rc = hip_check(hipMemcpyAsync(info_host, info, sizeof(int), hipMemcpyDeviceToHost, st),
                             "copy info")))

@@ -443,7 +443,7 @@ class BatchForcedPhotometry(object):
         if c:
             stamps = torch.empty((c, self.ph * self.pw), dtype=torch.float32, device=self.dev)
             combine(basis, self.ph * self.pw, b0, c, stamps)
-            check(lib.trh_render_point_batch(h, ptr(self.d_jobs) + rec * b0, c, self.pw,
+            check(lib.trh_render_point_batch(h, ptr(self.d_jobs) + rec * b0, c,
                                              ptr(stamps), self.ph, self.pw, ptr(self.patches),
                                              None, None, None, st), 'render_point_batch')
             keep.append(stamps)

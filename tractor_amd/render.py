@@ -168,8 +168,7 @@ def render_point_jobs(jobs, spectra, out=None, data=None, inverr=None, chi2=None
     lib = _lib.load()
     h = _lib.handle(torch.device(device).index or 0)
     ph, pw = spectra.shape
-    maxw = int(np.max(jobs['mw']))
     dj = jobs_to_device(jobs, device)
-    check(lib.trh_render_point_batch(h.value, ptr(dj), len(jobs), maxw, ptr(spectra.stamp), ph, pw,
+    check(lib.trh_render_point_batch(h.value, ptr(dj), len(jobs), ptr(spectra.stamp), ph, pw,
                                      ptr(out), ptr(data), ptr(inverr), ptr(chi2), None),
           'trh_render_point_batch')
