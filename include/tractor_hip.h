@@ -204,6 +204,22 @@ int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *out, int n, i
                             int width, const double *dx, const double *dy, void *stream);
 
 /*
+ * trh_lanczos_shift_batch without any allocation (no reference counterpart:
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The x-pass
+ * intermediate lives in a static device array of TRH_NOALLOC_SCRATCH_FLOATS
+ * floats (32 MiB) per GPU, held until the process ends, so n * height *
+ * width must not exceed it (TRH_E_ARG otherwise).  The handle's device must
+ * be current.  Every call on one GPU shares that array: calls that overlap
+ * in time on one GPU (other streams, other handles) race.  Same results as
+ * trh_lanczos_shift_batch (the same kernels).
+ */
+#define TRH_NOALLOC_SCRATCH_FLOATS (8 * 1024 * 1024)
+int trh_lanczos_shift_batch_noalloc(trh_handle *h, const float *in, float *out, int n,
+                                    int height, int width, const double *dx, const double *dy,
+                                    void *stream);
+
+/*
  * Batched pixelized-PSF point-source renders (PixelizedPSF.getPointSourcePatch
  * with a modelMask, tractor/psf.py:137-190; one SWIG lanczos_shift_3f call per
  * source in the reference).  Job fields as trh_fft_job with: (sx, sy) = window

@@ -17,6 +17,7 @@ TRH_MAXK = 16
 # per-source flags of the batched fitter (trh_fitter_flags)
 TRH_FIT_F_JAC_SIZE, TRH_FIT_F_LS_SIZE, TRH_FIT_F_MAXITER = 1, 2, 4
 TRH_FIT_F_LIMITED, TRH_FIT_F_MODEL_SIZE = 8, 16
+TRH_NOALLOC_SCRATCH_FLOATS = 8 * 1024 * 1024
 
 # struct trh_fft_job (include/tractor_hip.h)
 FFT_JOB = np.dtype([
@@ -66,6 +67,7 @@ SIGNATURES = {
     'trh_render_mog_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_render_mog_approx3_batch': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'trh_lanczos_shift_batch': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
+    'trh_lanczos_shift_batch_noalloc': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P]),
     'trh_render_point_batch': (_I, [_P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     'trh_psf_combine': (_I, [_P, _P, _I, ctypes.c_int64, _P, _P, _I, _P, _I, _P]),
     'trh_psfex_terms': (_I, [_P, _I, _P, _P, _I, _D, _D, _D, _D, _P, _P, _P]),

@@ -23,6 +23,9 @@ struct trh_handle {
     hipEvent_t order_ev = nullptr;
     hipStream_t order_stream = nullptr;
     bool order_recorded = false;
+    // device address of the allocation-free entry point's scratch array on
+    // this handle's device
+    void *noalloc_scratch = nullptr;
 };
 
 namespace trh {

@@ -15,6 +15,11 @@ using namespace trh;
 
 static thread_local std::string g_err;
 
+// Scratch of the allocation-free Lanczos entry point (the x-pass
+// intermediate): TRH_NOALLOC_SCRATCH_FLOATS floats, one copy per GPU, held
+// until the process ends.  Host code never uses this name as an address:
+// trh_create resolves the device address with hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[TRH_NOALLOC_SCRATCH_FLOATS];
 
 namespace trh {
 int set_err(int code, const char *fmt, ...) {
@@ -124,6 +129,14 @@ extern "C" int trh_create(trh_handle **hp, int device) {
     }
     if ((rc = hip_check(hipEventCreateWithFlags(&h->order_ev, hipEventDisableTiming),
                         "order event"))) {
+        (void)hipFree(h->tw);
+        delete h;
+        return rc;
+    }
+    // this device's copy of the allocation-free entry point's scratch
+    if ((rc = hip_check(hipGetSymbolAddress(&h->noalloc_scratch, HIP_SYMBOL(scratch)),
+                        "hipGetSymbolAddress(scratch)"))) {
+        (void)hipEventDestroy(h->order_ev);
         (void)hipFree(h->tw);
         delete h;
         return rc;
@@ -613,6 +626,35 @@ extern "C" int trh_lanczos_shift_batch(trh_handle *h, const float *in, float *ou
     if ((rc = lanczos_passes(h, in, (float *)h->scratch, out, n, height, width, dx, dy, st)))
         return rc;
     return shared_end(h, st);
+}
+
+// The same shift with no allocation: the intermediate goes to this device's
+// static scratch array (through the address trh_create resolved), so the
+// call makes no hipMalloc / hipFree.  Calls that overlap on one GPU share
+// the array and race; nothing in it needs clearing (the x pass writes every
+// element the y pass reads).
+extern "C" int trh_lanczos_shift_batch_noalloc(trh_handle *h, const float *in, float *out, int n,
+                                               int height, int width, const double *dx,
+                                               const double *dy, void *stream) {
+    if (!h || !in || !out || !dx || !dy || n <= 0 || height <= 0 || width <= 0)
+        return set_err(TRH_E_ARG, "trh_lanczos_shift_batch_noalloc: bad arguments");
+    const unsigned long long floats = (unsigned long long)n * height * width;
+    if (floats > (unsigned long long)TRH_NOALLOC_SCRATCH_FLOATS)
+        return set_err(TRH_E_ARG,
+                       "trh_lanczos_shift_batch_noalloc: %d images of %d x %d need %llu floats "
+                       "of scratch, more than the %d it has",
+                       n, height, width, floats, TRH_NOALLOC_SCRATCH_FLOATS);
+    if (!h->noalloc_scratch)
+        return set_err(TRH_E_ARG, "trh_lanczos_shift_batch_noalloc: handle has no scratch address");
+    int dev = -1;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev != h->device)
+        return set_err(TRH_E_ARG,
+                       "trh_lanczos_shift_batch_noalloc: device %d is current, the handle is on %d",
+                       dev, h->device);
+    return lanczos_passes(h, in, (float *)h->noalloc_scratch, out, n, height, width, dx, dy,
+                          (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------
