@@ -431,3 +431,105 @@ def test_optimize_loop_recovers_sersic_index():
     r = tr.optimize_loop()
     assert r['steps'] < 100 and tr.getLogProb() > lnp0
     np.testing.assert_allclose(gal.getParams(), truth.getParams()[2:], atol=5e-3, rtol=5e-3)
+
+
+def _scene_catalog(n, psf_stamp, rng):
+    srcs = []
+    for i in range(n):
+        x, y = rng.uniform(10, 90, 2)
+        k = i % 3
+        if k == 0:
+            srcs.append(T.PointSource(T.PixPos(x, y), T.Flux(rng.uniform(50, 500))))
+        else:
+            cls = T.ExpGalaxy if k == 1 else T.DevGalaxy
+            srcs.append(cls(T.PixPos(x, y), T.Flux(rng.uniform(50, 500)),
+                            T.EllipseESoft(np.log(rng.uniform(1., 4.)), rng.uniform(-0.3, 0.3),
+                                           rng.uniform(-0.3, 0.3))))
+    return srcs
+
+
+def test_model_image_and_derivs_are_batched():
+    """getModelImage and getDerivs plan every source into one RenderPlan:
+    one synchronisation and one render call per bucket (FFT size, stamp),
+    not one per source; the pixels equal the per-source renders bit for bit
+    (each render is one workgroup's, whatever else shares its launch)."""
+    from tractor_amd.plan import RenderPlan
+    import oracle.tractor_oracle as O
+    from tractor_amd import workloads as W
+    rng = np.random.default_rng(20261206)
+    psf = T.PixelizedPSF(W.moffat_stamp(31, 4.0, 3.5))
+    tim = _tim(psf)
+    srcs = _scene_catalog(24, psf, rng)
+    tr = T.Tractor([tim], srcs)
+    st = RenderPlan.stats
+    r0, c0 = st['runs'], st['calls']
+    mod = tr.getModelImage(0)
+    runs, calls = st['runs'] - r0, st['calls'] - c0
+    assert runs == 1
+    # point sources: one shift call (one stamp shape); galaxies: one call
+    # per FFT size in use
+    assert calls <= 1 + 3, calls
+    ref = np.zeros_like(mod)
+    for s in srcs:
+        p = s.getModelPatch(tim)
+        if p is not None:
+            p.addTo(ref)
+    assert np.array_equal(mod, ref)
+    # derivatives: every source's renders (unit + shape steps) in one plan
+    tr.freezeParam('images')
+    r0, c0 = st['runs'], st['calls']
+    allderivs = tr.getDerivs()
+    assert st['runs'] - r0 == 1 and st['calls'] - c0 <= 1 + 3
+    k = 0
+    for s in srcs:
+        own = s.getParamDerivatives(tim)
+        for d in own:
+            got = allderivs[k]
+            k += 1
+            if d is None:
+                assert got == []
+                continue
+            (gd, gimg), = got
+            assert gimg is tim and (gd.x0, gd.y0) == (d.x0, d.y0)
+            assert np.array_equal(gd.patch, d.patch)
+    assert k == len(allderivs)
+
+
+def test_line_search_is_one_plan():
+    """A constrained line search renders all of its trial steps in one plan
+    and picks the step the sequential rule picks."""
+    from tractor_amd.plan import RenderPlan
+    from tractor_amd import workloads as W
+    rng = np.random.default_rng(20261207)
+    psf = T.PixelizedPSF(W.moffat_stamp(31, 4.0, 3.5))
+    truth = _scene_catalog(6, psf, rng)
+    tim = _tim(psf)
+    tim.data = T.Tractor([tim], truth).getModelImage(0) + rng.normal(size=tim.shape).astype(
+        np.float32)
+    start = [s.copy() for s in truth]
+    for s in start:
+        s.brightness.setParams([s.brightness.getParams()[0] * 1.2])
+    tr = T.Tractor([tim], start, optimizer=T.ConstrainedOptimizer())
+    tr.freezeParam('images')
+    opt = tr.optimizer
+    X = opt.getLinearUpdateDirection(tr)
+    p0 = np.array(tr.getParams())
+    steps = opt.getParameterSteps(tr, X, None)
+    # the sequential reference rule on the same steps, one getLogProb each
+    lnp0 = tr.getLogProb()
+    best, ib = lnp0, None
+    for i, (a, p, _, _) in enumerate(steps):
+        tr.setParams(p)
+        lnp = tr.getLogProb()
+        if not np.isfinite(lnp) or lnp < best - 1.:
+            break
+        if lnp > best:
+            best, ib = lnp, i
+    tr.setParams(p0)
+    st = RenderPlan.stats
+    r0 = st['runs']
+    dlnp, alpha = opt.tryUpdates(tr, X)
+    assert st['runs'] - r0 == 1
+    assert ib is not None and alpha == steps[ib][0]
+    assert abs(dlnp - (best - lnp0)) <= 1e-9 * abs(best)
+    assert np.allclose(tr.getParams(), steps[ib][1], rtol=0, atol=0)

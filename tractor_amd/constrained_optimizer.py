@@ -1,10 +1,24 @@
-"""ConstrainedOptimizer (bounds + max steps in the line search) and the
+"""ConstrainedOptimizer (bounds and maximum steps in the line search) and the
 GPU-plugin counterpart (tractor/constrained_optimizer.py:1-159,
-tractor/gpu_optimizer.py:24-92)."""
+tractor/gpu_optimizer.py:24-92).
+
+The line search evaluates every trial step's log-probability in one
+RenderPlan (Tractor._log_probs) and then walks them in order with the
+reference's acceptance rule; the reference renders them one at a time and
+stops early, which picks the same step (the walk stops where it would).
+"""
 import numpy as np
 
 from .lsqr_optimizer import LsqrOptimizer
-from .optimize import ALPHAS
+from .optimize import ALPHAS, walk_line_search
+
+
+def _within(v, lo, hi):
+    if lo is not None and v < lo:
+        v = lo
+    if hi is not None and v > hi:
+        v = hi
+    return v
 
 
 class ConstrainedOptimizer(LsqrOptimizer):
@@ -16,101 +30,94 @@ class ConstrainedOptimizer(LsqrOptimizer):
 
     def optimize_loop(self, tractor, dchisq=0., steps=100, dchisq_limited=1e-6, check_step=None,
                       **kwargs):
-        R = {}
+        """Steps until one gains at most dchisq, or at most dchisq_limited
+        when the step was limited by a maximum step size
+        (constrained_optimizer.py:12-34).  Returns the last step's index and
+        whether the last accepted / any step reached a bound."""
         self.hit_limit = False
         self.last_step_hit_limit = False
-        step = 0
-        for step in range(steps):
+        k = 0
+        for k in range(steps):
             self.stepLimited = False
             dlnp, X, alpha = self.optimize(tractor, **kwargs)
-            if check_step is not None:
-                if not check_step(optimizer=self, tractor=tractor, dlnp=dlnp, X=X, alpha=alpha):
-                    break
-            if not self.stepLimited and dlnp <= dchisq:
+            if check_step is not None and not check_step(optimizer=self, tractor=tractor,
+                                                         dlnp=dlnp, X=X, alpha=alpha):
                 break
-            if self.stepLimited and dlnp <= dchisq_limited:
+            if dlnp <= (dchisq_limited if self.stepLimited else dchisq):
                 break
-        R.update(steps=step, hit_limit=self.last_step_hit_limit, ever_hit_limit=self.hit_limit)
-        return R
+        return dict(steps=k, hit_limit=self.last_step_hit_limit, ever_hit_limit=self.hit_limit)
 
     def tryUpdates(self, tractor, X, alphas=None, check_step=None):
-        p_best = tractor.getParams()
-        logprob_before = tractor.getLogProb()
-        logprob_best = logprob_before
-        alpha_best = None
+        """The constrained line search (constrained_optimizer.py:36-82) ->
+        (dlnp, alpha); leaves the tractor at the accepted step (or where it
+        started)."""
+        p_start = tractor.getParams()
         steps = self.getParameterSteps(tractor, X, alphas)
-        if len(steps) == 0:
+        if not steps:
             self.stepLimited = True
             self.last_step_hit_limit = True
             self.hit_limit = True
-        for alpha, p, step_limit, hit_limit in steps:
-            if hit_limit:
+        def visit(i):
+            # a step that reached a bound marks the search once it is tried
+            if steps[i][3]:
                 self.hit_limit = True
-            tractor.setParams(p)
-            if check_step is not None:
+        if check_step is None:
+            lnps = tractor._log_probs([None] + [p for (_, p, _, _) in steps])
+            lnp0, trial = lnps[0], lnps[1:]
+        else:
+            # a callback may end the search before a step is evaluated:
+            # evaluate them one at a time
+            lnp0 = tractor.getLogProb()
+            trial = []
+            for i, (alpha, p, _, _) in enumerate(steps):
+                visit(i)
+                tractor.setParams(p)
                 if not check_step(optimizer=self, tractor=tractor, X=X, alpha=alpha):
                     break
-            logprob = tractor.getLogProb()
-            if not np.isfinite(logprob):
-                break
-            if logprob < (logprob_best - 1.):
-                break
-            if logprob > logprob_best:
-                self.last_step_hit_limit = hit_limit
-                alpha_best = alpha
-                logprob_best = logprob
-                p_best = p
-        tractor.setParams(p_best)
-        if alpha_best is None:
+                trial.append(tractor.getLogProb())
+        best, lnp = walk_line_search(lnp0, trial, visit if check_step is None else None)
+        if best is None:
+            tractor.setParams(p_start)
             return 0., 0.
-        return logprob_best - logprob_before, alpha_best
+        alpha, p, _, hit = steps[best]
+        self.last_step_hit_limit = hit
+        tractor.setParams(p)
+        return lnp - lnp0, alpha
 
     def getParameterSteps(self, tractor, step_direction, alphas):
-        """constrained_optimizer.py:84-159."""
-        if alphas is None:
-            alphas = ALPHAS
-        p0 = tractor.getParams()
-        max_step_size = tractor.getMaxStep()
-        lowers = tractor.getLowerBounds()
-        uppers = tractor.getUpperBounds()
+        """[(alpha, parameters, step_limit, hit_limit)] for each alpha in
+        increasing order (constrained_optimizer.py:84-159): alpha is cut to
+        the largest value whose step stays within every parameter's maximum
+        step (step_limit; the last trial) and then to where the step meets
+        a bound (hit_limit); a cut below tiny_alpha ends the list."""
+        alphas = ALPHAS if alphas is None else alphas
         assert np.all(np.diff(alphas) > 0)
-        step_direction = np.array(step_direction)
-        results = []
-        for alpha in alphas:
-            do_break = False
-            step_limit = False
-            for i, mx in enumerate(max_step_size):
-                if mx is None:
-                    continue
-                step = alpha * abs(step_direction[i])
-                if step > mx:
-                    step_limit = True
-                    do_break = True
-                    alpha = mx / abs(step_direction[i])
-                    self.stepLimited = True
-            hit_limit = False
-            for i, (p_start, l, u) in enumerate(zip(p0, lowers, uppers)):
-                px = p_start + alpha * step_direction[i]
-                if l is not None and px < l:
-                    alpha = (l - p_start) / step_direction[i]
-                    hit_limit = True
-                if u is not None and px > u:
-                    alpha = (u - p_start) / step_direction[i]
-                    hit_limit = True
-            if alpha < self.tiny_alpha:
+        p0 = [float(v) for v in tractor.getParams()]
+        X = [float(v) for v in step_direction]
+        maxstep = tractor.getMaxStep()
+        bounds = list(zip(tractor.getLowerBounds(), tractor.getUpperBounds()))
+        out = []
+        for a in alphas:
+            limited = False
+            for mx, x in zip(maxstep, X):
+                if mx is not None and a * abs(x) > mx:
+                    a, limited = mx / abs(x), True
+            if limited:
+                self.stepLimited = True
+            hit = False
+            for p, x, (lo, hi) in zip(p0, X, bounds):
+                v = p + a * x
+                if lo is not None and v < lo:
+                    a, v, hit = (lo - p) / x, lo, True
+                if hi is not None and v > hi:
+                    a, hit = (hi - p) / x, True
+            if a < self.tiny_alpha:
                 break
-            p = []
-            for p_start, s, l, u in zip(p0, step_direction, lowers, uppers):
-                px = p_start + alpha * s
-                if l is not None and px < l:
-                    px = l
-                if u is not None and px > u:
-                    px = u
-                p.append(px)
-            results.append((alpha, p, step_limit, hit_limit))
-            if do_break:
+            out.append((a, [_within(p + a * x, lo, hi) for p, x, (lo, hi) in zip(p0, X, bounds)],
+                        limited, hit))
+            if limited:
                 break
-        return results
+        return out
 
 
 class GpuOptimizer(ConstrainedOptimizer):

@@ -1,14 +1,23 @@
-"""Exp / Dev / Gaussian galaxies (tractor/galaxy.py): MoG profiles rendered
-through the FFT path (pixelized PSF) or the pixel-space MoG path (MoG PSF)
-on the device, finite-difference derivatives as the reference takes them."""
+"""Exp / Dev / Gaussian galaxies (tractor/galaxy.py).
+
+Every render is planned into a RenderPlan (tractor_amd.plan): the
+reference's extent, FFT-size and hybrid-split rules run on the host when the
+render is planned, the pixels come from one batched device call per bucket.
+The single-call methods (getUnitFluxModelPatch, getModelPatch,
+getParamDerivatives) plan, run and read one plan; Tractor.getModelImage,
+getDerivs and the optimizers' line searches plan every source together.
+"""
+import math
+
 import numpy as np
 
 from . import mixture_profiles as mp
-from .utils import ParamList, MultiParams
+from . import render as R
 from .patch import Patch, ModelMask
+from .plan import Pending, ready, resolve
 from .pointsource import SingleProfileSource
 from .render import get_overlapping_region
-from . import gpu_render as G
+from .utils import ParamList, MultiParams
 
 
 class GalaxyShape(ParamList):
@@ -33,6 +42,58 @@ class GalaxyShape(ParamList):
         re_deg = max(1. / 30, self.re) / 3600.
         cp, sp = np.cos(phi), np.sin(phi)
         return re_deg * np.array([[cp, sp * self.ab], [-sp, cp * self.ab]])
+
+
+def _image_box(img, px, py, halfsize):
+    """The no-modelMask render box (tractor/galaxy.py:273-290): the square
+    of half-size `halfsize` about (px, py), clipped to the image, as
+    (x0, x1, y0, y1), or None when it misses the image."""
+    H, W = img.shape
+    xs, xin = get_overlapping_region(int(np.floor(px - halfsize)),
+                                     int(np.ceil(px + halfsize + 1)), 0, W - 1)
+    ys, yin = get_overlapping_region(int(np.floor(py - halfsize)),
+                                     int(np.ceil(py + halfsize + 1)), 0, H - 1)
+    if xin == [] or yin == []:
+        return None
+    return xs.start, xs.stop, ys.start, ys.stop
+
+
+def _distance_outside(px, py, mm):
+    """How far (px, py) lies outside the modelMask, or None when it is
+    inside (the source-out test of tractor/galaxy.py:354-365)."""
+    x0, y0, x1, y1 = mm.x0, mm.y0, mm.x0 + mm.w, mm.y0 + mm.h
+    if x0 <= px <= x1 - 1 and y0 <= py <= y1 - 1:
+        return None
+    ddx = (x0 - px) if px < x0 else ((px - x1) if px > x1 else 0.)
+    ddy = (y0 - py) if py < y0 else ((py - y1) if py > y1 else 0.)
+    return math.hypot(ddx, ddy)
+
+
+def hybrid_split(amp, var, pW, nsigma_in, nsigma_out):
+    """Split a sheared profile between the FFT and the pixel-space mixture
+    for a HybridPSF (tractor/galaxy.py:412-455): components wider than
+    nsigma_out sigma at the FFT box's edge (pW / 2) go to the mixture, those
+    narrower than nsigma_in to the FFT, with linear weights in between.
+    Returns ((fft_amp, fft_var) or None, (mog_amp, index) or None)."""
+    vv = var[:, 0, 0] + var[:, 1, 1]
+    half2 = (pW / 2.) ** 2
+    to_mog = half2 < nsigma_out ** 2 * vv
+    to_fft = half2 > nsigma_in ** 2 * vv
+    both = np.any(to_mog & to_fft)
+    ns = (pW / 2.) / np.maximum(1e-6, np.sqrt(vv))
+    width = nsigma_out - nsigma_in
+    mog = fft = None
+    if np.any(to_mog):
+        a = amp[to_mog]
+        if both:
+            a = a * np.minimum(1., (nsigma_out - ns[to_mog]) / width)
+        mog = (a, to_mog)
+    if np.any(to_fft):
+        a = amp[to_fft]
+        if both:
+            a = a * np.minimum(1., (ns[to_fft] - nsigma_in) / width)
+        fft = (a, var[to_fft])
+    return fft, mog
 
 
 class Galaxy(MultiParams, SingleProfileSource):
@@ -72,70 +133,89 @@ class Galaxy(MultiParams, SingleProfileSource):
         c.liquid = list(self.liquid)
         return c
 
-    def getParamDerivatives(self, img, modelMask=None, get_patch0=False, **kwargs):
-        """tractor/galaxy.py:137-223."""
-        pos0 = self.getPosition()
+    def getParamDerivatives(self, img, modelMask=None, **kwargs):
+        """Derivative patches of the thawed parameters (tractor/galaxy.py:
+        137-223): position by central differences of the unit render through
+        the WCS, brightness by scaling it, shape by forward differences of
+        renders on the unit render's window (force_sourceOut)."""
+        return resolve(self._plan_derivs, img, modelMask=modelMask, **kwargs)
+
+    def _stepped(self, plan, img, param, i, step, **kw):
+        """A unit render planned with parameter i of `param` moved by step
+        (the parameter is restored at once: the plan has read it)."""
+        vals = param.getParams()
+        old = param.setParam(i, vals[i] + step)
+        try:
+            return self._plan_unit(plan, img, **kw)
+        finally:
+            param.setParam(i, old)
+
+    def _plan_derivs(self, plan, img, modelMask=None, **kwargs):
         wcs = img.getWcs()
-        px0, py0 = wcs.positionToPixel(pos0, self)
+        pos0 = self.getPosition()
+        px, py = wcs.positionToPixel(pos0, self)
         counts = img.getPhotoCal().brightnessToCounts(self.brightness)
-        minsb = img.modelMinval
-        minval = minsb / counts if counts > 0 else None
-        patch0 = self.getUnitFluxModelPatch(img, px=px0, py=py0, minval=minval,
-                                            modelMask=modelMask, **kwargs)
-        derivs = []
-        if get_patch0:
-            derivs.append(patch0)
-        if patch0 is None:
-            derivs.extend([None] * self.numberOfParams())
-            return derivs
-        if modelMask is None:
-            x0, x1, y0, y1 = patch0.getExtent()
-            modelMask = ModelMask.fromExtent(x0, x1, y0, y1)
+        minval = img.modelMinval / counts if counts > 0 else None
+        base = self._plan_unit(plan, img, px=px, py=py, minval=minval, modelMask=modelMask,
+                               **kwargs)
+        n = self.numberOfParams()
+        if base.extent is None:
+            return ready([None] * n)
+        window = modelMask if modelMask is not None else ModelMask(*base.extent)
+        # one entry per thawed parameter, in parameter order: None, or how
+        # to make the column from the unit render
+        cols = []
         if self.isParamThawed('pos') and self.pos.numberOfParams() > 0:
-            if counts == 0:
-                derivs.extend([None] * len(pos0.getParams()))
-            else:
-                p0 = patch0.patch
-                dx = np.zeros_like(p0)
-                dx[1:-1, 1:-1] = (p0[1:-1, :-2] - p0[1:-1, 2:]) / 2.
-                dy = np.zeros_like(p0)
-                dy[1:-1, 1:-1] = (p0[:-2, 1:-1] - p0[2:, 1:-1]) / 2.
-                derivs.extend(wcs.pixelDerivsToPositionDerivs(
-                    pos0, self, counts, patch0, Patch(patch0.x0, patch0.y0, dx),
-                    Patch(patch0.x0, patch0.y0, dy)))
-        if not self.isParamFrozen('brightness'):
-            bsteps = self.brightness.getStepSizes()
-            params = self.brightness.getParams()
-            for i, bstep in enumerate(bsteps):
-                old = self.brightness.setParam(i, params[i] + bstep)
-                ci = img.getPhotoCal().brightnessToCounts(self.brightness)
-                self.brightness.setParam(i, old)
-                if ci == counts:
-                    df = None
-                else:
-                    df = patch0 * ((ci - counts) / bstep)
-                    df.setName('d(%s)/d(bright%i)' % (self.dname, i))
-                derivs.append(df)
+            npos = len(pos0.getParams())
+            cols += [None] * npos if counts == 0 else [('pos', npos)]
+        cols += self._brightness_columns(img, counts)
         if not self.isParamFrozen('shape'):
-            gsteps = self.shape.getStepSizes()
-            gnames = self.shape.getParamNames()
-            oldvals = self.shape.getParams()
+            names = self.shape.getParamNames()
             if counts == 0:
-                derivs.extend([None] * len(oldvals))
-                gsteps = []
-            for i, gstep in enumerate(gsteps):
-                old = self.shape.setParam(i, oldvals[i] + gstep)
-                patchx = self.getUnitFluxModelPatch(img, px=px0, py=py0, minval=minval,
-                                                    modelMask=modelMask, force_sourceOut=True,
-                                                    **kwargs)
-                self.shape.setParam(i, old)
-                if patchx is None:
-                    derivs.append(None)
-                    continue
-                dx = (patchx - patch0) * (counts / gstep)
-                dx.setName('d(%s)/d(%s)' % (self.dname, gnames[i]))
-                derivs.append(dx)
-        return derivs
+                cols += [None] * len(names)
+            else:
+                for i, step in enumerate(self.shape.getStepSizes()):
+                    r = self._stepped(plan, img, self.shape, i, step, px=px, py=py,
+                                      minval=minval, modelMask=window, force_sourceOut=True,
+                                      **kwargs)
+                    cols.append(('diff', r, counts / step,
+                                 'd(%s)/d(%s)' % (self.dname, names[i])))
+        cols += self._extra_columns(plan, img, px, py, modelMask, counts, kwargs)
+
+        def done():
+            p0 = base.get()
+            out = []
+            for c in cols:
+                if c is None:
+                    out.append(None)
+                elif c[0] == 'pos':
+                    gx, gy = _central(p0)
+                    out.extend(wcs.pixelDerivsToPositionDerivs(pos0, self, counts, p0, gx, gy))
+                elif c[0] == 'scale':
+                    d = p0 * c[1]
+                    d.setName(c[2])
+                    out.append(d)
+                else:
+                    q = c[1].get()
+                    if q is None:
+                        out.append(None)
+                        continue
+                    d = (q - p0) * c[2]
+                    d.setName(c[3])
+                    out.append(d)
+            return out
+        return Pending(base.extent, done)
+
+    def _extra_columns(self, plan, img, px, py, modelMask, counts, kwargs):
+        """Columns of parameters beyond position, brightness and shape (the
+        Sersic index); none here."""
+        return []
+
+
+def _central(p0):
+    from .psf import central_differences
+    gx, gy = central_differences(p0.patch)
+    return Patch(p0.x0, p0.y0, gx), Patch(p0.x0, p0.y0, gy)
 
 
 class ProfileGalaxy(object):
@@ -143,133 +223,93 @@ class ProfileGalaxy(object):
 
     def getUnitFluxModelPatch(self, img, px=None, py=None, minval=0.0, modelMask=None,
                               **kwargs):
-        if px is None or py is None:
-            px, py = img.getWcs().positionToPixel(self.getPosition(), self)
-        patch = self._realGetUnitFluxModelPatch(img, px, py, minval, modelMask=modelMask,
-                                                **kwargs)
+        patch = resolve(self._plan_unit, img, px=px, py=py, minval=minval,
+                        modelMask=modelMask, **kwargs)
         if patch is not None and modelMask is not None:
             assert patch.shape == modelMask.shape
         return patch
 
-    def _realGetUnitFluxModelPatch(self, img, px, py, minval, modelMask=None,
-                                   inner_real_nsigma=3., outer_real_nsigma=4.,
-                                   force_halfsize=None, force_sourceOut=False, **kwargs):
+    def _plan_mog(self, plan, img, psf, px, py, amix, mm, box, minval):
+        """The profile `amix` (means already at the source) convolved with
+        the PSF's mixture and evaluated in pixel space (run_mog,
+        tractor/galaxy.py:307-334): on the modelMask (masked rules when it
+        has a pixel map, else the grid), or without one on `box` =
+        (x0, x1, y0, y1) by mixture_to_patch (grid, or ring growth when
+        minval > 0)."""
+        cmix = amix.convolve(psf.getMixtureOfGaussians(px=px, py=py))
+        if mm is None:
+            return mp.plan_mixture(plan, cmix, box[0], box[1], box[2], box[3], minval)
+        masked = mm.mask is not None
+        slot = plan.mog(cmix.amp, cmix.mean, cmix.var, mm.x0, mm.y0, mm.w, mm.h,
+                        mask=mm.mask, masked_rules=masked)
+        x0, y0 = mm.x0, mm.y0
+        return Pending((x0, y0, mm.w, mm.h), lambda: Patch(x0, y0, plan.result(slot)))
+
+    def _plan_unit(self, plan, img, px=None, py=None, minval=0., modelMask=None,
+                   inner_real_nsigma=3., outer_real_nsigma=4., force_halfsize=None,
+                   force_sourceOut=False, **kwargs):
+        """Plan the unit-flux render of _realGetUnitFluxModelPatch
+        (tractor/galaxy.py:266-519)."""
         from .psf import HybridPSF
-        if modelMask is not None:
-            x0, y0 = modelMask.x0, modelMask.y0
-        else:
-            halfsize = self._getUnitFluxPatchSize(img, px=px, py=py, minval=minval)
-            H, W = img.shape
-            outx, inx = get_overlapping_region(int(np.floor(px - halfsize)),
-                                               int(np.ceil(px + halfsize + 1)), 0, W - 1)
-            outy, iny = get_overlapping_region(int(np.floor(py - halfsize)),
-                                               int(np.ceil(py + halfsize + 1)), 0, H - 1)
-            if inx == [] or iny == []:
-                return None
-            x0, x1 = outx.start, outx.stop
-            y0, y1 = outy.start, outy.stop
+        if px is None or py is None:
+            px, py = img.getWcs().positionToPixel(self.getPosition(), self)
         psf = img.getPsf()
         hybrid = isinstance(psf, HybridPSF)
-
-        def run_mog(amix=None, mm=None):
-            if amix is None:
-                amix = self._getAffineProfile(img, px, py)
-            if mm is None:
-                mm = modelMask
-            psfmix = psf.getMixtureOfGaussians(px=px, py=py)
-            cmix = amix.convolve(psfmix)
-            if mm is None:
-                if minval is not None and minval > 0:
-                    raise NotImplementedError('approximate (minval > 0) MoG rendering')
-                return Patch(x0, y0, G.mog_render(cmix.amp, cmix.mean, cmix.var, x0, y0,
-                                                  x1 - x0, y1 - y0))
-            if mm.mask is not None:
-                return Patch(mm.x0, mm.y0, G.mog_render(cmix.amp, cmix.mean, cmix.var, mm.x0,
-                                                        mm.y0, mm.w, mm.h, mask=mm.mask,
-                                                        masked_rules=True))
-            return Patch(mm.x0, mm.y0, G.mog_render(cmix.amp, cmix.mean, cmix.var, mm.x0, mm.y0,
-                                                    mm.w, mm.h))
-
+        mm = modelMask
+        box = halfsize = None
+        if mm is None:
+            halfsize = self._getUnitFluxPatchSize(img, px=px, py=py, minval=minval)
+            box = _image_box(img, px, py, halfsize)
+            if box is None:
+                return ready(None)
+        # a mixture PSF: the analytic convolution
         if hasattr(psf, 'getMixtureOfGaussians') and not hybrid:
-            return run_mog(mm=modelMask)
-
-        if modelMask is not None:
-            mh, mw = modelMask.shape
-            x1 = x0 + mw
-            y1 = y0 + mh
-            sourceOut = (px < x0 or px > x1 - 1 or py < y0 or py > y1 - 1)
-            if sourceOut:
-                neardx = neardy = 0.
-                if px < x0:
-                    neardx = x0 - px
-                if px > x1:
-                    neardx = px - x1
-                if py < y0:
-                    neardy = y0 - py
-                if py > y1:
-                    neardy = py - y1
-                nearest = np.hypot(neardx, neardy)
-                pixscale = img.wcs.pixscale_at((x0 + x1) / 2., (y0 + y1) / 2.)
-                if nearest > (self.getRadius() / pixscale) and not force_sourceOut:
-                    return None
-                if hybrid:
-                    return run_mog(mm=modelMask)
-                raise RuntimeError('sourceOut, not hybrid')
-
-        amix = self._getShearedProfile(img, px, py)
-        # FFT size and the hybrid split use the PSF spectrum's size
-        if modelMask is not None:
-            from .render import mm_halfsize
-            hs = mm_halfsize(px, py, (x0, y0, mw, mh), psf.shape)
-            if force_halfsize is not None:
-                hs = int(np.ceil(force_halfsize))
+            return self._plan_mog(plan, img, psf, px, py, self._getAffineProfile(img, px, py),
+                                  mm, box, minval)
+        if mm is not None:
+            far = _distance_outside(px, py, mm)
+            if far is not None:
+                pixscale = img.wcs.pixscale_at(mm.x0 + mm.w / 2., mm.y0 + mm.h / 2.)
+                if far > self.getRadius() / pixscale and not force_sourceOut:
+                    return ready(None)
+                if not hybrid:
+                    raise RuntimeError('sourceOut, not hybrid')
+                return self._plan_mog(plan, img, psf, px, py,
+                                      self._getAffineProfile(img, px, py), mm, None, minval)
+            mmt = (mm.x0, mm.y0, mm.w, mm.h)
             nomask = None
-            mmt = (x0, y0, mw, mh)
+            hs = (R.mm_halfsize(px, py, mmt, psf.shape) if force_halfsize is None
+                  else int(np.ceil(force_halfsize)))
         else:
-            hs = halfsize
-            nomask = (x0, x1, y0, y1, halfsize)
             mmt = None
-        pW = psf.getFourierTransformSize(hs)
-        fftamp, fftvar = amix.amp, amix.var
-        mogmix = None
+            nomask = tuple(box) + (halfsize,)
+            hs = halfsize
+        sheared = self._getShearedProfile(img, px, py)
+        fft = (sheared.amp, sheared.var)
+        mog = None
         if hybrid and inner_real_nsigma is not None and outer_real_nsigma is not None:
-            vv = amix.var[:, 0, 0] + amix.var[:, 1, 1]
-            n1, n2 = inner_real_nsigma, outer_real_nsigma
-            IM = ((pW / 2) ** 2 < (n2 ** 2 * vv))
-            IF = ((pW / 2) ** 2 > (n1 ** 2 * vv))
-            ramp = np.any(IM * IF)
-            ns = (pW / 2) / np.maximum(1e-6, np.sqrt(vv))
-            if np.any(IM):
-                amps = amix.amp[IM]
-                if ramp:
-                    amps = amps * np.minimum(1., (n2 - ns[IM]) / (n2 - n1))
-                mogmix = mp.MixtureOfGaussians(amps, amix.mean[IM, :] + np.array([px, py]),
-                                               amix.var[IM], quick=True)
-            if np.any(IF):
-                amps = amix.amp[IF]
-                if ramp:
-                    amps = amps * np.minimum(1., (ns[IF] - n1) / (n2 - n1))
-                fftamp, fftvar = amps, amix.var[IF]
-            else:
-                fftamp = None
-        if fftamp is not None:
-            pix = psf.pix if hybrid else psf
-            if mmt is not None and force_halfsize is not None:
-                raise NotImplementedError('force_halfsize with a modelMask')
-            ix0, iy0, Gp = G.fft_render(fftamp, fftvar, px, py, pix, mm=mmt, nomask_extent=nomask)
-        else:
-            # no FFT components: an empty image placed as the FFT path would be
-            from .render import galaxy_fft_job
-            _, _, win = galaxy_fft_job(np.zeros(1), np.zeros((1, 2, 2)), px, py, psf.shape,
-                                       mm=mmt, nomask_extent=nomask)
-            ix0, iy0 = win[0], win[1]
-            Gp = np.zeros((win[3], win[2]), np.float32)
-        if mogmix is not None:
-            gh, gw = Gp.shape
-            mpatch = run_mog(amix=mogmix, mm=modelMask if modelMask is not None
-                             else ModelMask(ix0, iy0, gw, gh))
-            Gp = Gp + mpatch.patch
-        return Patch(ix0, iy0, Gp)
+            fft, mog = hybrid_split(sheared.amp, sheared.var, psf.getFourierTransformSize(hs),
+                                    inner_real_nsigma, outer_real_nsigma)
+        pix = psf.pix if hybrid else psf
+        amp, var = fft if fft is not None else (np.zeros(1), np.zeros((1, 2, 2)))
+        sz, job, (X0, Y0, w, h) = R.galaxy_fft_job(np.asarray(amp), np.asarray(var), px, py,
+                                                   psf.shape, mm=mmt, nomask_extent=nomask,
+                                                   halfsize=hs if mmt is not None else None)
+        slot = plan.fft(pix, sz, job, w, h) if fft is not None else None
+        part = None
+        if mog is not None:
+            a, idx = mog
+            mmix = mp.MixtureOfGaussians(a, sheared.mean[idx] + np.array([px, py]),
+                                         sheared.var[idx], quick=True)
+            part = self._plan_mog(plan, img, psf, px, py, mmix,
+                                  mm if mm is not None else ModelMask(X0, Y0, w, h), None, 0.)
+
+        def done():
+            G = plan.result(slot) if slot is not None else np.zeros((h, w), np.float32)
+            if part is not None:
+                G = G + part.get().patch
+            return Patch(X0, Y0, G)
+        return Pending((X0, Y0, w, h), done)
 
 
 class HoggGalaxy(ProfileGalaxy, Galaxy):
@@ -281,39 +321,40 @@ class HoggGalaxy(ProfileGalaxy, Galaxy):
     def getRadius(self):
         return self.nre * self.shape.re
 
+    def _basis_in_pixels(self, img, px, py):
+        return np.dot(img.getWcs().cdInverseAtPixel(px, py), self.shape.getRaDecBasis())
+
     def _getAffineProfile(self, img, px, py):
-        cdinv = img.getWcs().cdInverseAtPixel(px, py)
-        Tinv = np.dot(cdinv, self.shape.getRaDecBasis())
-        return self.getProfile().apply_affine(np.array([px, py]), Tinv)
+        return self.getProfile().apply_affine(np.array([px, py]),
+                                              self._basis_in_pixels(img, px, py))
 
     def _getShearedProfile(self, img, px, py):
-        cdinv = img.getWcs().cdInverseAtPixel(px, py)
-        Tinv = np.dot(cdinv, self.shape.getRaDecBasis())
-        return self.getProfile().apply_shear(Tinv)
+        return self.getProfile().apply_shear(self._basis_in_pixels(img, px, py))
 
     def _getUnitFluxPatchSize(self, img, px=0., py=0., minval=0.):
+        """The no-modelMask half-size (galaxy.py:678-691): nre r_e in
+        pixels, no more than the distance to the far image edge, no less than
+        the PSF radius."""
         if hasattr(self, 'halfsize'):
             return self.halfsize
-        pixscale = img.wcs.pixscale_at(px, py)
-        halfsize = max(1., self.getRadius() / pixscale)
         h, w = img.shape
-        imsize = max(px, w - 1 - px, py, h - 1 - py) + 1
-        halfsize = min(halfsize, imsize)
-        halfsize = max(halfsize, img.psf.getRadius())
-        return int(np.ceil(halfsize))
+        far_edge = max(px, w - 1 - px, py, h - 1 - py) + 1
+        r = min(max(1., self.getRadius() / img.wcs.pixscale_at(px, py)), far_edge)
+        return int(np.ceil(max(r, img.psf.getRadius())))
 
     def getDerivativeShearedProfiles(self, img, px, py):
-        derivs = []
+        """(name, sheared profile at the stepped shape, step) per thawed
+        shape parameter (tractor/galaxy.py:693-709)."""
+        out = []
         if self.isParamThawed('shape'):
-            gsteps = self.shape.getStepSizes()
-            gnames = self.shape.getParamNames()
-            oldvals = self.shape.getParams()
-            for i, gstep in enumerate(gsteps):
-                old = self.shape.setParam(i, oldvals[i] + gstep)
+            names = self.shape.getParamNames()
+            for i, step in enumerate(self.shape.getStepSizes()):
+                vals = self.shape.getParams()
+                old = self.shape.setParam(i, vals[i] + step)
                 pro = self._getShearedProfile(img, px, py)
                 self.shape.setParam(i, old)
-                derivs.append(('shape.' + gnames[i], pro, gstep))
-        return derivs
+                out.append(('shape.' + names[i], pro, step))
+        return out
 
 
 class GaussianGalaxy(HoggGalaxy):

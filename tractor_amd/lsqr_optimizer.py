@@ -1,211 +1,219 @@
 """LsqrOptimizer: linearized least-squares update + line search
 (tractor/lsqr_optimizer.py:253-644).
 
-The update direction is the (column-scaled, damped) least-squares solution the
-reference obtains with scipy's LSQR; here the normal equations are formed on
-the device (FP32 products, FP64 sums) and solved by FP64 Cholesky
-(trh_normal_eq_dense / trh_solve_normal_f64).  Column scaling, the 1e-8 drop of
-tiny scaled values, prior rows and the shared-parameter map follow the
-reference.
+The update direction is the least-squares solution the reference obtains
+with scipy's LSQR on its sparse, column-scaled system; here the normal
+equations of that system are formed on the device (FP32 products, FP64
+sums, trh_normal_eq_dense) and solved by FP64 Cholesky
+(trh_solve_normal_f64).  The system's rules are the reference's: one row
+per image pixel (images stacked in the order their derivatives first
+appear) and one per prior term; weights inverr; columns scaled to unit norm
+and entries at most 1e-8 dropped after scaling; duplicate parameters merged.
 """
 import numpy as np
 import torch
 
-from .optimize import Optimizer
-from .engine import logverb
-from .utils import listmax
 from . import _lib
 from ._lib import check, ptr
+from .optimize import Optimizer
 
 
 def solve_least_squares(cols, b, nrows, ncols, damp=0.):
-    """Least squares over sparse columns: cols[c] = (row indices, values);
-    returns x minimising ||A x - b||^2 + damp^2 ||x||^2 (zero for empty
-    columns), computed on the device."""
+    """min ||A x - b||^2 + damp^2 ||x||^2 over sparse columns cols[c] =
+    (row indices, values), on the device (zero for empty columns)."""
     touched = [r for (r, v) in cols if len(r)]
-    if len(touched) == 0:
+    if not touched:
         return np.zeros(ncols)
     rows = np.unique(np.concatenate(touched))
-    nr = len(rows)
-    A = np.zeros((ncols + 1, nr), np.float32)
+    A = np.zeros((ncols + 1, len(rows)), np.float32)
     for c, (r, v) in enumerate(cols):
         if len(r):
             np.add.at(A[c], np.searchsorted(rows, r), v)
     A[ncols] = b[rows]
     dev = 'cuda'
     dA = torch.from_numpy(A.ravel()).to(dev)
-    nb = torch.zeros((ncols + 1) * (ncols + 1), dtype=torch.float64, device=dev)
+    nb = torch.zeros((ncols + 1) ** 2, dtype=torch.float64, device=dev)
     x = torch.zeros(ncols, dtype=torch.float64, device=dev)
     lib = _lib.load()
-    h = _lib.handle(0)
-    check(lib.trh_normal_eq_dense(h.value, ptr(dA), nr, ncols, ptr(nb), None), 'normal_eq_dense')
+    h = _lib.handle(0).value
+    check(lib.trh_normal_eq_dense(h, ptr(dA), len(rows), ncols, ptr(nb), None), 'normal_eq_dense')
     info = np.zeros(1, np.int32)
-    # solve in the (already scaled) variables: no further column scaling
-    check(lib.trh_solve_normal_f64(h.value, ptr(nb), ncols, float(damp), 0, ptr(x), ptr(info), None),
+    # the columns are already scaled: no further scaling in the solve
+    check(lib.trh_solve_normal_f64(h, ptr(nb), ncols, float(damp), 0, ptr(x), ptr(info), None),
           'solve_normal_f64')
     if info[0] != 0:
-        # rank deficient: a tiny ridge gives the minimum-norm-like solution
-        check(lib.trh_solve_normal_f64(h.value, ptr(nb), ncols, 1e-6, 0, ptr(x), ptr(info), None),
+        # rank deficient: a tiny ridge picks a finite solution
+        check(lib.trh_solve_normal_f64(h, ptr(nb), ncols, 1e-6, 0, ptr(x), ptr(info), None),
               'solve_normal_f64 (ridge)')
     return x.cpu().numpy()
+
+
+class _SparseSystem(object):
+    """The weighted design matrix of one linearized step, column by column."""
+
+    def __init__(self, ncols):
+        self.first_row = {}       # image -> its first row
+        self.nrows = 0
+        self.rows = [[] for _ in range(ncols)]
+        self.vals = [[] for _ in range(ncols)]
+        self.sumsq = np.zeros(ncols)
+
+    def image_row(self, img):
+        r = self.first_row.get(img)
+        if r is None:
+            r = self.first_row[img] = self.nrows
+            self.nrows += img.numberOfPixels()
+        return r
+
+    def add(self, col, rows, vals, keep=True):
+        self.sumsq[col] += np.dot(vals, vals)
+        if keep:
+            self.rows[col].append(rows)
+            self.vals[col].append(vals)
+
+    def add_patch(self, col, deriv, img, keep=True):
+        """deriv x inverr on the image pixels it covers, nonzero entries only."""
+        H, W = img.shape
+        d = deriv.copy()
+        d.clipTo(W, H)
+        pix = d.getPixelIndices(img)
+        if len(pix) == 0:
+            return
+        v = (d.getImage() * img.getInvError()[d.getSlice(img)]).ravel()
+        nz = np.flatnonzero(v)
+        if len(nz):
+            self.add(col, self.image_row(img) + pix[nz], v[nz], keep)
+
+    def entries(self):
+        for c in range(len(self.rows)):
+            for r, v in zip(self.rows[c], self.vals[c]):
+                yield c, r, v
 
 
 class LsqrOptimizer(Optimizer):
 
     def getLinearUpdateDirection(self, tractor, damp=0, priors=True, scale_columns=True,
                                  shared_params=True, variance=False, **kwargs):
-        allderivs = tractor.getDerivs()
-        return self.getUpdateDirection(tractor, allderivs, damp=damp, priors=priors,
+        return self.getUpdateDirection(tractor, tractor.getDerivs(), damp=damp, priors=priors,
                                        scale_columns=scale_columns, shared_params=shared_params,
                                        variance=variance)
 
     def optimize(self, tractor, alphas=None, damp=0, priors=True, scale_columns=True,
                  shared_params=True, variance=False, just_variance=False, **nil):
-        X = self.getLinearUpdateDirection(tractor, damp=damp, priors=priors,
-                                          scale_columns=scale_columns, shared_params=shared_params,
-                                          variance=variance)
-        if X is None:
+        """(dlnp, X, alpha), with the variances appended when `variance`
+        (only them with just_variance); (0, None, 0) when no direction can
+        be found (lsqr_optimizer.py:280-315)."""
+        got = self.getLinearUpdateDirection(tractor, damp=damp, priors=priors,
+                                            scale_columns=scale_columns,
+                                            shared_params=shared_params, variance=variance)
+        if got is None:
             return (0., None, 0.)
+        var = None
         if variance:
-            if len(X) == 0:
-                return 0, X, 0, None
-            X, var = X
+            if len(got) == 0:
+                return 0, got, 0, None
+            got, var = got
             if just_variance:
                 return var
-        if len(X) == 0:
-            return 0, X, 0.
-        dlogprob, alpha = self.tryUpdates(tractor, X, alphas=alphas)
-        if variance:
-            return dlogprob, X, alpha, var
-        return dlogprob, X, alpha
+        if len(got) == 0:
+            return (0, got, 0.) if not variance else (0, got, 0., var)
+        dlnp, alpha = self.tryUpdates(tractor, got, alphas=alphas)
+        return (dlnp, got, alpha) if not variance else (dlnp, got, alpha, var)
 
     def optimize_loop(self, tractor, dchisq=0., steps=50, **kwargs):
-        R = {}
-        step = 0
-        for step in range(steps):
-            dlnp, X, alpha = self.optimize(tractor, **kwargs)
-            if dlnp <= dchisq:
+        """Steps until one gains at most dchisq (lsqr_optimizer.py:317-326);
+        'steps' is the index of the last one."""
+        k = 0
+        for k in range(steps):
+            if self.optimize(tractor, **kwargs)[0] <= dchisq:
                 break
-        R.update(steps=step)
-        return R
+        return dict(steps=k)
 
     def getUpdateDirection(self, tractor, allderivs, damp=0., priors=True, scale_columns=True,
                            scales_only=False, chiImages=None, variance=False,
                            shared_params=True, get_A_matrix=False):
-        """tractor/lsqr_optimizer.py:328-644."""
-        if shared_params:
-            p0 = tractor.getParams()
-            tractor.setParams(np.arange(len(p0)))
-            p1 = tractor.getParams()
-            tractor.setParams(p0)
-            U, I = np.unique(p1, return_inverse=True)
-            paramindexmap = I
-        sprows, spcols, spvals = [], [], []
-        imgoffs = {}
-        nextrow = 0
+        """The least-squares step X of tractor/lsqr_optimizer.py:328-644 (see
+        the module docstring); [] when nothing constrains it, None on
+        non-finite derivatives; (X, 1 / column norms^2) with `variance`;
+        the column norms alone with scales_only."""
+        ncols = len(allderivs)
+        sysm = _SparseSystem(ncols)
         for param in allderivs:
-            for deriv, img in param:
-                if img in imgoffs:
-                    continue
-                imgoffs[img] = nextrow
-                nextrow += img.numberOfPixels()
-        Nrows = nextrow
-        Ncols = len(allderivs)
-        colscales = np.zeros(Ncols)
+            for _, img in param:
+                sysm.image_row(img)
         for col, param in enumerate(allderivs):
-            for (deriv, img) in param:
-                inverrs = img.getInvError()
-                H, W = img.shape
-                row0 = imgoffs[img]
-                deriv = deriv.copy()
-                deriv.clipTo(W, H)
-                pix = deriv.getPixelIndices(img)
-                if len(pix) == 0:
-                    continue
-                dimg = deriv.getImage()
-                w = inverrs[deriv.getSlice(img)]
-                vals = dimg * w
-                nz = np.flatnonzero(vals)
-                if len(nz) == 0:
-                    continue
-                vals = vals.flat[nz]
-                if scale_columns:
-                    colscales[col] += np.dot(vals, vals)
-                if scales_only:
-                    continue
-                sprows.append(row0 + pix[nz])
-                spvals.append(vals)
-                spcols.append(col)
+            for deriv, img in param:
+                sysm.add_patch(col, deriv, img, keep=not scales_only)
         b = None
         if priors:
-            X = tractor.getLogPriorDerivatives()
-            if X is not None:
-                rA, cA, vA, pb, _ = X
-                for ri, ci, vi in zip(rA, cA, vA):
-                    if scale_columns:
-                        colscales[ci] += np.dot(vi, vi)
-                    if scales_only:
-                        continue
-                    sprows.append(np.array(ri) + Nrows)
-                    spcols.append(ci)
-                    spvals.append(np.array(vi, np.float64))
-                oldnrows = Nrows
-                Nrows += listmax(rA, -1) + 1
-                b = np.zeros(Nrows, np.float32)
-                b[oldnrows:] = np.hstack(pb)
-        if len(spcols) == 0:
+            pd = tractor.getLogPriorDerivatives()
+            if pd is not None:
+                prow, pcol, pval, pb, _ = pd
+                base = sysm.nrows
+                for r, c, v in zip(prow, pcol, pval):
+                    sysm.add(c, np.asarray(r) + base, np.asarray(v, np.float64),
+                             keep=not scales_only)
+                sysm.nrows += max([max(r) for r in prow if len(r)] or [-1]) + 1
+                b = np.zeros(sysm.nrows, np.float32)
+                b[base:] = np.hstack(pb)
+        if not any(sysm.rows) and not scales_only:
             return []
         if scales_only:
-            return np.sqrt(colscales)
-        if variance:
-            var = colscales.copy()
-        if scale_columns:
-            colscales = np.sqrt(colscales)
-            rr, cc, vv = [], [], []
-            for r, c, v in zip(sprows, spcols, spvals):
-                scale = colscales[c]
-                if scale == 0:
-                    continue
-                v = v / scale
-                I = np.flatnonzero(np.abs(v) > 1e-8)   # drop tiny values (lsqr_optimizer.py:479-481)
-                if len(I) == 0:
-                    continue
-                rr.append(r[I])
-                vv.append(v[I])
-                cc.append(c)
-            sprows, spcols, spvals = rr, cc, vv
+            return np.sqrt(sysm.sumsq)
+        var = sysm.sumsq.copy() if variance else None
+        scale = np.sqrt(sysm.sumsq) if scale_columns else None
+        colmap = None
         if shared_params:
-            spcols = [paramindexmap[c] for c in spcols]
-            Ncols = int(np.max(spcols)) + 1 if len(spcols) else 0
+            # columns of the same underlying parameter merge
+            # (lsqr_optimizer.py:367-376, 495-505)
+            p_now = tractor.getParams()
+            tractor.setParams(np.arange(len(p_now)))
+            ids = tractor.getParams()
+            tractor.setParams(p_now)
+            _, colmap = np.unique(ids, return_inverse=True)
+        cols = {}
+        for c, r, v in sysm.entries():
+            if scale is not None:
+                if scale[c] == 0:
+                    continue
+                v = v / scale[c]
+                keep = np.flatnonzero(np.abs(v) > 1e-8)
+                if len(keep) == 0:
+                    continue
+                r, v = r[keep], v[keep]
+            cc = int(colmap[c]) if colmap is not None else c
+            cols.setdefault(cc, []).append((r, v))
+        if not cols:
+            return []
+        n_out = int(colmap.max()) + 1 if colmap is not None else ncols
         if b is None:
-            b = np.zeros(Nrows, np.float32)
-        chimap = {}
-        if chiImages is not None:
-            for img, chi in zip(tractor.getImages(), chiImages):
-                chimap[img] = chi
-        for img, row0 in imgoffs.items():
-            chi = chimap.get(img, None)
+            b = np.zeros(sysm.nrows, np.float32)
+        chis = dict(zip(tractor.getImages(), chiImages)) if chiImages is not None else {}
+        for img, r0 in sysm.first_row.items():
+            chi = chis.get(img)
             if chi is None:
                 chi = tractor.getChiImage(img=img)
-            chi = chi.ravel()
-            b[row0:row0 + len(chi)] = chi
-        if not np.all(np.isfinite(np.hstack(spvals))):
+            chi = np.ravel(chi)
+            b[r0:r0 + len(chi)] = chi
+        allv = [v for lst in cols.values() for (_, v) in lst]
+        if not np.all(np.isfinite(np.concatenate(allv))):
             print('Warning: infinite derivatives; bailing out')
             return None
-        cols = [(np.zeros(0, np.int64), np.zeros(0))] * Ncols
-        cols = [list(c) for c in cols]
-        for r, c, v in zip(sprows, spcols, spvals):
-            cr, cv = cols[c]
-            cols[c] = [np.concatenate([cr, r]), np.concatenate([cv, v])]
-        Xs = solve_least_squares([tuple(c) for c in cols], b.astype(np.float64), Nrows, Ncols,
-                                 damp=damp)
-        if shared_params:
-            Xs = Xs[paramindexmap]
-        X = np.array(Xs, np.float64)
-        if scale_columns:
-            X[colscales > 0] /= colscales[colscales > 0]
+        packed = []
+        for c in range(n_out):
+            lst = cols.get(c, [])
+            if lst:
+                packed.append((np.concatenate([r for r, _ in lst]),
+                               np.concatenate([v for _, v in lst])))
+            else:
+                packed.append((np.zeros(0, np.int64), np.zeros(0)))
+        X = solve_least_squares(packed, b.astype(np.float64), sysm.nrows, n_out, damp=damp)
+        if colmap is not None:
+            X = X[colmap]
+        X = np.array(X, np.float64)
+        if scale is not None:
+            ok = scale > 0
+            X[ok] /= scale[ok]
         if variance:
-            if shared_params:
-                var = var[paramindexmap]
-            return X, 1. / np.array(var)
+            return X, 1. / (var[colmap] if colmap is not None else var)
         return X

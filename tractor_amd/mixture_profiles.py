@@ -72,10 +72,15 @@ class MixtureOfGaussians(object):
     def evaluate_grid(self, x0, x1, y0, y1, cx, cy):
         """Exact grid (evaluate_grid_dstn, tractor/mixture_profiles.py:270-283,
         c_gauss_2d_grid) on the device -> Patch."""
+        from .plan import resolve
+        return resolve(self.plan_grid, x0, x1, y0, y1, cx, cy)
+
+    def plan_grid(self, plan, x0, x1, y0, y1, cx, cy):
+        """evaluate_grid as a Pending in `plan`."""
         from .patch import Patch
-        from . import gpu_render as G
-        return Patch(x0, y0, G.mog_render(self.amp, self.mean, self.var, x0, y0, x1 - x0, y1 - y0,
-                                          cx, cy))
+        from .plan import Pending
+        slot = plan.mog(self.amp, self.mean, self.var, x0, y0, x1 - x0, y1 - y0, cx, cy)
+        return Pending((x0, y0, x1 - x0, y1 - y0), lambda: Patch(x0, y0, plan.result(slot)))
 
     def evaluate_grid_approx3(self, x0, x1, y0, y1, fx, fy, minval, derivs=False, minradius=3,
                               doslice=True, maxmargin=100):
@@ -109,10 +114,21 @@ class MixtureOfGaussians(object):
 
 def mixture_to_patch(mixture, x0, x1, y0, y1, minval=0., exactExtent=False):
     """tractor/mixture_profiles.py:425-438."""
+    from .plan import resolve
+    return resolve(plan_mixture, mixture, x0, x1, y0, y1, minval, exactExtent)
+
+
+def plan_mixture(plan, mixture, x0, x1, y0, y1, minval=0., exactExtent=False):
+    """mixture_to_patch as a Pending in `plan`: the exact grid at minval 0,
+    else the ring growth of approx3 (evaluated at once: its window is the
+    evaluated region)."""
+    from .plan import ready
     if minval == 0. or minval is None:
-        return mixture.evaluate_grid(x0, x1, y0, y1, 0., 0.)
-    return mixture.evaluate_grid_approx3(x0, x1, y0, y1, 0., 0., minval,
-                                         doslice=not exactExtent)
+        return mixture.plan_grid(plan, x0, x1, y0, y1, 0., 0.)
+    p = mixture.evaluate_grid_approx3(x0, x1, y0, y1, 0., 0., minval, doslice=not exactExtent)
+    if p is None:
+        return ready(None)
+    return ready(p, (p.x0, p.y0, p.patch.shape[1], p.patch.shape[0]))
 
 
 def get_exp_mixture():

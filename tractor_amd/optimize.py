@@ -8,6 +8,23 @@ from .engine import logverb, logmsg, OptResult
 ALPHAS = np.append(2. ** np.arange(-10, 1), [np.sqrt(2.), 2.])
 
 
+def walk_line_search(lnp0, lnps, visit=None):
+    """The acceptance rule of the line searches (tractor/optimize.py:
+    448-458, constrained_optimizer.py:57-77): walk the trials in order; a
+    non-finite log-probability, or one more than 1 below the best so far,
+    ends the walk; the best trial that improves on lnp0 is kept.  visit(i)
+    is called on each trial reached.  Returns (index or None, best lnp)."""
+    best_i, best = None, lnp0
+    for i, lnp in enumerate(lnps):
+        if visit is not None:
+            visit(i)
+        if not np.isfinite(lnp) or lnp < best - 1.:
+            break
+        if lnp > best:
+            best_i, best = i, lnp
+    return best_i, best
+
+
 class Optimizer(object):
     """optimize(tractor, alphas, damp, priors, scale_columns, shared_params,
     variance, just_variance) -> (dlnp, X, alpha); optimize_loop -> dict;
@@ -21,30 +38,19 @@ class Optimizer(object):
         raise NotImplementedError
 
     def tryUpdates(self, tractor, X, alphas=None):
-        """Sequential line search (tractor/optimize.py:425-470)."""
-        if alphas is None:
-            alphas = ALPHAS
-        pBefore = tractor.getLogProb()
-        pBest = pBefore
-        alphaBest = None
-        p0 = tractor.getParams()
-        for alpha in alphas:
-            pa = [p + alpha * d for p, d in zip(p0, X)]
-            tractor.setParams(pa)
-            pAfter = tractor.getLogProb()
-            if not np.isfinite(pAfter):
-                logmsg('  Got bad log-prob', pAfter)
-                break
-            if pAfter < (pBest - 1.):
-                break
-            if pAfter > pBest:
-                alphaBest = alpha
-                pBest = pAfter
-        if alphaBest is None:
+        """The line search along X (tractor/optimize.py:425-470) -> (dlnp,
+        alpha): every trial evaluated in one plan, then walked in order with
+        walk_line_search; the tractor is left at the accepted trial."""
+        alphas = ALPHAS if alphas is None else alphas
+        p0 = np.asarray(tractor.getParams(), np.float64)
+        trials = [p0 + a * np.asarray(X, np.float64) for a in alphas]
+        lnps = tractor._log_probs([None] + trials)
+        i, best = walk_line_search(lnps[0], lnps[1:])
+        if i is None:
             tractor.setParams(p0)
             return 0, 0.
-        tractor.setParams([p + alphaBest * d for p, d in zip(p0, X)])
-        return pBest - pBefore, alphaBest
+        tractor.setParams(trials[i])
+        return best - lnps[0], alphas[i]
 
     # ---- forced photometry (tractor/optimize.py:16-222, lsqr_optimizer.py:11-251)
     def forced_photometry(self, tractor, alphas=None, damp=0, priors=False, minsb=0., mindlnp=1.,
@@ -209,16 +215,31 @@ class Optimizer(object):
         return fs
 
     def _get_umodels(self, tractor, srcs, imgs, minsb, **kwargs):
-        """Unit-flux patches per (image, source brightness) (optimize.py:147-222)."""
-        umodels = []
+        """Unit-flux patches per (image, source brightness) (optimize.py:
+        147-222), every image's and source's renders planned together."""
+        from .plan import RenderPlan, ready
+        plan = RenderPlan()
+        todo = []
         for img in imgs:
             pcal = img.getPhotoCal()
-            umods = []
+            row = []
             for src in srcs:
                 counts = sum([pcal.brightnessToCounts(b) for b in src.getBrightnesses()])
                 mv = 1e-3 if counts <= 0 else minsb / counts
                 mask = tractor._getModelMaskFor(img, src)
-                ums = src.getUnitFluxModelPatches(img, minval=mv, modelMask=mask, **kwargs)
+                fn = getattr(src, '_plan_unit_patches', None)
+                if fn is None:
+                    row.append([ready(u) for u in src.getUnitFluxModelPatches(
+                        img, minval=mv, modelMask=mask, **kwargs)])
+                else:
+                    row.append(fn(plan, img, minval=mv, modelMask=mask, **kwargs))
+            todo.append(row)
+        plan.run()
+        umodels = []
+        for row in todo:
+            umods = []
+            for pend in row:
+                ums = [p.get() for p in pend]
                 for um in ums:
                     if um is not None and um.patch is not None:
                         assert np.all(np.isfinite(um.patch))

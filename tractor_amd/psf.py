@@ -4,18 +4,46 @@ HybridPixelizedPSF.  Pixels are rendered by libtractor_hip."""
 import numpy as np
 
 from .utils import BaseParams, ParamList, MultiParams
-from .mixture_profiles import MixtureOfGaussians, mixture_to_patch
+from .mixture_profiles import MixtureOfGaussians, mixture_to_patch, plan_mixture
 from .patch import Patch
-from . import gpu_render as G
+from .plan import Pending, RenderPlan, ready, resolve
 from . import render as R
 
 
 def lanczos_shift_image(img, dx, dy, inplace=False, force_python=False):
     """tractor/psf.py:26-58 -- Lanczos-3 shift on the device."""
-    out = G.lanczos_shift(img, dx, dy)
+    plan = RenderPlan()
+    s = plan.shift(img, dx, dy)
+    out = plan.run().result(s)
     if inplace:
         img[:, :] = out
     return out
+
+
+def central_differences(p):
+    """(d/dx, d/dy) of a patch by central differences, negated (the
+    derivative with respect to the source position), with a one-pixel zero
+    border: the rule of PixelizedPSF.getPointSourcePatch (tractor/psf.py:
+    196-207) and Galaxy.getParamDerivatives (tractor/galaxy.py:168-172)."""
+    gx = np.zeros_like(p)
+    gy = np.zeros_like(p)
+    gx[1:-1, 1:-1] = 0.5 * (p[1:-1, :-2] - p[1:-1, 2:])
+    gy[1:-1, 1:-1] = 0.5 * (p[:-2, 1:-1] - p[2:, 1:-1])
+    return gx, gy
+
+
+def plan_point_patch(psf, plan, px, py, **kwargs):
+    """psf's point-source patch as a Pending in `plan`; PSF classes without
+    a planner render at once (getPointSourcePatch)."""
+    fn = getattr(psf, '_plan_point', None)
+    if fn is None:
+        r = psf.getPointSourcePatch(px, py, **kwargs)
+        if r is None:
+            return ready(None)
+        p0 = r[0] if isinstance(r, tuple) else r
+        return ready(r, (p0.x0, p0.y0, p0.patch.shape[1], p0.patch.shape[0]),
+                     has_derivs=isinstance(r, tuple))
+    return fn(plan, px, py, **kwargs)
 
 
 class HybridPSF(object):
@@ -76,37 +104,53 @@ class PixelizedPSF(BaseParams):
 
     def getPointSourcePatch(self, px, py, minval=0., modelMask=None, radius=None, derivs=False,
                             **kwargs):
-        """tractor/psf.py:137-207."""
-        img = self.getImage(px, py)
-        if radius is not None:
-            R_ = int(np.ceil(radius))
-            H, W = img.shape
+        """tractor/psf.py:137-207: the Lanczos-shifted stamp, on the
+        modelMask if given (None without overlap), with (dx, dy) patches by
+        central differences when derivs and a modelMask are given."""
+        return resolve(self._plan_point, px, py, minval=minval, modelMask=modelMask,
+                       radius=radius, derivs=derivs)
+
+    def _cropped(self, radius):
+        # the stamp trimmed to `radius` about its centre (psf.py:149-156),
+        # cached per radius so its device copy is made once
+        R_ = int(np.ceil(radius))
+        crops = self.__dict__.setdefault('_crops', {})
+        if R_ not in crops:
+            H, W = self.img.shape
             cx, cy = W // 2, H // 2
-            img = img[max(cy - R_, 0):min(cy + R_ + 1, H - 1), max(cx - R_, 0):min(cx + R_ + 1, W - 1)]
-        H, W = img.shape
+            sub = self.img[max(cy - R_, 0):min(cy + R_ + 1, H - 1),
+                           max(cx - R_, 0):min(cx + R_ + 1, W - 1)]
+            c = PixelizedPSF.__new__(PixelizedPSF)
+            c.img = np.ascontiguousarray(sub, np.float32)
+            c.H, c.W = c.img.shape
+            crops[R_] = c
+        return crops[R_]
+
+    def _plan_point(self, plan, px, py, minval=0., modelMask=None, radius=None, derivs=False,
+                    **kwargs):
+        src = self if radius is None else self._cropped(radius)
+        stamp = src.img
+        H, W = stamp.shape
         ix = int(round(float(px)))
         iy = int(round(float(py)))
-        dx = px - ix
-        dy = py - iy
-        x0 = ix - W // 2
-        y0 = iy - H // 2
+        x0, y0 = ix - W // 2, iy - H // 2
         if modelMask is None:
-            return Patch(x0, y0, G.lanczos_shift(img, dx, dy))
+            slot = plan.shift(stamp, px - ix, py - iy)
+            return Pending((x0, y0, W, H), lambda: Patch(x0, y0, plan.result(slot)))
         mm = (modelMask.x0, modelMask.y0, modelMask.w, modelMask.h)
-        if img is self.img:
-            p = G.point_render(self, px, py, mm)
-        else:
-            p = G.point_render(PixelizedPSF(img), px, py, mm)
-        if p is None:
-            return None
-        patch = Patch(modelMask.x0, modelMask.y0, p)
-        if not derivs:
-            return patch
-        ddx = np.zeros_like(p)
-        ddx[1:-1, 1:-1] = -(p[1:-1, 2:] - p[1:-1, :-2]) / 2.
-        ddy = np.zeros_like(p)
-        ddy[1:-1, 1:-1] = -(p[2:, 1:-1] - p[:-2, 1:-1]) / 2.
-        return patch, Patch(patch.x0, patch.y0, ddx), Patch(patch.x0, patch.y0, ddy)
+        J = R.point_job(px, py, stamp.shape, mm)
+        if J is None:
+            return ready(None, has_derivs=derivs)
+        slot = plan.point(src, J, modelMask.w, modelMask.h)
+
+        def done():
+            p = plan.result(slot)
+            patch = Patch(mm[0], mm[1], p)
+            if not derivs:
+                return patch
+            gx, gy = central_differences(p)
+            return patch, Patch(mm[0], mm[1], gx), Patch(mm[0], mm[1], gy)
+        return Pending(mm, done, has_derivs=derivs)
 
     def getFourierTransformSize(self, radius):
         return R.fft_size(radius)
@@ -116,7 +160,8 @@ class PixelizedPSF(BaseParams):
         device half spectrum (torch complex64 view) instead of NumPy."""
         sz = self.getFourierTransformSize(radius)
         if sz not in self.fftcache:
-            spec, cx, cy = G._spectra(self).get(sz)
+            from .plan import _spectra_of
+            spec, cx, cy = _spectra_of(self).get(sz)
             self.fftcache[sz] = (spec, (cx, cy), (sz, sz),
                                  (np.fft.rfftfreq(sz), np.fft.fftfreq(sz)))
         return self.fftcache[sz]
@@ -185,61 +230,63 @@ class GaussianMixturePSF(BaseParams):
 
     def getPointSourcePatch(self, px, py, minval=0., radius=None, derivs=False, minradius=None,
                             modelMask=None, clipExtent=None, **kwargs):
-        """tractor/psf.py:458-528: exact grid, masked (with derivatives) or ring-growth
-        (minval > 0) evaluation."""
+        """tractor/psf.py:458-528: exact grid, masked (with derivatives) or
+        ring-growth (minval > 0) evaluation."""
+        return resolve(self._plan_point, px, py, minval=minval, radius=radius, derivs=derivs,
+                       minradius=minradius, modelMask=modelMask, clipExtent=clipExtent)
+
+    def _box(self, px, py, r, minval):
+        # the evaluation box about (px, py) of radius r; r None: where every
+        # component falls below minval (an overestimate)
+        if r is None:
+            r = 0.
+            for v in self.mog.var:
+                r2 = (v[0, 0] + v[1, 1]) * -2. * np.log(minval * 2. * np.pi * np.linalg.det(v))
+                r = max(r, np.sqrt(r2)) if r2 > 0 else r
+            r = int(np.ceil(r))
+        return (int(np.floor(px - r)), int(np.ceil(px + r)) + 1,
+                int(np.floor(py - r)), int(np.ceil(py + r)) + 1)
+
+    def _plan_point(self, plan, px, py, minval=0., radius=None, derivs=False, minradius=None,
+                    modelMask=None, clipExtent=None, **kwargs):
+        g = self.mog
         if modelMask is not None:
-            if modelMask.mask is None:
-                return Patch(modelMask.x0, modelMask.y0,
-                             G.mog_render(self.mog.amp, self.mog.mean, self.mog.var,
-                                          modelMask.x0, modelMask.y0, modelMask.w, modelMask.h,
-                                          px, py))
-            r = G.mog_render(self.mog.amp, self.mog.mean, self.mog.var, modelMask.x0,
-                             modelMask.y0, modelMask.w, modelMask.h, px, py,
-                             mask=modelMask.mask, derivs=derivs, masked_rules=True)
+            mm = modelMask
+            ext = (mm.x0, mm.y0, mm.w, mm.h)
+            if mm.mask is None:
+                slot = plan.mog(g.amp, g.mean, g.var, mm.x0, mm.y0, mm.w, mm.h, px, py)
+                return Pending(ext, lambda: Patch(mm.x0, mm.y0, plan.result(slot)))
+            slot = plan.mog(g.amp, g.mean, g.var, mm.x0, mm.y0, mm.w, mm.h, px, py, mask=mm.mask,
+                            derivs=derivs, masked_rules=True)
             if derivs:
-                return tuple(Patch(modelMask.x0, modelMask.y0, q) for q in r)
-            return Patch(modelMask.x0, modelMask.y0, r)
-        if minval is None:
-            minval = 0.
-        if minval > 0. or minradius is not None:
-            if radius is not None:
-                rr = radius
-            elif self.radius is not None:
-                rr = self.radius
-            else:
-                # overestimate of the radius where each component falls to minval
-                r = 0.
-                for v in self.mog.var:
-                    vv = v[0, 0] + v[1, 1]
-                    norm = 2. * np.pi * np.linalg.det(v)
-                    r2 = vv * -2. * np.log(minval * norm)
-                    if r2 > 0:
-                        r = max(r, np.sqrt(r2))
-                rr = int(np.ceil(r))
-            x0 = int(np.floor(px - rr))
-            x1 = int(np.ceil(px + rr)) + 1
-            y0 = int(np.floor(py - rr))
-            y1 = int(np.ceil(py + rr)) + 1
-            if clipExtent is not None:
-                xl, xh, yl, yh = clipExtent
-                x0, x1, y0, y1 = max(x0, xl), min(x1, xh), max(y0, yl), min(y1, yh)
-            if x0 >= x1 or y0 >= y1:
-                return None
-            kwa = {} if minradius is None else dict(minradius=minradius)
-            return self.mog.evaluate_grid_approx3(x0, x1, y0, y1, px, py, minval, derivs=derivs,
-                                                  **kwa)
-        r = self.getRadius() if radius is None else radius
-        x0 = int(np.floor(px - r))
-        x1 = int(np.ceil(px + r)) + 1
-        y0 = int(np.floor(py - r))
-        y1 = int(np.ceil(py + r)) + 1
+                return Pending(ext, lambda: tuple(Patch(mm.x0, mm.y0, q)
+                                                  for q in plan.result_derivs(slot)),
+                               has_derivs=True)
+            return Pending(ext, lambda: Patch(mm.x0, mm.y0, plan.result(slot)))
+        minval = 0. if minval is None else minval
+        approx = minval > 0. or minradius is not None
+        if approx:
+            r = radius if radius is not None else self.radius
+        else:
+            r = self.getRadius() if radius is None else radius
+        x0, x1, y0, y1 = self._box(px, py, r, minval)
         if clipExtent is not None:
             xl, xh, yl, yh = clipExtent
             x0, x1, y0, y1 = max(x0, xl), min(x1, xh), max(y0, yl), min(y1, yh)
         if x0 >= x1 or y0 >= y1:
-            return None
-        return Patch(x0, y0, G.mog_render(self.mog.amp, self.mog.mean, self.mog.var, x0, y0,
-                                          x1 - x0, y1 - y0, px, py))
+            return ready(None)
+        if approx:
+            # ring growth (minval > 0) is evaluated at once; its window is the
+            # evaluated region, known only afterwards
+            kwa = {} if minradius is None else dict(minradius=minradius)
+            r = g.evaluate_grid_approx3(x0, x1, y0, y1, px, py, minval, derivs=derivs, **kwa)
+            if r is None:
+                return ready(None)
+            p0 = r[0] if derivs else r
+            return ready(r, (p0.x0, p0.y0, p0.patch.shape[1], p0.patch.shape[0]),
+                         has_derivs=derivs)
+        slot = plan.mog(g.amp, g.mean, g.var, x0, y0, x1 - x0, y1 - y0, px, py)
+        return Pending((x0, y0, x1 - x0, y1 - y0), lambda: Patch(x0, y0, plan.result(slot)))
 
 
 class NCircularGaussianPSF(MultiParams):
@@ -299,6 +346,11 @@ class NCircularGaussianPSF(MultiParams):
     def getPointSourcePatch(self, px, py, minval=0., radius=None, modelMask=None,
                             clipExtent=None, **kwargs):
         """tractor/psf.py:898-929 via mixture_to_patch (exact grid or ring growth)."""
+        return resolve(self._plan_point, px, py, minval=minval, radius=radius,
+                       modelMask=modelMask, clipExtent=clipExtent)
+
+    def _plan_point(self, plan, px, py, minval=0., radius=None, modelMask=None, clipExtent=None,
+                    **kwargs):
         if modelMask is not None:
             x0, x1, y0, y1 = modelMask.extent
         else:
@@ -314,8 +366,8 @@ class NCircularGaussianPSF(MultiParams):
         mean[:, 0] += px
         mean[:, 1] += py
         mix = MixtureOfGaussians(mix.amp, mean, mix.var, quick=True)
-        return mixture_to_patch(mix, x0, x1, y0, y1, minval=minval,
-                                exactExtent=(modelMask is not None))
+        return plan_mixture(plan, mix, x0, x1, y0, y1, minval=minval,
+                            exactExtent=(modelMask is not None))
 
 
 class HybridPixelizedPSF(HybridPSF):

@@ -50,11 +50,14 @@ def mm_halfsize(px, py, mm, psf_shape):
     return int(np.ceil(halfsize))
 
 
-def galaxy_fft_job(amp, var_pix, px, py, psf_shape, mm=None, nomask_extent=None):
+def galaxy_fft_job(amp, var_pix, px, py, psf_shape, mm=None, nomask_extent=None,
+                   halfsize=None):
     """Plan one FFT-path render (tractor/galaxy.py:336-508).
 
     amp: (K,), var_pix: (K,2,2) sheared profile in pixels; mm: modelMask
-    (x0, y0, w, h) or None with nomask_extent (x0, x1, y0, y1, halfsize).
+    (x0, y0, w, h) or None with nomask_extent (x0, x1, y0, y1, halfsize);
+    halfsize: the FFT halfsize to use with a modelMask instead of its rule
+    (force_halfsize, galaxy.py:348-349).
     Returns (sz, job record, (X0, Y0, w, h)) where the last tuple is the
     output window in image coordinates.
     """
@@ -64,7 +67,7 @@ def galaxy_fft_job(amp, var_pix, px, py, psf_shape, mm=None, nomask_extent=None)
     psfh, psfw = psf_shape
     if mm is not None:
         x0, y0, mw, mh = mm
-        halfsize = mm_halfsize(px, py, mm, psf_shape)
+        halfsize = mm_halfsize(px, py, mm, psf_shape) if halfsize is None else halfsize
     else:
         x0, x1, y0, y1, halfsize = nomask_extent
     sz = fft_size(halfsize)

@@ -127,32 +127,21 @@ class SersicGalaxy(HoggGalaxy):
         c.liquid = list(self.liquid)
         return c
 
-    def getParamDerivatives(self, img, modelMask=None, **kwargs):
-        derivs = super(SersicGalaxy, self).getParamDerivatives(img, modelMask=modelMask,
-                                                               get_patch0=True, **kwargs)
-        patch0 = derivs.pop(0)
-        if patch0 is None or self.isParamFrozen('sersicindex'):
-            return derivs
-        if modelMask is None:
-            from .patch import ModelMask
-            x0, x1, y0, y1 = patch0.getExtent()
-            modelMask = ModelMask.fromExtent(x0, x1, y0, y1)
-        px0, py0 = img.getWcs().positionToPixel(self.getPosition(), self)
-        counts = img.getPhotoCal().brightnessToCounts(self.brightness)
-        step = self.sersicindex.getStepSizes()[0]
-        n0 = self.sersicindex.getValue()
-        if n0 + step > self.sersicindex.upper:
-            step = -step
-        self.sersicindex.setValue(n0 + step)
-        patch = self.getUnitFluxModelPatch(img, px=px0, py=py0, modelMask=modelMask, **kwargs)
-        self.sersicindex.setValue(n0)
-        if patch is None or counts == 0:
-            derivs.append(None)
-            return derivs
-        d = (patch - patch0) * (counts / step)
-        d.setName('d(%s)/d(sersicindex)' % self.dname)
-        derivs.append(d)
-        return derivs
+    def _extra_columns(self, plan, img, px, py, modelMask, counts, kwargs):
+        """The Sersic-index column (sersic.py:402-442): a forward difference
+        of the unit render on the caller's modelMask, the step reversed when
+        it would pass the upper bound."""
+        if self.isParamFrozen('sersicindex'):
+            return []
+        si = self.sersicindex
+        names, vals, ups = si.getParamNames(), si.getParams(), si.getUpperBounds()
+        cols = []
+        for i, step in enumerate(si.getStepSizes()):
+            if vals[i] + step > ups[i]:
+                step = -step
+            r = self._stepped(plan, img, si, i, step, px=px, py=py, modelMask=modelMask, **kwargs)
+            cols.append(('diff', r, counts / step, 'd(%s)/d(%s)' % (self.dname, names[i])))
+        return cols
 
 
 def _pieces(nodes, vals, k):
