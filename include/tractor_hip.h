@@ -581,6 +581,59 @@ int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations, doubl
 #define TRH_FIT_F_MODEL_SIZE 16
 int trh_fitter_flags(trh_fitter *f, int32_t *flags, void *stream);
 
+/*
+ * The fitter's stages as standalone batched calls (device arrays unless
+ * noted), for callers that drive their own LM loop.
+ *
+ * trh_jacobian_normal_eq_batch: the normal equations of the single-source
+ *   least-squares step of GpuOptimizer.gpu_one_source_update
+ *   (tractor/gpu_optimizer.py:670-792: columns from the finite-difference
+ *   renders, weighted by inverr, and b = (data - counts R0) inverr), for n
+ *   sources.  Source s: kind[s] as in trh_fitter_create, its P x P window
+ *   of data / inverr at pix_off[s] with row stride row_stride[s], and its
+ *   render planes at renders + render_off[s], P*P floats each: plane 0 the
+ *   unit-flux render, planes 1..3 the renders with the three shape
+ *   parameters each moved by shape_step (galaxies), plane 4 the render with
+ *   the Sersic index moved by index_step[s] (kind 3).  counts[s] = flux.
+ *   Columns: x, y (central differences of plane 0 through cdi, a host
+ *   row-major 2x2, NULL = identity, times counts), flux (plane 0), the
+ *   shape and index forward differences times counts / step.  neq[s][40]:
+ *   the packed upper triangle of the NC x NC matrix J^T W J (NC = 6, or 7
+ *   for kind 3; point sources fill the x, y, flux entries), then the NC
+ *   entries of J^T W r, chi^2 at [36] and the chi^2 of a zero model at [37]
+ *   (point sources); FP32 products, FP64 sums.  With index_step NULL the
+ *   kind-3 sources are skipped.
+ * trh_solve_damped_cholesky_f64_batch: the column-scaled, damped FP64
+ *   Cholesky solve of each source's system in neq (as written above;
+ *   lambda is added to the scaled diagonal, raised tenfold from 1e-12 when
+ *   the system is not positive definite; all-zero columns give X = 0,
+ *   gpu_optimizer.py:752-792): X[s][8] (x, y, flux, shape..., index).
+ *   With trial given (and params [s][8], nsteps, steplim), also the
+ *   line-search trials of ConstrainedOptimizer.getParameterSteps
+ *   (constrained_optimizer.py:84-159): trial[s][13][8], their count and
+ *   whether a maximum step limited them.
+ * trh_line_search_chi2_batch: chi2[s][a] = the chi^2 of source s at trial
+ *   parameters trial[s][a][8] over its window, for a < nsteps[s] (13 slots
+ *   per source), rendered with the fitter's sources, PSF and data
+ *   (gpu_one_source_try_updates, gpu_optimizer.py:341-425, as chi^2 rather
+ *   than its change); a trial that needs an FFT size outside 32..512 gets
+ *   +inf and TRH_FIT_F_LS_SIZE.  Uses the fitter's job buffers: not
+ *   concurrent with its iterations.
+ */
+int trh_jacobian_normal_eq_batch(trh_handle *h, int n, const int32_t *kind, const int32_t *P,
+                                 const int32_t *row_stride, const int64_t *pix_off,
+                                 const float *data, const float *inverr,
+                                 const int64_t *render_off, const float *renders,
+                                 const double *counts, const double *index_step,
+                                 const double *cdi, double shape_step, double *neq,
+                                 void *stream);
+int trh_solve_damped_cholesky_f64_batch(trh_handle *h, int n, const int32_t *kind,
+                                        const double *neq, double lambda, const double *params,
+                                        double *X, double *trial, int32_t *nsteps,
+                                        int32_t *steplim, void *stream);
+int trh_line_search_chi2_batch(trh_fitter *f, const double *trial, const int32_t *nsteps,
+                               double *chi2, void *stream);
+
 /* Device pointer to the fitter's Jacobian renders (float32, per source
  * 1 + n_shape (+1 Sersic index) planes of P*P at render_off[s]) and normal
  * equations (double [S][40]: the packed upper triangle of the NC x NC matrix,

@@ -104,6 +104,10 @@ SIGNATURES = {
     'trh_fitter_render_models': (_I, [_P, _P, _I, _P]),
     'trh_fitter_status': (_I, [_P, _P, _P, _P, _P, _P, _P]),
     'trh_fitter_flags': (_I, [_P, _P, _P]),
+    'trh_jacobian_normal_eq_batch': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                          _D, _P, _P]),
+    'trh_solve_damped_cholesky_f64_batch': (_I, [_P, _I, _P, _P, _D, _P, _P, _P, _P, _P, _P]),
+    'trh_line_search_chi2_batch': (_I, [_P, _P, _P, _P, _P]),
     'trh_fitter_debug': (_I, [_P, _PP, _PP, _PP, _PP]),
     'trh_fitter_timing': (_I, [_P, _P]),
     'trh_fitter_profile': (_I, [_P, _I]),

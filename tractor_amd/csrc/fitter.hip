@@ -686,8 +686,8 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
                 const int32_t *__restrict__ stride,
                 const int64_t *__restrict__ pix_off, const int64_t *__restrict__ roff,
                 const float *__restrict__ data, const float *__restrict__ inverr,
-                const float *__restrict__ renders, const double *__restrict__ params,
-                const double *__restrict__ sstep,
+                const float *__restrict__ renders, const double *__restrict__ counts,
+                int cstride, double shape_step, const double *__restrict__ sstep,
                 const int32_t *__restrict__ active, double cdi00, double cdi01, double cdi10,
                 double cdi11, double *__restrict__ neq, const int32_t *__restrict__ order) {
     constexpr int NQ = NeqPix<NC>::NQ;
@@ -695,17 +695,18 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
     __shared__ double red[NEQ_TB / 64][NEQ_W];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = blockIdx.x; i < S; i += gridDim.x) {
-        const int s = order[i];
-        if (!active[s] || ((kind[s] == 3) != (NC == 7))) continue;
+        const int s = order ? order[i] : i;
+        if ((active && !active[s]) || ((kind[s] == 3) != (NC == 7))) continue;
         NeqPix<NC> px;
         px.kd = kind[s];
         px.P = Parr[s];
         px.PP = px.P * px.P;
         px.ds = stride[s];
         px.invP = 1.0f / (float)px.P;
-        px.counts = (float)params[8 * s + 2];
-        px.cstep = (float)(params[8 * s + 2] / 0.01);
-        px.istep = NC == 7 ? (float)(params[8 * s + 2] / sstep[s]) : 0.f;
+        const double cs = counts[(size_t)cstride * s];
+        px.counts = (float)cs;
+        px.cstep = (float)(cs / shape_step);
+        px.istep = NC == 7 ? (float)(cs / sstep[s]) : 0.f;
         px.c00 = (float)cdi00;
         px.c01 = (float)cdi01;
         px.c10 = (float)cdi10;
@@ -799,8 +800,8 @@ __device__ __forceinline__ void solve_source(int s, const double *__restrict__ q
     for (int i = 0; i < NP; ++i) x[i] = 0.0;
     if (!any) {
         // all derivatives zero: no update (gpu_optimizer.py:762-767)
-        nsteps[s] = 0;
-        steplim[s] = 0;
+        if (nsteps) nsteps[s] = 0;
+        if (steplim) steplim[s] = 0;
         for (int i = 0; i < 8; ++i) X[8 * s + i] = 0.0;
         return;
     }
@@ -860,6 +861,7 @@ __device__ __forceinline__ void solve_source(int s, const double *__restrict__ q
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) X[8 * s + i] = i < NP ? x[i < NP ? i : 0] : 0.0;
+    if (!trial) return;   // the step alone (trh_solve_damped_cholesky_f64_batch)
 
     // getParameterSteps: maxstep PixPos [1,1] (wcs.py:384), Flux none,
     // EllipseESoft [1, 0.1, 0.1] (ellipses.py:48), SersicIndex 0.1; bounds
@@ -911,17 +913,18 @@ __global__ void __launch_bounds__(64) k_fit_solve(int S, const int32_t *__restri
                             int32_t *__restrict__ nsteps, int32_t *__restrict__ steplim) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
-    if (!active[s]) {
-        nsteps[s] = 0;
+    if (active && !active[s]) {
+        if (nsteps) nsteps[s] = 0;
         return;
     }
     const double *q = neq + (size_t)NEQ_W * s;
+    const double *p0 = params ? params + 8 * s : nullptr;
     if (kind[s] == 3)
-        solve_source<7, 7>(s, q, params + 8 * s, lambda, X, trial, nsteps, steplim);
+        solve_source<7, 7>(s, q, p0, lambda, X, trial, nsteps, steplim);
     else if (kind[s])
-        solve_source<6, 6>(s, q, params + 8 * s, lambda, X, trial, nsteps, steplim);
+        solve_source<6, 6>(s, q, p0, lambda, X, trial, nsteps, steplim);
     else
-        solve_source<3, 6>(s, q, params + 8 * s, lambda, X, trial, nsteps, steplim);
+        solve_source<3, 6>(s, q, p0, lambda, X, trial, nsteps, steplim);
 }
 
 // One thread per (source, trial step) of the line search.
@@ -942,7 +945,7 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slot / NALPHA, a = slot - s * NALPHA;
     int b = -1;
-    if (s < S && active[s] && a < nsteps[s]) {
+    if (s < S && (!active || active[s]) && a < nsteps[s]) {
         const int kd = kind[s];
         const int P = Parr[s];
         const double *p = trial + (size_t)slot * 8;
@@ -1351,7 +1354,7 @@ extern "C" int trh_fitter_set_sersic(trh_fitter *f, const double *tab, int ntab)
 
 static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *list,
                         const int32_t *count, int bound, float *out, double *chi2,
-                        hipStream_t st, int mode = TRH_STORE) {
+                        hipStream_t st, int mode = TRH_STORE, bool window_chi0 = true) {
     if (bound <= 0) return TRH_OK;
     const size_t lds = point_lds_bytes(f->ph, f->pw);
     int wg = (int)((160 * 1024) / (lds + sizeof(RenderScratch)));
@@ -1364,7 +1367,7 @@ static int launch_point(trh_fitter *f, const trh_fft_job *jobs, const int32_t *l
         // the same iteration (neq[32 s + 28], indexed by the job's pad0)
         hipLaunchKernelGGL(k_render_point<TRH_CHI2>, dim3(grid), dim3(256), lds, st, jobs, list,
                            count, bound, f->stamp, f->ph, f->pw, out, f->data, f->inverr, chi2,
-                           f->neq);
+                           window_chi0 ? f->neq : nullptr);
     else if (mode == TRH_STORE)
         hipLaunchKernelGGL(k_render_point<TRH_STORE>, dim3(grid), dim3(256), lds, st, jobs, list,
                            count, bound, f->stamp, f->ph, f->pw, out, f->data, f->inverr, chi2,
@@ -1428,17 +1431,17 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
         const int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
         if (S > f->n_ser) {
             hipLaunchKernelGGL(k_fit_normal_eq<6>, dim3(grid), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
-                               f->stride, f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
-                               f->sstep, f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3],
-                               f->neq, f->neq_order);
+                               f->stride, f->pix_off, f->render_off, f->data, f->inverr,
+                               f->renders, f->params + 2, 8, 0.01, f->sstep, f->active,
+                               f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq, f->neq_order);
             if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq<6>"))) return rc;
         }
         if (f->n_ser) {
             const int g7 = f->n_ser < f->h->n_cu * 8 ? f->n_ser : f->h->n_cu * 8;
             hipLaunchKernelGGL(k_fit_normal_eq<7>, dim3(g7), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
-                               f->stride, f->pix_off, f->render_off, f->data, f->inverr, f->renders, f->params,
-                               f->sstep, f->active, f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3],
-                               f->neq, f->neq_order);
+                               f->stride, f->pix_off, f->render_off, f->data, f->inverr,
+                               f->renders, f->params + 2, 8, 0.01, f->sstep, f->active,
+                               f->cdi[0], f->cdi[1], f->cdi[2], f->cdi[3], f->neq, f->neq_order);
             if ((rc = hip_check(hipGetLastError(), "k_fit_normal_eq<7>"))) return rc;
         }
     }
@@ -1670,4 +1673,78 @@ extern "C" int trh_fitter_timing(trh_fitter *f, float *ms6) {
     for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&ms6[i], f->ev[i], f->ev[i + 1]);
     (void)hipEventElapsedTime(&ms6[5], f->ev[0], f->ev[5]);
     return TRH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The fitter's stages as standalone batched entry points
+// ---------------------------------------------------------------------------
+extern "C" int trh_jacobian_normal_eq_batch(trh_handle *h, int n, const int32_t *kind,
+                                            const int32_t *P, const int32_t *row_stride,
+                                            const int64_t *pix_off, const float *data,
+                                            const float *inverr, const int64_t *render_off,
+                                            const float *renders, const double *counts,
+                                            const double *index_step, const double *cdi,
+                                            double shape_step, double *neq, void *stream) {
+    if (!h || n <= 0 || !kind || !P || !row_stride || !pix_off || !data || !inverr ||
+        !render_off || !renders || !counts || !neq || !(shape_step != 0.0))
+        return set_err(TRH_E_ARG, "trh_jacobian_normal_eq_batch: bad arguments");
+    const double I2[4] = {1.0, 0.0, 0.0, 1.0};
+    const double *c = cdi ? cdi : I2;
+    hipStream_t st = (hipStream_t)stream;
+    const int grid = n < h->n_cu * 8 ? n : h->n_cu * 8;
+    hipLaunchKernelGGL(k_fit_normal_eq<6>, dim3(grid), dim3(NEQ_TB), 0, st, n, kind, P, row_stride,
+                       pix_off, render_off, data, inverr, renders, counts, 1, shape_step,
+                       index_step, nullptr, c[0], c[1], c[2], c[3], neq, nullptr);
+    int rc = hip_check(hipGetLastError(), "k_fit_normal_eq<6>");
+    if (rc || !index_step) return rc;
+    hipLaunchKernelGGL(k_fit_normal_eq<7>, dim3(grid), dim3(NEQ_TB), 0, st, n, kind, P, row_stride,
+                       pix_off, render_off, data, inverr, renders, counts, 1, shape_step,
+                       index_step, nullptr, c[0], c[1], c[2], c[3], neq, nullptr);
+    return hip_check(hipGetLastError(), "k_fit_normal_eq<7>");
+}
+
+extern "C" int trh_solve_damped_cholesky_f64_batch(trh_handle *h, int n, const int32_t *kind,
+                                                   const double *neq, double lambda,
+                                                   const double *params, double *X,
+                                                   double *trial, int32_t *nsteps,
+                                                   int32_t *steplim, void *stream) {
+    if (!h || n <= 0 || !kind || !neq || !X || !(lambda >= 0.0) ||
+        (trial && (!params || !nsteps || !steplim)))
+        return set_err(TRH_E_ARG, "trh_solve_damped_cholesky_f64_batch: bad arguments");
+    hipLaunchKernelGGL(k_fit_solve, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, n, kind,
+                       neq, params, nullptr, lambda, X, trial, trial ? nsteps : nullptr,
+                       trial ? steplim : nullptr);
+    return hip_check(hipGetLastError(), "k_fit_solve");
+}
+
+extern "C" int trh_line_search_chi2_batch(trh_fitter *f, const double *trial,
+                                          const int32_t *nsteps, double *chi2, void *stream) {
+    if (!f || !trial || !nsteps || !chi2)
+        return set_err(TRH_E_ARG, "trh_line_search_chi2_batch: bad arguments");
+    if (f->n_ser && !f->ser)
+        return set_err(TRH_E_ARG, "trh_line_search_chi2_batch: Sersic galaxies need "
+                                  "trh_fitter_set_sersic");
+    hipStream_t st = (hipStream_t)stream;
+    const int S = f->S, cap_ls = NALPHA * S;
+    const int nb_ls = (cap_ls + PREP_TB - 1) / PREP_TB;
+    int rc;
+    hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
+    hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb_ls), dim3(PREP_TB), 0, st, S, f->kind, f->P,
+                       f->stride, f->pix_off, trial, nsteps, nullptr, f->prof, f->ser, f->ph,
+                       f->pw, f->jobs_ls, f->bk_ls, f->bcnt_ls, f->flags, chi2, nullptr);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
+    hipLaunchKernelGGL(k_fit_lists, dim3(nb_ls), dim3(PREP_TB), 0, st, cap_ls, f->bk_ls,
+                       f->bcnt_ls, f->list_ls, cap_ls, f->cnt + LS_CNT);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
+    for (int b = 0; b < NFFT && f->n_gal; ++b) {
+        if (kSizes[b] < f->min_sz) continue;
+        if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
+                                 f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b,
+                                 NALPHA * f->n_gal, f->maxrows[b], f->spec[b], nullptr, f->data,
+                                 f->inverr, chi2, st, fitter_hbuf(f, kSizes[b]))))
+            return rc;
+    }
+    // point sources over their whole windows (no normal equations assumed)
+    return launch_point(f, f->jobs_ls, f->list_ls + (size_t)PTB * cap_ls, f->cnt + LS_CNT + PTB,
+                        NALPHA * f->n_pt, nullptr, chi2, st, TRH_STORE, false);
 }
