@@ -37,7 +37,7 @@ __host__ __device__ inline size_t fft_hbuf_slot_floats(int sz) {
 __host__ __device__ inline size_t fft_lds_bytes(int sz, int nwaves, int maxrows) {
     const int nseq = sz >= 64 ? 1 : 64 / sz;
     const int lbuf = sz + sz / 32;
-    const size_t ltab = sz >= 256 ? 2 * (size_t)sz : 0;   // Lanczos responses (global rows)
+    const size_t ltab = sz >= 128 ? 2 * (size_t)sz : 0;   // Lanczos responses
     return sizeof(float2) * ((size_t)nwaves * nseq * 2 * lbuf + (size_t)maxrows * (sz / 2 + 1) +
                              ltab);
 }
@@ -84,6 +84,15 @@ __device__ __forceinline__ void compute_taps(double mux, double muy, RenderScrat
 // sz = 128 chi^2 renders: 1.73 -> 1.42 GB read per LM iteration of 4096
 // sources, against 1.14 GB of pixels).
 __device__ __forceinline__ float ld_stream(const float *p) { return __builtin_nontemporal_load(p); }
+
+// Whether the buffers a job's epilogue mode writes and reads were passed:
+// out for TRH_STORE / TRH_ADD, data, inverr and chi2 slots for TRH_CHI2;
+// unknown modes have none.
+__device__ __forceinline__ bool epilogue_buffers_ok(int mode, const float *out, const float *data,
+                                                    const float *inverr, const double *chi2) {
+    if (mode == TRH_CHI2) return data && inverr && chi2;
+    return (mode == TRH_STORE || mode == TRH_ADD) && out;
+}
 
 // Epilogue of one output pixel.  Returns the chi^2 contribution (TRH_CHI2).
 __device__ __forceinline__ double epilogue(int mode, float val, float scale, int64_t idx,
@@ -342,7 +351,10 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     static_assert(!G::GH || (NSEQ == 1 && CW == 2), "global rows store column pairs");
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    // the wave index is the same on every lane: scalar, so everything
+    // derived from it (column / row-pair numbers, their addresses) is too
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6),
+              nw = nthr >> 6;
     const int sl = lane & (SEQL - 1), sq = lane / SEQL;
     float2 *lbw = lbuf + (size_t)wave * PER * G::LBUF;
     const int64_t off = Jp->out_offset;
@@ -350,6 +362,9 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy;
     // MODE >= 0: epilogue fixed at compile time; -1: read from the job
     const int K = min(max(Jp->K, 0), TRH_MAXK), mode = MODE >= 0 ? MODE : Jp->mode;
+    // a job whose epilogue has no buffer to use renders nothing (the same
+    // decision in every thread of the block, so no barrier is left waiting)
+    if (!epilogue_buffers_ok(mode, out, data, inverr, chi2)) return;
     const float scale = Jp->scale;
     const float2 *__restrict__ P = psf_spec + (size_t)Jp->psf * SZ * NV;
 
@@ -366,9 +381,11 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     // FD: the Lanczos shift as a spectrum filter (the window keeps every tap
     // inside the box): y response at the lane's row frequencies (phase A),
     // x response at its column frequencies (phase B), both at sl + SEQL r.
-    // Held in registers up to SZ = 128; from 256 on (R >= 4) in the LDS table
-    // ltab (y at [0, SZ), x at [SZ, 2 SZ)), filled by the block per job.
-    constexpr bool LREG = FD && !G::GH;
+    // Held in registers up to SZ = 64; from 128 on in the LDS table ltab (y at
+    // [0, SZ), x at [SZ, 2 SZ)), filled by the block per job (registers there
+    // cost more in spills than the table's reads: 69 -> 57 ns per sz = 128
+    // chi^2 render).
+    constexpr bool LREG = FD && SZ <= 64;
     float2 Ly[LREG ? R : 1], Lx[LREG ? R : 1];
     if constexpr (LREG) {
 #pragma unroll
@@ -671,6 +688,7 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     const int stride = Jp->out_stride;
     const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy;
     const int mode = MODE >= 0 ? MODE : Jp->mode;
+    if (!epilogue_buffers_ok(mode, out, data, inverr, chi2)) return;   // see render_fft_one
     const float scale = Jp->scale;
     compute_taps(Jp->mux, Jp->muy, S);
     const float *fx = S.taps;
@@ -822,6 +840,7 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
         maxrows = SZ;
     } else {
         hbuf = lbuf + (size_t)(blockDim.x >> 6) * G::NSEQ * CW * G::LBUF;
+        if constexpr (SZ >= 128) ltab = hbuf + (size_t)maxrows * G::HS;
     }
     WaveTw<SZ> T;
     T.load(twg, (threadIdx.x & 63) & (G::SEQL - 1));
