@@ -174,17 +174,18 @@ __device__ __forceinline__ void lane_transpose(float &a, float &b) {
 // stages (span d >= 64) need tw[(sl + 64 q) * SZ/(2d)], q < d/64; the stages
 // of span d = 32..4, done in registers after a lane transpose, need
 // tw[(sl & (d-1)) * SZ/(2d)] (the lane bits below d are still the element
-// bits below d).  The stages of span 2 and 1 stay across lanes: t = x + s *
-// x[lane ^ d] with s = +1 on the lower lane of each pair and -1 on the upper,
-// then t * w' with w' = 1 on the lower lane and -tw[(sl & (d-1)) * SZ/(2d)] on
-// the upper, so the upper lane gets (x_hi - x_lo)(-w) = (x_lo - x_hi) w with
-// no divergent branch.
+// bits below d).  The stages of span 2 and 1 stay across lanes: t = s x +
+// x[lane ^ d] with s = +1 on the lower lane of each pair and -1 on the upper
+// gives x_lo + x_hi on the lower lane and x_lo - x_hi on the upper, whose
+// twiddle tw[(sl & (d-1)) * SZ/(2d)] is 1 at span 1 and 1 or i at span 2 (i on
+// the odd pairs, `rot`): neither stage multiplies, and no branch diverges.
 template <int SZ> struct WaveTw {
     static constexpr int NREG = SZ >= 128 ? SZ / 64 - 1 : 0;
-    static constexpr int NLANE = SZ >= 64 ? 6 : Log2<SZ>::v;
+    static constexpr int NLANE = (SZ >= 64 ? 6 : Log2<SZ>::v) - 2;
     float2 reg[NREG > 0 ? NREG : 1];
-    float2 lane[NLANE];   // spans 32 (or 16) .. 1
+    float2 lane[NLANE];   // spans 32 (or 16) .. 4
     float sgn[2];         // spans 2, 1
+    bool rot;             // span 2: upper lane of an odd pair (times i)
     __device__ __forceinline__ void load(const float2 *tw, int sl) {
         int n = 0;
 #pragma unroll
@@ -193,16 +194,11 @@ template <int SZ> struct WaveTw {
             for (int q = 0; q < d / 64; ++q) reg[n++] = tw[(sl + 64 * q) * (SZ / (2 * d))];
         n = 0;
 #pragma unroll
-        for (int d = (SZ >= 64 ? 32 : SZ / 2); d >= 1; d >>= 1) {
-            const float2 w = tw[(sl & (d - 1)) * (SZ / (2 * d))];
-            if (d >= 4) {
-                lane[n++] = w;
-            } else {
-                const bool up = (sl & d) != 0;
-                lane[n++] = up ? make_float2(-w.x, -w.y) : make_float2(1.f, 0.f);
-                sgn[d == 2 ? 0 : 1] = up ? -1.f : 1.f;
-            }
-        }
+        for (int d = (SZ >= 64 ? 32 : SZ / 2); d >= 4; d >>= 1)
+            lane[n++] = tw[(sl & (d - 1)) * (SZ / (2 * d))];
+        sgn[0] = (sl & 2) ? -1.f : 1.f;
+        sgn[1] = (sl & 1) ? -1.f : 1.f;
+        rot = (sl & 3) == 3;
     }
 };
 
@@ -236,17 +232,21 @@ __device__ __forceinline__ void idif_transposed_stage(float2 (&x)[C][WaveGeom<SZ
 }
 
 // One cross-lane DIF stage of span D = 2 or 1 on every register: lower lane
-// x_lo + x_hi, upper lane (x_lo - x_hi) w (see WaveTw).
+// x_lo + x_hi, upper lane (x_lo - x_hi) w with w = i on the `rot` lanes of
+// span 2 and 1 elsewhere (see WaveTw).
 template <int SZ, int D, int C>
-__device__ __forceinline__ void idif_lane_stage(float2 (&x)[C][WaveGeom<SZ>::R], float2 w,
-                                                float s) {
+__device__ __forceinline__ void idif_lane_stage(float2 (&x)[C][WaveGeom<SZ>::R], float s,
+                                                bool rot) {
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int r = 0; r < WaveGeom<SZ>::R; ++r) {
             const float px = xlane<D>(x[c][r].x), py = xlane<D>(x[c][r].y);
-            const float tx = fmaf(s, px, x[c][r].x), ty = fmaf(s, py, x[c][r].y);
-            x[c][r] = make_float2(tx * w.x - ty * w.y, tx * w.y + ty * w.x);
+            const float tx = fmaf(s, x[c][r].x, px), ty = fmaf(s, x[c][r].y, py);
+            if constexpr (D == 2)
+                x[c][r] = rot ? make_float2(-ty, tx) : make_float2(tx, ty);
+            else
+                x[c][r] = make_float2(tx, ty);
         }
 }
 
@@ -283,8 +283,8 @@ __device__ __forceinline__ void wave_idif(float2 (&x)[C][WaveGeom<SZ>::R], const
     idif_transposed_stage<SZ, 4, C>(x, T.lane[TOP - 4]);
     idif_transposed_stage<SZ, 3, C>(x, T.lane[TOP - 3]);
     idif_transposed_stage<SZ, 2, C>(x, T.lane[TOP - 2]);
-    idif_lane_stage<SZ, 2, C>(x, T.lane[TOP - 1], T.sgn[0]);
-    idif_lane_stage<SZ, 1, C>(x, T.lane[TOP], T.sgn[1]);
+    idif_lane_stage<SZ, 2, C>(x, T.sgn[0], T.rot);
+    idif_lane_stage<SZ, 1, C>(x, T.sgn[1], false);
 }
 
 // Where wave_idif leaves register (c, r) of lane l: the sequence slot
