@@ -67,6 +67,16 @@ def build_workload(name, torch):
                 inverr=inverr, pix_off=pix_off, start=start)
 
 
+def tune_launch_shapes(wl):
+    """Autotune the render launch shapes (waves per block, G rows per pass)
+    of every FFT bucket this workload's patch sizes use, before the fitter
+    records its graph; returns {sz: [threads, rows, ns per render]}."""
+    from tractor_amd import autotune
+    got = autotune.tune_fft(wl['psf'], np.unique(wl['P']))
+    log('launch shapes', got)
+    return {str(k): [int(v[0]), int(v[1]), round(float(v[2]), 2)] for k, v in got.items()}
+
+
 def run_step(wl, torch, dchisq=0., graph=True):
     f = wl['fitter']
     if wl['cfg'] == 4:
@@ -169,6 +179,11 @@ def main():
                     help='skip the extra per-launch-timed step after the timed region (its '
                          'event timing and byte counters slow that step down; use this under '
                          'rocprofv3 so the trace holds the timed fit only); roofline is then null')
+    ap.add_argument('--no-surrogate', action='store_true',
+                    help='skip the surrogate-initialised fit (CNN) reported beside the config-4 '
+                         'line')
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='keep the default render launch shapes')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--dchisq', type=float, default=0.,
                     help='optimize_loop stopping tolerance on dlnp (reference default 0)')
@@ -204,6 +219,7 @@ def main():
         return
     log('building workload', args.workload)
     wl = build_workload(args.workload, torch)
+    shapes = None if args.no_autotune else tune_launch_shapes(wl)
     f = wl['fitter']
     for _ in range(args.warmup):
         run_step(wl, torch, args.dchisq)
@@ -280,12 +296,17 @@ def main():
             'sources': wl['n'], 'lm_iterations_per_step': float(np.mean(iters)),
             'dchisq': args.dchisq,
             'parallelism': 'single GPU',
+            'launch_shapes': shapes,
         },
-        'roofline': roofline_of(prof) if prof is not None else None,
+        'roofline': roofline_of(prof, args.workload) if prof is not None else None,
         'cpu_baseline': cpu,
     }
     if parity is not None:
         out['parity'] = parity
+    if wl['cfg'] == 4 and not args.no_surrogate:
+        log('surrogate leg')
+        plain = f.status()
+        out['surrogate'] = surrogate_leg(wl, torch, args, dt, plain)['summary']
     if wl['cfg'] == 4 and not args.no_sersic:
         log('sersic leg')
         out['sersic_fit'] = sersic_leg(torch)
@@ -305,45 +326,76 @@ def render_flops(sz, K=MEAN_K):
     return K * sz * nv * 9 + 6 * sz * nv + 2.5 * sz * sz * math.log2(sz * sz) + 28 * sz * sz
 
 
-def roofline_of(prof):
+# kernel (as rocprofv3 names it, template arguments included) of each slot
+SLOT_KERNEL = {0: 'k_render_fft<32, 0>', 1: 'k_render_fft<64, 0>', 2: 'k_render_fft<128, 0>',
+               3: 'k_render_fft<256, 0>', 4: 'k_render_fft<512, 0>', 5: 'k_render_point<0>',
+               6: 'k_fit_normal_eq<6>', 7: 'k_fit_solve', 8: 'k_render_fft<32, 2>',
+               9: 'k_render_fft<64, 2>', 10: 'k_render_fft<128, 2>', 11: 'k_render_fft<256, 2>',
+               12: 'k_render_fft<512, 2>', 13: 'k_render_point<2>'}
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'render_fit_config4_hbm_traffic.json')
+
+
+def measured_traffic(slot, workload):
+    """HBM bytes per launch of a slot's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the same workload (copy-calibrated),
+    or None when there is none for this workload."""
+    if slot not in SLOT_KERNEL or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        tr = json.load(fh)
+    if tr.get('workload_key') != workload:
+        return None
+    for name, v in tr['kernels'].items():
+        if SLOT_KERNEL[slot] + '(' in name.replace('void ', '').replace('trh::', ''):
+            return dict(bytes=round(v['read_bytes_per_dispatch'] + v['write_bytes_per_dispatch']),
+                        source='rocprofv3 PMC FETCH_SIZE + WRITE_SIZE of %s, mean over %d '
+                               'dispatches (%s)' % (name.split('(')[0].replace('void ', ''),
+                                                    v['dispatches'], tr['workload']))
+    return None
+
+
+def roofline_of(prof, workload):
     """Dominant kernel slot (largest total time over the profiled step) ->
-    roofline dict: algorithmic HBM bytes per launch / mean launch time, and
-    for an FFT render slot the VALU arithmetic rate as well."""
+    roofline dict.  Both ceilings are evaluated from the live per-launch HIP
+    event timing: HBM (algorithmic bytes per launch / mean launch time, vs
+    8 TB/s) and, for an FFT render slot, VALU FP32 (algorithmic flops of
+    render_flops, vs the published 157.3 TFLOP/s).  `bound` is the one whose
+    fraction of peak is higher, and the top-level figures are that one's."""
     k = int(np.argmax(prof[:, 0]))
     ms, launches, jobs, nbytes = prof[k]
     avg_s = ms / 1e3 / max(launches, 1)
-    achieved = nbytes / max(launches, 1) / avg_s / 1e9
-    out = {'bound': 'hbm', 'kernel': SLOT_NAMES[k], 'achieved': round(achieved, 2),
-           'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
-           'traffic': None, 'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
-           'bytes_per_launch': round(nbytes / max(launches, 1)), 'jobs_per_launch': round(
-               jobs / max(launches, 1), 1)}
-    tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_render_fft128_chi2.json')
-    if os.path.exists(tfile):
-        # measured HBM bytes (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated)
-        # per render of this kernel, scaled to this launch's mean job count
-        with open(tfile) as fh:
-            tr = json.load(fh)
-        if tr['kernel'] == SLOT_NAMES[k]:
-            per_job = (tr['read_bytes'] + tr['write_bytes']) / tr['jobs']
-            out['traffic'] = round(per_job * jobs / max(launches, 1))
-            out['traffic_source'] = 'rocprofv3 PMC, %s, %.0f B per render' % (tr['workload'], per_job)
+    gbs = nbytes / max(launches, 1) / avg_s / 1e9
+    hbm = {'achieved': round(gbs, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+           'frac': round(gbs / HBM_PEAK_GBS, 5)}
+    valu = None
     if k in SLOT_SZ:
         tf = jobs * render_flops(SLOT_SZ[k]) / (ms / 1e3) / 1e12
-        out['valu'] = {'achieved': round(tf, 2), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                       'frac': round(tf / FP32_PEAK_TFLOPS, 4)}
+        valu = {'achieved': round(tf, 2), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                'frac': round(tf / FP32_PEAK_TFLOPS, 4),
+                'frac_of_unpacked_fp32': round(tf / (FP32_PEAK_TFLOPS / 2), 4)}
+    bound = 'valu' if valu is not None and valu['frac'] > hbm['frac'] else 'hbm'
+    top = valu if bound == 'valu' else hbm
+    tr = measured_traffic(k, workload)
+    out = {'bound': bound, 'kernel': SLOT_NAMES[k], 'achieved': top['achieved'],
+           'peak': top['peak'], 'unit': top['unit'], 'frac': top['frac'],
+           'traffic': tr['bytes'] if tr else None,
+           'traffic_source': tr['source'] if tr else None,
+           'avg_launch_ms': round(avg_s * 1e3, 4), 'launches': int(launches),
+           'bytes_per_launch': round(nbytes / max(launches, 1)),
+           'jobs_per_launch': round(jobs / max(launches, 1), 1), 'hbm': hbm}
+    if valu is not None:
+        out['valu'] = valu
     return out
 
 
-def main_c5(args, torch):
-    """Surrogate-initialised fitting of the config-4 workload: train the CNN
-    on rendered cutouts (reported, not timed), then time the surrogate
-    pipeline (cutouts + CNN + batched fit + fallback) and the plain batched
-    fit on the same data, same stopping rule."""
+def surrogate_leg(wl, torch, args, plain_dt, plain):
+    """Surrogate-initialised fitting of a config-4 workload `wl` whose plain
+    fit took `plain_dt` s per fit and ended at `plain` (BatchFitter.status):
+    train the CNN on rendered cutouts (reported, not timed), then time the
+    surrogate pipeline (cutouts + CNN + batched fit from the proposals +
+    re-fit from the standard start of every source whose final chi^2 exceeds
+    the plain fit's by more than 1e-4 relative) over args.steps runs."""
     from tractor_amd import surrogate as S
-    log('building workload c4 for c5')
-    wl = build_workload('c4', torch)
-    f = wl['fitter']
     n = wl['n']
     t0 = time.perf_counter()
     X, T, L = S.make_training_set(args.cnn_train, chunk=16384, seed=20261203 + 5000)
@@ -357,7 +409,51 @@ def main_c5(args, torch):
         args.cnn_train, t1 - t0, args.cnn_epochs, t2 - t1, hist[0][1], hist[-1][1]))
     del X, T, L
     torch.cuda.empty_cache()
+    kw = dict(plain_kind=wl['kind'], dchisq=args.dchisq, plain_chisq=plain['chisq'])
+    args_fit = (net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'], wl['psf'], wl['start'])
+    for _ in range(args.warmup):
+        S.fit_with_fallback(*args_fit, **kw)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for _ in range(args.steps):
+        res = S.fit_with_fallback(*args_fit, **kw)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - ta) / args.steps
+    prof = {}
+    if not args.no_slot_profile:
+        S.fit_with_fallback(*args_fit, profile=prof, **kw)
+    rel = (res['chisq'] - plain['chisq']) / plain['chisq']
+    acc = float(np.mean(res['proposed_kind'] == wl['kind']))
+    log('surrogate %.0f sources/s vs plain %.0f; timings %s; iterations %.2f vs %.2f' % (
+        n / dt, n / plain_dt, res['timings'], res['iterations'].mean(), plain['iterations'].mean()))
+    summary = {
+        'sources_per_s': round(n / dt, 3),
+        'plain_sources_per_s': round(n / plain_dt, 3),
+        'speedup_vs_plain': round(plain_dt / dt, 4),
+        'fallback_rule': 'final chi2 > plain fit chi2 * (1 + 1e-4) -> re-fit from the standard start',
+        'cnn_train_s': round(t2 - t1, 3), 'cnn_render_s': round(t1 - t0, 3),
+        'cnn_train_cutouts': args.cnn_train, 'cnn_epochs': args.cnn_epochs,
+        'loss_first_last': [round(hist[0][1], 4), round(hist[-1][1], 4)],
+        'type_accuracy': round(acc, 4),
+        'mean_iterations_plain': round(float(plain['iterations'].mean()), 3),
+        'mean_iterations_surrogate': round(float(res['iterations'].mean()), 3),
+        'timings_s': {k: (round(v, 4) if isinstance(v, float) else v)
+                      for k, v in res['timings'].items()},
+        'frac_not_worse_than_plain_by_1e-4': round(float(np.mean(rel <= 1e-4)), 5),
+        'frac_within_1e-4_of_plain': round(float(np.mean(np.abs(rel) <= 1e-4)), 5),
+    }
+    return dict(summary=summary, res=res, dt=dt, prof=prof.get('slots'))
 
+
+def main_c5(args, torch):
+    """Surrogate-initialised fitting of the config-4 workload: the plain
+    batched fit is timed first (its final chi^2 is the fallback reference),
+    then the surrogate pipeline (surrogate_leg) on the same data, same
+    stopping rule."""
+    log('building workload c4 for c5')
+    wl = build_workload('c4', torch)
+    shapes = tune_launch_shapes(wl)
+    n = wl['n']
     for _ in range(args.warmup):
         run_step(wl, torch, args.dchisq)
     torch.cuda.synchronize()
@@ -366,28 +462,9 @@ def main_c5(args, torch):
         run_step(wl, torch, args.dchisq)
     torch.cuda.synchronize()
     plain_dt = (time.perf_counter() - ta) / args.steps
-    plain = f.status()
-
-    kw = dict(plain_kind=wl['kind'], dchisq=args.dchisq)
-    for _ in range(args.warmup):
-        S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'], wl['psf'],
-                            wl['start'], **kw)
-    torch.cuda.synchronize()
-    ta = time.perf_counter()
-    for i in range(args.steps):
-        res = S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'],
-                                  wl['psf'], wl['start'], **kw)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - ta) / args.steps
-    prof = {}
-    S.fit_with_fallback(net, wl['data'], wl['inverr'], wl['pix_off'], wl['P'], wl['psf'],
-                        wl['start'], profile=prof, **kw)
-    value = n / dt
-    rel = (res['chisq'] - plain['chisq']) / plain['chisq']
-    acc = float(np.mean(res['proposed_kind'] == wl['kind']))
-    log('surrogate %.0f sources/s vs plain %.0f; timings %s; iterations %.2f vs %.2f' % (
-        value, n / plain_dt, res['timings'], res['iterations'].mean(), plain['iterations'].mean()))
-
+    plain = wl['fitter'].status()
+    leg = surrogate_leg(wl, torch, args, plain_dt, plain)
+    res, dt = leg['res'], leg['dt']
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -400,7 +477,7 @@ def main_c5(args, torch):
                   'frac_not_worse_than_fp64_by_1e-4': float(np.mean(e <= 1e-4)),
                   'max_rel_chisq_vs_fp64': float(np.max(np.abs(e)))}
     out = {
-        'metric': METRIC, 'value': round(value, 3), 'unit': 'sources/s',
+        'metric': METRIC, 'value': round(n / dt, 3), 'unit': 'sources/s',
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': round(dt * 1e3, 3), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None,
@@ -412,22 +489,11 @@ def main_c5(args, torch):
                         '4x conv3x3/s2 32ch + MLP, trained on %d rendered cutouts (Moffat FWHM '
                         '3-6 px), %d epochs' % (args.cnn_train, args.cnn_epochs),
             'sources': n, 'dchisq': args.dchisq, 'parallelism': 'single GPU',
+            'launch_shapes': shapes,
         },
-        'roofline': roofline_of(prof['slots']),
+        'roofline': roofline_of(leg['prof'], 'c4') if leg['prof'] is not None else None,
         'cpu_baseline': cpu,
-        'surrogate': {
-            'plain_sources_per_s': round(n / plain_dt, 3),
-            'speedup_vs_plain': round(plain_dt / dt, 4),
-            'cnn_train_s': round(t2 - t1, 3), 'cnn_render_s': round(t1 - t0, 3),
-            'loss_first_last': [round(hist[0][1], 4), round(hist[-1][1], 4)],
-            'type_accuracy': round(acc, 4),
-            'mean_iterations_plain': round(float(plain['iterations'].mean()), 3),
-            'mean_iterations_surrogate': round(float(res['iterations'].mean()), 3),
-            'timings_s': {k: (round(v, 4) if isinstance(v, float) else v)
-                          for k, v in res['timings'].items()},
-            'frac_not_worse_than_plain_by_1e-4': round(float(np.mean(rel <= 1e-4)), 5),
-            'frac_within_1e-4_of_plain': round(float(np.mean(np.abs(rel) <= 1e-4)), 5),
-        },
+        'surrogate': leg['summary'],
     }
     if parity is not None:
         out['parity'] = parity

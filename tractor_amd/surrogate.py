@@ -11,7 +11,9 @@ with Moffat PSFs of FWHM 3-6 px and unit-variance noise.
 
 The fit that follows is the plain batched LM fit; `fit_with_fallback`
 re-fits from the standard start every source whose surrogate-started fit
-fails a chi-square self-check, and keeps the better of the two.
+ends worse than the plain fit of the same source (final chi^2 above the
+plain fit's by more than 1e-4 relative), or, without the plain fit's chi^2,
+fails a chi-square self-check; the lower chi^2 is kept.
 """
 import ctypes
 import math
@@ -308,15 +310,18 @@ def selfcheck_bad(chisq, P, nparams, nsig=5.):
 
 def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kind=None,
                       max_iter=100, dchisq=0., cdi=Bt.PIXEL_CDI, type_conf=0.0, timing=None,
-                      profile=None):
+                      profile=None, plain_chisq=None, rel_tol=1e-4):
     """Surrogate-initialised batched fit.
 
     1. cutouts -> CNN -> proposed type and start;
     2. one batched LM fit from the proposals; a source whose type probability
        is below `type_conf` (default 0: never) is fit under its second most
        likely type too, in the same batch, and keeps the lower chi^2;
-    3. sources failing the chi-square self-check are re-fit with the standard
-       start (and, if given, the standard type); the lower chi^2 is kept.
+    3. fallback: with `plain_chisq` (the final chi^2 of the plain fit of the
+       same sources), every source whose chi^2 exceeds it by more than
+       `rel_tol` relative is re-fit from the standard start (and, if given,
+       the standard type); without it, the sources failing the chi-square
+       self-check are.  The lower chi^2 is kept.
     Returns dict(kind, params, chisq, iterations, fallback mask, timings)."""
     t = {}
     n = len(P)
@@ -370,8 +375,10 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
         iters[amb] = np.maximum(iters[amb], st['iterations'][n:])
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    npar = np.where(fk == 0, 3, 6)
-    bad = selfcheck_bad(chisq, P, npar)
+    if plain_chisq is not None:
+        bad = chisq > np.asarray(plain_chisq, np.float64) * (1. + rel_tol)
+    else:
+        bad = selfcheck_bad(chisq, P, np.where(fk == 0, 3, 6))
     nb = int(bad.sum())
     if nb:
         idx = np.flatnonzero(bad)
