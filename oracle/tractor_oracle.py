@@ -26,7 +26,8 @@ __all__ = [
     'column_scaled_lstsq', 'parameter_steps', 'accept_line_search', 'loop_should_stop',
     'SourceProblem', 'source_render', 'source_jacobian', 'source_update',
     'source_chisq', 'fit_source', 'ALPHAS', 'round_half_even', 'sersic_mixture',
-    'sersic_fd_step',
+    'sersic_fd_step', 'image_galaxy_unit_patch', 'image_model', 'image_chisq',
+    'fit_image_lsqr',
 ]
 
 # ---------------------------------------------------------------------------
@@ -1106,6 +1107,101 @@ def fit_source(prob, params0, dchisq=0., steps=100, dchisq_limited=1e-6, alphas=
         if loop_should_stop(step_limited, dlnp, dchisq, dchisq_limited):
             break
     return params, it + 1, source_chisq(prob, params)
+
+
+# ---------------------------------------------------------------------------
+# One galaxy in a whole image without modelMasks (the single-galaxy scene):
+# LsqrOptimizer.optimize_loop (lsqr_optimizer.py:280-326) with the
+# unconstrained line search of Optimizer.tryUpdates (optimize.py:425-470).
+# ---------------------------------------------------------------------------
+def image_galaxy_unit_patch(kind, params, psf_img, img_shape, mm=None):
+    """Unit-flux patch (x0, y0, float32 patch) of an Exp (kind 1) / Dev
+    (kind 2) galaxy with EllipseESoft shape params[3:6] at pixel (params[0],
+    params[1]), NullWCS(pixscale=1) shapes, PixelizedPSF psf_img.  Without a
+    window the extent is the reference's no-modelMask rule: halfsize = nre r_e
+    (nre 4 exp, 8 dev; galaxy.py:728-756), at most the distance to the
+    farthest image edge, at least the PSF radius hypot(H/2, W/2) (psf.py:97),
+    rounded up (galaxy.py:678-691); pixels [floor(p - halfsize),
+    ceil(p + halfsize + 1)] clipped to the image (galaxy.py:273-290).  With
+    mm = (x0, y0, w, h): rendered on that window with force_sourceOut, as the
+    shape-stepped renders of getParamDerivatives are (galaxy.py:199-205).
+    None when the patch misses the image."""
+    amp, var = _profile(kind)
+    vp = shear_mixture(var, np.eye(2) * 3600., ellipse_esoft_basis(*params[3:6]))
+    px, py = float(params[0]), float(params[1])
+    if mm is not None:
+        return render_galaxy_fft(amp, vp, px, py, psf_img, mm=mm, force_sourceOut=True)
+    ph, pw = np.shape(psf_img)
+    nre = 4. if kind == 1 else 8.
+    hs = galaxy_patch_halfsize(nre * math.exp(params[3]), px, py, img_shape,
+                               math.hypot(ph / 2., pw / 2.))
+    H, W = img_shape
+    ox, ix = get_overlapping_region(int(math.floor(px - hs)), int(math.ceil(px + hs + 1)), 0, W - 1)
+    oy, iy = get_overlapping_region(int(math.floor(py - hs)), int(math.ceil(py + hs + 1)), 0, H - 1)
+    if ix == [] or iy == []:
+        return None
+    return render_galaxy_fft(amp, vp, px, py, psf_img,
+                             nomask_extent=(ox.start, ox.stop, oy.start, oy.stop, hs))
+
+
+def image_model(kind, params, psf_img, img_shape):
+    """float32 model image: counts x unit patch added into zeros
+    (engine.py:424-445; counts = flux, NullPhotoCal)."""
+    mod = np.zeros(img_shape, np.float32)
+    r = image_galaxy_unit_patch(kind, params, psf_img, img_shape)
+    if r is not None:
+        x0, y0, p = r
+        mod[y0:y0 + p.shape[0], x0:x0 + p.shape[1]] += (p * np.float32(params[2]))
+    return mod
+
+
+def image_chisq(kind, params, psf_img, data, inverr):
+    """chi^2 of the whole image: float32 chi, FP64 sum of squares
+    (engine.py:455-479)."""
+    chi = ((data - image_model(kind, params, psf_img, data.shape)) * inverr).astype(np.float32)
+    return float((chi.astype(np.float64) ** 2).sum())
+
+
+def fit_image_lsqr(kind, params0, psf_img, data, inverr, cdi, dchisq=1e-3, steps=50,
+                   alphas=None):
+    """LsqrOptimizer.optimize_loop on one galaxy of a whole image, all six
+    parameters thawed, no priors: per step the column-scaled least-squares
+    direction of the derivative columns (galaxy.py:137-223 with cdi the
+    position-derivative matrix, ducks.py:413-423) over the patch pixels, then
+    every alpha of Optimizer.tryUpdates walked by accept_line_search; the
+    loop stops after a step that gains at most dchisq (lsqr_optimizer.py:
+    317-326).  Returns (params, last step index, chisq, [(dlnp, alpha), ...])."""
+    alphas = ALPHAS if alphas is None else np.asarray(alphas)
+    params = np.array(params0, np.float64)
+    trace = []
+    step = 0
+    for step in range(steps):
+        x0, y0, p0 = image_galaxy_unit_patch(kind, params, psf_img, data.shape)
+        h, w = p0.shape
+
+        def render(k):
+            if k is None:
+                return p0
+            q = params.copy()
+            q[3 + k] += 0.01
+            return image_galaxy_unit_patch(kind, q, psf_img, data.shape, mm=(x0, y0, w, h))[2]
+        _, derivs = galaxy_param_derivs(render, params[2], cdi, [0.01, 0.01, 0.01])
+        ie = inverr[y0:y0 + h, x0:x0 + w]
+        A = np.stack([(d * ie).astype(np.float32).ravel() for d in derivs], axis=1)
+        chi = ((data - image_model(kind, params, psf_img, data.shape)) * inverr).astype(np.float32)
+        X = column_scaled_lstsq(A, chi[y0:y0 + h, x0:x0 + w].ravel())
+        if X is None:
+            break
+        lnp0 = -0.5 * float((chi.astype(np.float64) ** 2).sum())
+        trials = [(a, params + a * X, False, False) for a in alphas]
+        lnps = [-0.5 * image_chisq(kind, p, psf_img, data, inverr) for (_, p, _, _) in trials]
+        ib, dlnp, alpha, _ = accept_line_search(lnp0, trials, lnps)
+        if ib is not None:
+            params = trials[ib][1]
+        trace.append((dlnp, alpha))
+        if dlnp <= dchisq:
+            break
+    return params, step, image_chisq(kind, params, psf_img, data, inverr), trace
 
 
 # ---------------------------------------------------------------------------

@@ -35,6 +35,33 @@ def gaussian_mixture_stamp(size, sigmas, weights):
     return (img / img.sum()).astype(np.float32)
 
 
+def config1_scene(size=64, truth_e=(0.2, -0.1)):
+    """The single-galaxy scene: an ExpGalaxy near the centre of a size x size
+    image, flux 1000, r_e 3 px, EllipseE (e1, e2) = truth_e carried as
+    EllipseESoft (log r_e, ee1, ee2); 31x31 pixel-centre stamp of
+    0.9 N(sigma 1.5) + 0.1 N(sigma 4.0), unit sum; unit-variance noise.
+    Start: position + N(0, 0.5) px, flux and r_e x U(0.8, 1.2).
+    Returns dict(psf, truth, start, noise), parameters (x, y, flux, log r_e,
+    ee1, ee2)."""
+    rng = np.random.default_rng(seed(1, 0))
+    u = rng.uniform(0., 1.)
+    re = 3.
+    e1, e2 = truth_e
+    e = math.hypot(e1, e2)
+    # EllipseESoft.fromEllipseE (ellipses.py:199-203): |ee| = -log(1 - |e|),
+    # same direction
+    ee = -math.log(1. - e) / e if e > 0 else 1.
+    truth = np.array([31.5 + u, 31.5 + u, 1000., math.log(re), e1 * ee, e2 * ee])
+    r1 = np.random.default_rng(seed(1, 1))
+    start = truth.copy()
+    start[:2] += r1.normal(0., 0.5, 2)
+    start[2] *= r1.uniform(0.8, 1.2)
+    start[3] = math.log(re * r1.uniform(0.8, 1.2))
+    noise = np.random.default_rng(seed(1, 2)).normal(0., 1., (size, size)).astype(np.float32)
+    return dict(psf=gaussian_mixture_stamp(31, (1.5, 4.0), (0.9, 0.1)), truth=truth,
+                start=start, noise=noise)
+
+
 def draw_sources(n, config=2, P_choices=None, frac=(0.3, 0.4, 0.3), rng=None):
     """Source catalogue of the batched configurations: kind, patch size P,
     truth params (x, y, flux, logre, ee1, ee2) in patch pixels.
