@@ -542,6 +542,28 @@ int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int ma
                              int n_iter, void *stream);
 
 /*
+ * Hybrid PSF (HybridPixelizedPSF, tractor/psf.py:537-625, as the GPU plugin
+ * requires, tractor/gpu_optimizer.py:827): every galaxy render splits its
+ * sheared components by the FFT box size sz (ProfileGalaxy, tractor/
+ * galaxy.py:412-455; the same ramp as gpu_optimizer.py:961-1011): component
+ * k with pixel variance trace vv_k goes to the FFT when (sz/2)^2 >
+ * inner_nsigma^2 vv_k and to pixel space when (sz/2)^2 < outer_nsigma^2
+ * vv_k; where any component is in both, both amplitudes are ramped by
+ * ns_k = (sz/2) / sqrt(vv_k).  The pixel-space part is convolved with the
+ * PSF's mixture (K components: amp[K], mean[K][2] in pixels, var[K][3] =
+ * var00, var01, var11 in pixels^2) and evaluated on the source's window at
+ * the galaxy's position (c_gauss_2d_grid, tractor/mix.i:452-524, skipping
+ * Mahalanobis distances >= 100), then added to the FFT render (galaxy.py:
+ * 510-516).  Point sources keep the pixelized stamp (the hybrid PSF's
+ * getPointSourcePatch).  K = 0 turns the split off.  K <= 4; reference
+ * defaults inner 3, outer 4.  Allocates its buffers on the first call
+ * (13 P^2 floats per galaxy for the line search's pixel-space renders);
+ * the next trh_fitter_iterate_graph captures a new graph.
+ */
+int trh_fitter_set_hybrid(trh_fitter *f, int K, const double *amp, const double *mean,
+                          const double *var, double inner_nsigma, double outer_nsigma);
+
+/*
  * Render every source's model (counts x unit-flux patch at the current
  * parameters, as getModelPatch, tractor/pointsource.py:36-60) into `out`,
  * laid out like `data` (window s at pix_off[s], row stride row_stride[s]):
@@ -573,12 +595,17 @@ int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterations, doubl
  *   TRH_FIT_F_LIMITED     the last iteration was step-limited (its stop test
  *                         used dchisq_limited = 1e-6)
  *   TRH_FIT_F_MODEL_SIZE  trh_fitter_render_models could not render it
+ *   TRH_FIT_F_HYBRID      with a hybrid PSF, a render's pixel-space part had
+ *                         more than 64 components (wide galaxy components x
+ *                         PSF components): as TRH_FIT_F_JAC_SIZE /
+ *                         TRH_FIT_F_LS_SIZE / TRH_FIT_F_MODEL_SIZE
  */
 #define TRH_FIT_F_JAC_SIZE    1
 #define TRH_FIT_F_LS_SIZE     2
 #define TRH_FIT_F_MAXITER     4
 #define TRH_FIT_F_LIMITED     8
 #define TRH_FIT_F_MODEL_SIZE 16
+#define TRH_FIT_F_HYBRID     32
 int trh_fitter_flags(trh_fitter *f, int32_t *flags, void *stream);
 
 /*

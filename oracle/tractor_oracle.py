@@ -905,8 +905,13 @@ class SourceProblem(object):
     cdi:  2x2 position-derivative matrix (ducks.py:413-423): NullWCS gives
           3600/pixscale * I (the reference's NullWCS factor), a pixel-unit
           WCS gives I.
+    hybrid_mog: (amp[K], mean[K, 2], var[K, 2, 2]) of a HybridPixelizedPSF's
+          mixture (psf.py:537-625): galaxies then split their components
+          between the FFT and pixel space (galaxy.py:412-455, 510-516); the
+          modelMask is rectangular, so the pixel part is the FP64 grid.
     """
-    def __init__(self, kind, params, P, psf_img, data=None, inverr=None, cdi=None):
+    def __init__(self, kind, params, P, psf_img, data=None, inverr=None, cdi=None,
+                 hybrid_mog=None):
         self.kind = kind
         self.params = np.array(params, np.float64)
         self.P = P
@@ -914,6 +919,7 @@ class SourceProblem(object):
         self.data = data
         self.inverr = inverr
         self.cdi = np.eye(2) if cdi is None else np.asarray(cdi, np.float64)
+        self.hybrid_mog = hybrid_mog
 
     @property
     def nparams(self):
@@ -1037,7 +1043,8 @@ def source_render(prob, params, shape_step=None, index_step=0.):
     basis = ellipse_esoft_basis(*sp)
     cdinv = np.eye(2) * 3600.  # NullWCS pixscale 1 (wcs.py:24-28)
     vp = shear_mixture(var, cdinv, basis)
-    r = render_galaxy_fft(amp, vp, x, y, prob.psf, mm=mm, force_sourceOut=True)
+    r = render_galaxy_fft(amp, vp, x, y, prob.psf, mm=mm, force_sourceOut=True,
+                          hybrid_mog=prob.hybrid_mog)
     return r[2]
 
 

@@ -1,0 +1,186 @@
+"""GPU parity of the batched fitter with a hybrid PSF (trh_fitter_set_hybrid)
+against the FP64 oracle's hybrid renders (render_galaxy_fft with
+hybrid_mog: galaxy.py:412-455, 510-516).
+
+The scene makes the split matter: galaxies with r_e 2-8 px on P = 24..56
+windows (FFT box 64), so their outer components are too wide for the box
+and go to pixel space, convolved with a two-Gaussian mixture of the PSF
+(the PixelizedPSF stamp is a Moffat profile; the mixture is what a
+HybridPixelizedPSF carries beside it).
+
+Tolerances: renders within 1e-5 of their peak (FP32 FFT and pixel-space
+parts), line-search chi^2 within 1e-5 relative, the converged fit's chi^2
+within 1e-4 relative and parameters within 1e-4 relative or 0.05 sigma (as
+the pure-pixelized fit test; the count within 0.01 sigma is printed)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tractor_amd import _lib, batch as B, workloads as W
+from tractor_amd._lib import check, ptr
+
+pytestmark = pytest.mark.gpu
+
+HM = (np.array([0.75, 0.25]), np.array([[0., 0.], [0.1, -0.05]]),
+      np.array([np.eye(2) * 1.6 ** 2, [[3.2 ** 2, 0.4], [0.4, 3.0 ** 2]]]))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope='module')
+def hyb():
+    assert torch.cuda.is_available()
+    rng = np.random.default_rng(W.seed(2, 30))
+    S = 40
+    kind = rng.choice([0, 1, 2], S, p=[0.2, 0.4, 0.4]).astype(np.int32)
+    P = rng.choice([24, 32, 40, 48, 56], S).astype(np.int32)
+    truth = np.zeros((S, 6))
+    truth[:, 0] = P / 2. + rng.uniform(0, 1, S)
+    truth[:, 1] = P / 2. + rng.uniform(0, 1, S)
+    truth[:, 2] = np.exp(rng.uniform(np.log(200.), np.log(5e3), S))
+    re = np.exp(rng.uniform(np.log(2.), np.log(8.), S))
+    e = rng.uniform(0, 0.5, S)
+    th = rng.uniform(0, np.pi, S)
+    for s in range(S):
+        if kind[s]:
+            truth[s, 3:] = O.esoft_from_e(re[s], e[s] * np.cos(2 * th[s]), e[s] * np.sin(2 * th[s]))
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    pix_off, total = W.patch_offsets(P)
+    data = torch.zeros(total, dtype=torch.float32, device='cuda')
+    inverr = torch.ones(total, dtype=torch.float32, device='cuda')
+    f = B.BatchFitter(kind, P, pix_off, data, inverr, psf)
+    f.set_hybrid(*HM)
+    f.set_params(truth)
+    f.render_models(data)
+    g = torch.Generator(device='cuda')
+    g.manual_seed(W.seed(2, 31))
+    data.add_(torch.randn(total, generator=g, device='cuda'))
+    torch.cuda.synchronize()
+    start = W.perturb(kind, truth, np.random.default_rng(W.seed(2, 32)))
+    return dict(kind=kind, P=P, truth=truth, psf=psf, f=f, data=data, inverr=inverr,
+                pix_off=pix_off, start=start, dh=data.cpu().numpy(), ih=inverr.cpu().numpy())
+
+
+def _prob(pb, s, p, hybrid=True):
+    Pz, o = int(pb['P'][s]), int(pb['pix_off'][s])
+    return O.SourceProblem(int(pb['kind'][s]), p, Pz, pb['psf'],
+                           data=pb['dh'][o:o + Pz * Pz].reshape(Pz, Pz),
+                           inverr=pb['ih'][o:o + Pz * Pz].reshape(Pz, Pz), cdi=B.PIXEL_CDI,
+                           hybrid_mog=HM if hybrid else None)
+
+
+def test_split_is_exercised_and_models_match(hyb):
+    say('hybrid models vs oracle')
+    pb = hyb
+    f = pb['f']
+    out = torch.zeros_like(pb['data'])
+    f.set_params(pb['truth'])
+    f.render_models(out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    split = 0
+    for s in range(len(pb['kind'])):
+        Pz, o = int(pb['P'][s]), int(pb['pix_off'][s])
+        t = pb['truth'][s]
+        ref = O.source_render(_prob(pb, s, t), t) * np.float32(t[2])
+        g = got[o:o + Pz * Pz].reshape(Pz, Pz)
+        e = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+        assert e <= 1e-5, (s, pb['kind'][s], Pz, e)
+        if pb['kind'][s]:
+            plain = O.source_render(_prob(pb, s, t, hybrid=False), t) * np.float32(t[2])
+            split += np.max(np.abs(plain - ref)) > 1e-4 * np.max(np.abs(ref))
+    ngal = int(np.sum(pb['kind'] > 0))
+    say('galaxies whose render the split changes: %d / %d' % (split, ngal))
+    assert split >= ngal // 3
+
+
+def test_jacobian_renders_match(hyb):
+    say('hybrid Jacobian renders vs oracle')
+    pb = hyb
+    f = pb['f']
+    f.set_params(pb['start'])
+    f.iterate(0., 0.)
+    dbg = f.debug_arrays()
+    for s in range(len(pb['kind'])):
+        Pz = int(pb['P'][s])
+        p = pb['start'][s]
+        prob = _prob(pb, s, p)
+        planes = [None] if pb['kind'][s] == 0 else [None, 0, 1, 2]
+        for k, st in enumerate(planes):
+            ref = O.source_render(prob, p, shape_step=st)
+            o = dbg['render_off'][s] + k * Pz * Pz
+            g = dbg['renders'][o:o + Pz * Pz].reshape(Pz, Pz)
+            e = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+            assert e <= 1e-5, (s, k, e)
+
+
+def test_line_search_chi2_matches(hyb):
+    say('hybrid line-search chi2 vs oracle')
+    pb = hyb
+    S = len(pb['kind'])
+    rng = np.random.default_rng(17)
+    trial = np.zeros((S, 13, 8))
+    nsteps = rng.integers(1, 14, S).astype(np.int32)
+    for s in range(S):
+        for a in range(13):
+            p = pb['start'][s].copy()
+            p[:2] += rng.uniform(-0.4, 0.4, 2)
+            p[2] *= rng.uniform(0.8, 1.2)
+            if pb['kind'][s]:
+                p[3:6] += rng.uniform(-0.05, 0.05, 3)
+            trial[s, a, :6] = p
+    d_trial = torch.from_numpy(trial).cuda()
+    d_n = torch.from_numpy(nsteps).cuda()
+    chi2 = torch.full((S, 13), float('nan'), dtype=torch.float64, device='cuda')
+    lib = _lib.load()
+    check(lib.trh_line_search_chi2_batch(pb['f'].f, ptr(d_trial), ptr(d_n), ptr(chi2), None),
+          'trh_line_search_chi2_batch')
+    torch.cuda.synchronize()
+    chi2 = chi2.cpu().numpy()
+    worst = 0.
+    for s in range(S):
+        for a in range(int(nsteps[s])):
+            p = trial[s, a, :6]
+            ref = O.source_chisq(_prob(pb, s, p), p)
+            e = abs(chi2[s, a] - ref) / ref
+            worst = max(worst, e)
+            assert e <= 1e-5, (s, a, chi2[s, a], ref)
+    say('  worst', worst)
+
+
+def test_full_fit_matches_oracle(hyb):
+    say('hybrid full fit vs oracle')
+    pb = hyb
+    f = pb['f']
+    f.set_params(pb['start'])
+    nit = f.fit(max_iter=100)
+    st = f.status()
+    got = f.get_params()
+    assert np.all(f.flags() & _lib.TRH_FIT_F_HYBRID == 0)
+    say('gpu iterations', nit)
+    nbad = tight = 0
+    S = len(pb['kind'])
+    for s in range(S):
+        prob = _prob(pb, s, pb['start'][s])
+        ref, nref, chi = O.fit_source(prob, pb['start'][s])
+        k = prob.nparams
+        _, derivs = O.source_jacobian(prob, ref)
+        A = np.stack([(d * prob.inverr).astype(np.float64).ravel() for d in derivs], 1)
+        sig = np.sqrt(np.diag(np.linalg.pinv(A.T @ A)))
+        err = np.abs(got[s, :k] - ref[:k])
+        ok = np.all((err <= 1e-4 * np.abs(ref[:k])) | (err <= 0.05 * sig))
+        tight += np.all((err <= 1e-4 * np.abs(ref[:k])) | (err <= 0.01 * sig))
+        chi_ok = abs(st['chisq'][s] - chi) <= 1e-4 * chi
+        if not (ok and chi_ok):
+            nbad += 1
+            say('source', s, 'kind', prob.kind, 'P', prob.P, 'gpu', got[s, :k], 'oracle', ref[:k],
+                'sig', sig, 'chi', st['chisq'][s], chi, 'iters', st['iterations'][s], nref)
+    say('sources within 1e-4 rel or 0.01 sigma: %d / %d' % (tight, S))
+    assert nbad == 0

@@ -16,7 +16,7 @@ TRH_STORE, TRH_ADD, TRH_CHI2 = 0, 1, 2
 TRH_MAXK = 16
 # per-source flags of the batched fitter (trh_fitter_flags)
 TRH_FIT_F_JAC_SIZE, TRH_FIT_F_LS_SIZE, TRH_FIT_F_MAXITER = 1, 2, 4
-TRH_FIT_F_LIMITED, TRH_FIT_F_MODEL_SIZE = 8, 16
+TRH_FIT_F_LIMITED, TRH_FIT_F_MODEL_SIZE, TRH_FIT_F_HYBRID = 8, 16, 32
 TRH_NOALLOC_SCRATCH_FLOATS = 8 * 1024 * 1024
 
 # struct trh_fft_job (include/tractor_hip.h)
@@ -99,6 +99,7 @@ SIGNATURES = {
     'trh_fitter_set_params': (_I, [_P, _P, _P]),
     'trh_fitter_get_params': (_I, [_P, _P, _P]),
     'trh_fitter_set_sersic': (_I, [_P, _P, _I]),
+    'trh_fitter_set_hybrid': (_I, [_P, _I, _P, _P, _P, _D, _D]),
     'trh_fitter_iterate': (_I, [_P, _D, _D, _I, _P]),
     'trh_fitter_iterate_graph': (_I, [_P, _D, _D, _I, _I, _P]),
     'trh_fitter_render_models': (_I, [_P, _P, _I, _P]),

@@ -67,6 +67,24 @@ class BatchFitter(object):
             check(self.lib.trh_fitter_set_sersic(f, ptr(tab), len(tab)), 'trh_fitter_set_sersic')
         self.stream = torch.cuda.current_stream().cuda_stream
 
+    def set_hybrid(self, amp=None, mean=None, var=None, inner_nsigma=3., outer_nsigma=4.):
+        """Hybrid PSF (HybridPixelizedPSF, tractor/psf.py:537-625): galaxy
+        components too wide for the FFT box go to pixel space, convolved with
+        the PSF mixture amp[K], mean[K, 2], var[K, 2, 2] (pixels); see
+        trh_fitter_set_hybrid.  No arguments: back to the pure pixelized PSF."""
+        if amp is None:
+            check(self.lib.trh_fitter_set_hybrid(self.f, 0, None, None, None, 3., 4.),
+                  'trh_fitter_set_hybrid')
+            return
+        amp = np.ascontiguousarray(amp, np.float64).ravel()
+        K = len(amp)
+        mean = np.ascontiguousarray(np.asarray(mean, np.float64).reshape(K, 2))
+        v = np.asarray(var, np.float64).reshape(K, 2, 2)
+        v3 = np.ascontiguousarray(np.stack([v[:, 0, 0], v[:, 0, 1], v[:, 1, 1]], 1))
+        check(self.lib.trh_fitter_set_hybrid(self.f, K, ptr(amp), ptr(mean), ptr(v3),
+                                             float(inner_nsigma), float(outer_nsigma)),
+              'trh_fitter_set_hybrid')
+
     def close(self):
         if self.f:
             self.lib.trh_fitter_destroy(self.f)

@@ -203,6 +203,36 @@ __device__ double sersic_fd_step(const SersicTab &T, double n0) {
     return step;
 }
 
+// ---- Hybrid PSF (HybridPixelizedPSF): galaxy components split between the
+// FFT render and a pixel-space mixture convolved with the PSF's own mixture
+// (galaxy.py:412-455, 510-516; the same ramp weights as gpu_optimizer.py:
+// 961-1011).  The pixel-space part is evaluated on the source's window at
+// the galaxy's exact position (c_gauss_2d_grid, mix.i:452-524) and added to
+// the FFT render's unit-flux patch.
+#define FIT_PSF_MAXK 4
+#define FIT_MOG_MAXC (TRH_MAXK * FIT_PSF_MAXK)
+
+struct FitHybrid {
+    int K;
+    double amp[FIT_PSF_MAXK], mx[FIT_PSF_MAXK], my[FIT_PSF_MAXK];
+    double v0[FIT_PSF_MAXK], v1[FIT_PSF_MAXK], v2[FIT_PSF_MAXK];
+    double n1, n2;   // inner / outer real-space nsigma (3, 4)
+};
+
+// One pixel-space render: components k < K centred at (px, py) + m[k], with
+// c_gauss_2d_grid's constants q[k] = (amp / (2 pi sqrt det), -V2 / (2 det),
+// V1 / det, -V0 / (2 det)) (the exponent of pixel offset (dx, dy) is
+// q.y dx^2 + q.z dx dy + q.w dy^2, and terms with exponent <= -50 are
+// skipped, as the reference skips Mahalanobis distances >= 100).
+struct FitMogJob {
+    int64_t off;       // window offset in the output plane
+    int32_t stride, P, K;
+    float scale;       // counts for a model render, 1 for a unit-flux one
+    double px, py;
+    float4 q[FIT_MOG_MAXC];
+    float2 m[FIT_MOG_MAXC];
+};
+
 struct trh_fitter {
     trh_handle *h = nullptr;
     int S = 0, n_gal = 0, n_pt = 0, ph = 0, pw = 0;
@@ -259,6 +289,15 @@ struct trh_fitter {
     // the fitter's own row slots of the SZ >= 256 renders (sizes 256, 512),
     // so that fitters on different streams never share them
     float2 *hbuf[2] = {nullptr, nullptr};
+    // hybrid PSF (trh_fitter_set_hybrid): the PSF mixture, the pixel-space
+    // jobs beside the Jacobian / model and line-search FFT jobs (same
+    // slots), and the line search's pixel-space renders: galaxy s's NALPHA
+    // P x P windows at ls_base + boff[s] + a P^2
+    FitHybrid *hyb = nullptr;       // null: pure pixelized PSF
+    FitHybrid *hyb_dev = nullptr;   // its device copy, kept while hybrid is off
+    FitMogJob *mog_jac = nullptr, *mog_ls = nullptr;
+    float *ls_base = nullptr;
+    int64_t *boff = nullptr;
     std::vector<void *> allocs;
 };
 
@@ -281,7 +320,7 @@ __device__ __forceinline__ int sz_bucket(int sz) {
 // pixscale 1).  Returns the FFT size.
 __device__ int build_gal_job(trh_fft_job &J, const double *amp, const double *var, int K,
                              double x, double y, double logre, double ee1, double ee2, int P,
-                             int ph, int pw) {
+                             int ph, int pw, double (*dvar)[3] = nullptr) {
     const double re = exp(logre);
     const double ee = hypot(ee1, ee2);
     const double e = 1.0 - exp(-ee);
@@ -303,6 +342,11 @@ __device__ int build_gal_job(trh_fft_job &J, const double *amp, const double *va
         J.var[k][0] = (float)(v * M00);
         J.var[k][1] = (float)(v * M01);
         J.var[k][2] = (float)(v * M11);
+        if (dvar) {
+            dvar[k][0] = v * M00;
+            dvar[k][1] = v * M01;
+            dvar[k][2] = v * M11;
+        }
     }
     // halfsize with a modelMask (galaxy.py:337-351)
     double hs = fmax(P / 2.0, P / 2.0);
@@ -340,6 +384,125 @@ __device__ void build_point_job(trh_fft_job &J, double x, double y, int P, int p
     J.out_stride = P;
     J.psf = 0;
     J.K = 0;
+}
+
+// Split the sheared components (FP64 pixel variances dv) of a render with
+// FFT box size sz: the FFT job keeps the components narrow enough for the
+// box (amplitudes ramped where both apply) and the MoG job gets the wide
+// ones convolved with the PSF mixture (PSF component outermost, as
+// MixtureOfGaussians.convolve orders them, mixture_profiles.py:158-172).
+// A galaxy centred outside its window (sourceOut) is all pixel space
+// (galaxy.py:354-372, run_mog on the modelMask).  Returns false when the MoG
+// part has more than FIT_MOG_MAXC components.
+__device__ bool hybrid_split(const FitHybrid &H, int sz, int K, const double *amp,
+                             const double (*dv)[3], trh_fft_job &J, FitMogJob &M, double px,
+                             double py, int P) {
+    const double hw = sz / 2.0;   // pW / 2 of the padded PSF
+    const bool out = px < 0.0 || px > P - 1.0 || py < 0.0 || py > P - 1.0;
+    bool im[TRH_MAXK], iff[TRH_MAXK], ramp = false;
+    double ns[TRH_MAXK];
+    for (int k = 0; k < K; ++k) {
+        const double vv = dv[k][0] + dv[k][2];
+        im[k] = out || hw * hw < H.n2 * H.n2 * vv;
+        iff[k] = !out && hw * hw > H.n1 * H.n1 * vv;
+        ramp = ramp || (im[k] && iff[k]);
+        ns[k] = hw / fmax(1e-6, sqrt(vv));
+    }
+    int nf = 0;
+    for (int k = 0; k < K; ++k) {
+        if (!iff[k]) continue;
+        double a = amp[k];
+        if (ramp) a *= fmin(1.0, (ns[k] - H.n1) / (H.n2 - H.n1));
+        J.amp[nf] = (float)a;
+        J.var[nf][0] = (float)dv[k][0];
+        J.var[nf][1] = (float)dv[k][1];
+        J.var[nf][2] = (float)dv[k][2];
+        ++nf;
+    }
+    J.K = nf;
+    int nm = 0;
+    for (int k = 0; k < K; ++k) nm += im[k] ? 1 : 0;
+    M.px = px;
+    M.py = py;
+    M.K = 0;
+    M.scale = 1.f;
+    if (nm * H.K > FIT_MOG_MAXC) return false;
+    int c = 0;
+    for (int j = 0; j < H.K; ++j)
+        for (int k = 0; k < K; ++k) {
+            if (!im[k]) continue;
+            double a = amp[k];
+            if (ramp) a *= fmin(1.0, (H.n2 - ns[k]) / (H.n2 - H.n1));
+            a *= H.amp[j];
+            const double V0 = dv[k][0] + H.v0[j], V1 = dv[k][1] + H.v1[j], V2 = dv[k][2] + H.v2[j];
+            const double det = V0 * V2 - V1 * V1;
+            float4 q = make_float4((float)(a / sqrt(4.0 * M_PI * M_PI * det)),
+                                   (float)(-0.5 * V2 / det), (float)(V1 / det),
+                                   (float)(-0.5 * V0 / det));
+            if (!(det > 0.0)) q = make_float4(0.f, -1.f, 0.f, -1.f);
+            M.q[c] = q;
+            M.m[c] = make_float2((float)H.mx[j], (float)H.my[j]);
+            ++c;
+        }
+    M.K = c;
+    return true;
+}
+
+// Pixel-space parts of the galaxy jobs listed in the galaxy buckets
+// lists[b * cap + i], i < cnt[b], b < NFFT (their FitMogJob at the same
+// slot), into out: mode 0 stores, 1 adds (the FFT render of the same pixels
+// was stored first, in stream order), 2 adds atomically.  Plane:
+// out + (slot % nplanes) * plane (nplanes 1: one plane).  One block per job.
+__global__ void __launch_bounds__(256)
+k_fit_mog(const FitMogJob *__restrict__ jobs, const int32_t *__restrict__ lists, int cap,
+          const int32_t *__restrict__ cnt, int mode, float *__restrict__ out, int nplanes,
+          int64_t plane) {
+    __shared__ int pre[NFFT + 1];
+    __shared__ float4 Q[FIT_MOG_MAXC];
+    __shared__ int2 CI[FIT_MOG_MAXC];
+    __shared__ float2 CF[FIT_MOG_MAXC];
+    if (threadIdx.x == 0) {
+        pre[0] = 0;
+        for (int b = 0; b < NFFT; ++b) pre[b + 1] = pre[b] + cnt[b];
+    }
+    __syncthreads();
+    const int n = pre[NFFT];
+    for (int jj = blockIdx.x; jj < n; jj += gridDim.x) {
+        int b = 0;
+        while (jj >= pre[b + 1]) ++b;
+        const int slot = lists[(size_t)b * cap + (jj - pre[b])];
+        const FitMogJob &J = jobs[slot];
+        const int K = J.K, P = J.P;
+        if (K <= 0) continue;
+        __syncthreads();
+        if ((int)threadIdx.x < K) {
+            const double mx = J.px + J.m[threadIdx.x].x, my = J.py + J.m[threadIdx.x].y;
+            const double fx = floor(mx), fy = floor(my);
+            Q[threadIdx.x] = J.q[threadIdx.x];
+            CI[threadIdx.x] = make_int2((int)fx, (int)fy);
+            CF[threadIdx.x] = make_float2((float)(mx - fx), (float)(my - fy));
+        }
+        __syncthreads();
+        float *o = out + (int64_t)(slot % nplanes) * plane + J.off;
+        for (int pix = threadIdx.x; pix < P * P; pix += blockDim.x) {
+            const int oy = pix / P, ox = pix - oy * P;
+            float r = 0.f;
+            for (int k = 0; k < K; ++k) {
+                const float4 q = Q[k];
+                if (q.x == 0.f) continue;
+                const float dx = (float)(ox - CI[k].x) - CF[k].x;
+                const float dy = (float)(oy - CI[k].y) - CF[k].y;
+                const float e = q.y * dx * dx + q.z * dx * dy + q.w * dy * dy;
+                if (!(e > -50.f)) continue;
+                r += q.x * __expf(e);
+            }
+            const int64_t idx = (int64_t)oy * J.stride + ox;
+            r *= J.scale;
+            if (mode == 0) o[idx] = r;
+            else if (mode == 1) o[idx] += r;
+            else atomicAdd(o + idx, r);
+        }
+    }
 }
 
 __global__ void k_fit_reset(int32_t *cnt, int n) {
@@ -464,7 +627,8 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
                                int ph, int pw, trh_fft_job *__restrict__ jobs,
                                int8_t *__restrict__ bk, int32_t *__restrict__ bcnt,
                                int32_t *__restrict__ flags, int32_t *__restrict__ act_out,
-                               unsigned long long *__restrict__ acc) {
+                               unsigned long long *__restrict__ acc,
+                               const FitHybrid *__restrict__ hyb, FitMogJob *__restrict__ mog) {
     __shared__ trh_fft_job sj[PREP_TB];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = t / nj, k = t - s * nj;
@@ -497,16 +661,27 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
                 sstep[s] = st;
                 n += st;
             }
-            double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1];
+            double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1], dv[TRH_MAXK][3];
             const double *amp, *var;
             const int K = gal_profile(*pr, ser, kd, n, amp, var, sa, sv);
-            const int sz = build_gal_job(J, amp, var, K, p[0], p[1], sh[0], sh[1], sh[2], P, ph, pw);
+            const int sz = build_gal_job(J, amp, var, K, p[0], p[1], sh[0], sh[1], sh[2], P, ph,
+                                         pw, hyb ? dv : nullptr);
             J.out_offset = roff[s] + (int64_t)k * P * P;
             b = sz_bucket(sz);
+            if (b >= 0 && hyb) {
+                FitMogJob &M = mog[t];
+                M.off = J.out_offset;
+                M.stride = P;
+                M.P = P;
+                if (!hybrid_split(*hyb, sz, K, amp, dv, J, M, p[0], p[1], P)) {
+                    atomicOr(&flags[s], TRH_FIT_F_HYBRID);
+                    b = -1;
+                }
+            }
             if (b < 0) {
                 // no Jacobian for this source: it stops at its current
                 // parameters, flagged (the other sources go on)
-                atomicOr(&flags[s], TRH_FIT_F_JAC_SIZE);
+                if (sz_bucket(sz) < 0) atomicOr(&flags[s], TRH_FIT_F_JAC_SIZE);
                 act_out[s] = 0;
             }
         }
@@ -538,7 +713,8 @@ __global__ void k_fit_prep_model(int S, int nj, const int32_t *__restrict__ kind
                                  const SersicTab *__restrict__ ser, int ph, int pw,
                                  trh_fft_job *__restrict__ jobs, int32_t *__restrict__ lists,
                                  int cap, int32_t *__restrict__ cnt,
-                                 int32_t *__restrict__ flags) {
+                                 int32_t *__restrict__ flags, const FitHybrid *__restrict__ hyb,
+                                 FitMogJob *__restrict__ mog) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const int kd = kind[s];
@@ -550,14 +726,26 @@ __global__ void k_fit_prep_model(int S, int nj, const int32_t *__restrict__ kind
         build_point_job(J, p[0], p[1], P, ph, pw);
         b = PTB;
     } else {
-        double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1];
+        double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1], dv[TRH_MAXK][3];
         const double *amp, *var;
         const int K = gal_profile(*pr, ser, kd, p[6], amp, var, sa, sv);
-        const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
+        const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw,
+                                     hyb ? dv : nullptr);
         b = sz_bucket(sz);
         if (b < 0) {
             atomicOr(&flags[s], TRH_FIT_F_MODEL_SIZE);
             return;
+        }
+        if (hyb) {
+            FitMogJob &M = mog[(size_t)nj * s];
+            M.off = pix_off[s];
+            M.stride = stride[s];
+            M.P = P;
+            if (!hybrid_split(*hyb, sz, K, amp, dv, J, M, p[0], p[1], P)) {
+                atomicOr(&flags[s], TRH_FIT_F_HYBRID);
+                return;
+            }
+            M.scale = (float)p[2];
         }
     }
     J.out_offset = pix_off[s];
@@ -940,7 +1128,9 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
                               trh_fft_job *__restrict__ jobs, int8_t *__restrict__ bk,
                               int32_t *__restrict__ bcnt, int32_t *__restrict__ flags,
                               double *__restrict__ chi2ls,
-                              unsigned long long *__restrict__ acc) {
+                              unsigned long long *__restrict__ acc,
+                              const FitHybrid *__restrict__ hyb, FitMogJob *__restrict__ mog,
+                              const int64_t *__restrict__ boff) {
     __shared__ trh_fft_job sj[PREP_TB];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slot / NALPHA, a = slot - s * NALPHA;
@@ -955,16 +1145,32 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
             J.pad0 = NEQ_W * s + NEQ_CHI0;   // the window's zero-model chi^2 in neq
             b = PTB;
         } else {
-            double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1];
+            double sa[TRH_MAXK + 1], sv[TRH_MAXK + 1], dv[TRH_MAXK][3];
             const double *amp, *var;
             const int K = gal_profile(*pr, ser, kd, p[6], amp, var, sa, sv);
-            const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw);
+            const int sz = build_gal_job(J, amp, var, K, p[0], p[1], p[3], p[4], p[5], P, ph, pw,
+                                         hyb ? dv : nullptr);
             b = sz_bucket(sz);
+            J.pad0 = 0;   // no base plane
+            if (b >= 0 && hyb) {
+                FitMogJob &M = mog[slot];
+                M.off = boff[s] + (int64_t)a * P * P;
+                M.stride = P;
+                M.P = P;
+                if (!hybrid_split(*hyb, sz, K, amp, dv, J, M, p[0], p[1], P)) {
+                    atomicOr(&flags[s], TRH_FIT_F_HYBRID);
+                    b = -1;
+                } else if (M.K > 0) {
+                    // its pixel-space part, rendered first (fits int32:
+                    // checked by trh_fitter_set_hybrid)
+                    J.pad0 = (int32_t)(M.off + 1);
+                }
+            }
             if (b < 0) {
                 // not rendered: the trial counts as non-finite, which ends
                 // the walk over the steps there (tryUpdates' isfinite break)
                 chi2ls[slot] = INFINITY;
-                atomicOr(&flags[s], TRH_FIT_F_LS_SIZE);
+                if (sz_bucket(sz) < 0) atomicOr(&flags[s], TRH_FIT_F_LS_SIZE);
             }
         }
         J.out_offset = pix_off[s];
@@ -1389,6 +1595,18 @@ static float2 *fitter_hbuf(trh_fitter *f, int sz) {
     return sz == 256 ? f->hbuf[0] : (sz == 512 ? f->hbuf[1] : nullptr);
 }
 
+// The pixel-space parts of a hybrid fitter's galaxy jobs (k_fit_mog).
+static int launch_mog(trh_fitter *f, const FitMogJob *jobs, const int32_t *lists, int cap,
+                      const int32_t *cnt, int mode, float *out, int nplanes, int64_t plane,
+                      hipStream_t st) {
+    if (!f->hyb || !f->n_gal) return TRH_OK;
+    int grid = f->h->n_cu * 8;
+    if (grid > NALPHA * f->n_gal) grid = NALPHA * f->n_gal;
+    hipLaunchKernelGGL(k_fit_mog, dim3(grid), dim3(256), 0, st, jobs, lists, cap, cnt, mode, out,
+                       nplanes, plane);
+    return hip_check(hipGetLastError(), "k_fit_mog");
+}
+
 static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_iter, hipStream_t st,
                         bool capture) {
     const int S = f->S;
@@ -1405,7 +1623,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
     const int nb_jac = (cap_jac + PREP_TB - 1) / PREP_TB, nb_ls = (cap_ls + PREP_TB - 1) / PREP_TB;
     hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb_jac), dim3(PREP_TB), 0, st, S, f->nj, f->kind, f->P,
                        f->render_off, f->params, f->active, f->prof, f->ser, f->sstep, f->ph, f->pw, f->jobs_jac,
-                       f->bk_jac, f->bcnt_jac, f->flags, f->active, acc);
+                       f->bk_jac, f->bcnt_jac, f->flags, f->active, acc, f->hyb, f->mog_jac);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_jac"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_jac), dim3(PREP_TB), 0, st, cap_jac, f->bk_jac,
                        f->bcnt_jac, f->list_jac, cap_jac, f->cnt);
@@ -1420,6 +1638,9 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
             return rc;
         SLOT_END(b);
     }
+    // hybrid PSF: the pixel-space parts added to the galaxies' FFT renders
+    if ((rc = launch_mog(f, f->mog_jac, f->list_jac, cap_jac, f->cnt, 1, f->renders, 1, 0, st)))
+        return rc;
     SLOT_BEGIN(SL_JPT);
     if ((rc = launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt,
                            f->renders, nullptr, st)))
@@ -1455,17 +1676,23 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
     if (!capture) (void)hipEventRecord(f->ev[3], st);
     hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb_ls), dim3(PREP_TB), 0, st, S, f->kind, f->P,
                        f->stride, f->pix_off, f->trial, f->nsteps, f->active, f->prof, f->ser,
-                       f->ph, f->pw, f->jobs_ls, f->bk_ls, f->bcnt_ls, f->flags, f->chi2ls, acc);
+                       f->ph, f->pw, f->jobs_ls, f->bk_ls, f->bcnt_ls, f->flags, f->chi2ls, acc,
+                       f->hyb, f->mog_ls, f->boff);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_ls), dim3(PREP_TB), 0, st, cap_ls, f->bk_ls,
                        f->bcnt_ls, f->list_ls, cap_ls, f->cnt + LS_CNT);
     if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
+    // hybrid PSF: each trial's pixel-space part into its ls_base window,
+    // which its chi^2 render adds to the FFT model
+    if ((rc = launch_mog(f, f->mog_ls, f->list_ls, cap_ls, f->cnt + LS_CNT, 0, f->ls_base, 1, 0, st)))
+        return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         if (kSizes[b] < f->min_sz) continue;
         SLOT_BEGIN(SL_LS + b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
                                  f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
-                                 nullptr, f->data, f->inverr, f->chi2ls, st, fitter_hbuf(f, kSizes[b]))))
+                                 nullptr, f->data, f->inverr, f->chi2ls, st, fitter_hbuf(f, kSizes[b]),
+                                 f->hyb ? f->ls_base : nullptr)))
             return rc;
         SLOT_END(SL_LS + b);
     }
@@ -1599,6 +1826,81 @@ extern "C" int trh_fitter_profile_read(trh_fitter *f, double *out) {
     return TRH_OK;
 }
 
+extern "C" int trh_fitter_set_hybrid(trh_fitter *f, int K, const double *amp, const double *mean,
+                                     const double *var, double inner_nsigma,
+                                     double outer_nsigma) {
+    if (!f || K < 0 || K > FIT_PSF_MAXK || (K > 0 && (!amp || !mean || !var)))
+        return set_err(TRH_E_ARG, "trh_fitter_set_hybrid: K must be 0..%d with amp, mean, var",
+                       FIT_PSF_MAXK);
+    if (K > 0 && !(inner_nsigma > 0 && outer_nsigma > inner_nsigma))
+        return set_err(TRH_E_ARG, "trh_fitter_set_hybrid: need 0 < inner_nsigma < outer_nsigma");
+    int rc;
+    if ((rc = hip_check(hipSetDevice(f->h->device), "hipSetDevice"))) return rc;
+    // the captured iteration's launches change: capture again on next use
+    if (f->gexec) {
+        (void)hipGraphExecDestroy(f->gexec);
+        f->gexec = nullptr;
+    }
+    if (K == 0) {
+        f->hyb = nullptr;
+        return TRH_OK;
+    }
+    FitHybrid H{};
+    H.K = K;
+    H.n1 = inner_nsigma;
+    H.n2 = outer_nsigma;
+    for (int k = 0; k < K; ++k) {
+        H.amp[k] = amp[k];
+        H.mx[k] = mean[2 * k];
+        H.my[k] = mean[2 * k + 1];
+        H.v0[k] = var[3 * k];
+        H.v1[k] = var[3 * k + 1];
+        H.v2[k] = var[3 * k + 2];
+        if (!(var[3 * k] > 0 && var[3 * k] * var[3 * k + 2] - var[3 * k + 1] * var[3 * k + 1] > 0))
+            return set_err(TRH_E_ARG, "trh_fitter_set_hybrid: PSF component %d variance is not "
+                                      "positive definite", k);
+    }
+    const int S = f->S;
+    if (!f->mog_jac) {
+        std::vector<int32_t> kind(S), P(S);
+        if ((rc = hip_check(hipMemcpy(kind.data(), f->kind, 4 * (size_t)S, hipMemcpyDeviceToHost),
+                            "read kinds")) ||
+            (rc = hip_check(hipMemcpy(P.data(), f->P, 4 * (size_t)S, hipMemcpyDeviceToHost),
+                            "read sizes")))
+            return rc;
+        std::vector<int64_t> boff(S, 0);
+        int64_t tot = 0;
+        for (int s = 0; s < S; ++s) {
+            boff[s] = tot;
+            if (kind[s]) tot += (int64_t)NALPHA * P[s] * P[s];
+        }
+        // the chi^2 jobs carry their window's offset + 1 in an int32
+        if (tot >= (int64_t)INT32_MAX)
+            return set_err(TRH_E_UNSUPPORTED, "trh_fitter_set_hybrid: %lld line-search pixels "
+                           "exceed 2^31 - 1", (long long)tot);
+        FitMogJob *mj = nullptr, *ml = nullptr;
+        FitHybrid *hy = nullptr;
+        int64_t *bo = nullptr;
+        float *lb = nullptr;
+        if ((rc = falloc(f, &hy, 1)) || (rc = falloc(f, &mj, (size_t)f->nj * S)) ||
+            (rc = falloc(f, &ml, (size_t)NALPHA * S)) || (rc = falloc(f, &bo, S)) ||
+            (rc = falloc(f, &lb, tot > 0 ? tot : 1)))
+            return rc;
+        if ((rc = hip_check(hipMemcpy(bo, boff.data(), 8 * (size_t)S, hipMemcpyHostToDevice),
+                            "window offsets")))
+            return rc;
+        f->mog_jac = mj;
+        f->mog_ls = ml;
+        f->boff = bo;
+        f->ls_base = lb;
+        f->hyb_dev = hy;
+    }
+    if ((rc = hip_check(hipMemcpy(f->hyb_dev, &H, sizeof(H), hipMemcpyHostToDevice), "hybrid PSF")))
+        return rc;
+    f->hyb = f->hyb_dev;
+    return TRH_OK;
+}
+
 extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, int accumulate,
                                         void *stream) {
     if (!f || !out) return set_err(TRH_E_ARG, "trh_fitter_render_models: null argument");
@@ -1612,7 +1914,7 @@ extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, int accumulat
     const int mode = accumulate ? TRH_ADD : TRH_STORE;
     hipLaunchKernelGGL(k_fit_prep_model, dim3((S + 255) / 256), dim3(256), 0, st, S, f->nj, f->kind, f->P,
                        f->stride, mode, f->pix_off, f->params, f->prof, f->ser, f->ph, f->pw,
-                       f->jobs_jac, f->list_jac, cap_jac, f->cnt, f->flags);
+                       f->jobs_jac, f->list_jac, cap_jac, f->cnt, f->flags, f->hyb, f->mog_jac);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_model"))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         if ((rc = launch_fft_any(f->h, kSizes[b], accumulate ? -1 : TRH_STORE, f->jobs_jac,
@@ -1620,6 +1922,9 @@ extern "C" int trh_fitter_render_models(trh_fitter *f, float *out, int accumulat
                                  nullptr, nullptr, st, fitter_hbuf(f, kSizes[b]))))
             return rc;
     }
+    if ((rc = launch_mog(f, f->mog_jac, f->list_jac, cap_jac, f->cnt, accumulate ? 2 : 1, out, 1, 0,
+                         st)))
+        return rc;
     return launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt, out,
                         nullptr, st, mode);
 }
@@ -1731,17 +2036,21 @@ extern "C" int trh_line_search_chi2_batch(trh_fitter *f, const double *trial,
     hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
     hipLaunchKernelGGL(k_fit_prep_ls, dim3(nb_ls), dim3(PREP_TB), 0, st, S, f->kind, f->P,
                        f->stride, f->pix_off, trial, nsteps, nullptr, f->prof, f->ser, f->ph,
-                       f->pw, f->jobs_ls, f->bk_ls, f->bcnt_ls, f->flags, chi2, nullptr);
+                       f->pw, f->jobs_ls, f->bk_ls, f->bcnt_ls, f->flags, chi2, nullptr, f->hyb,
+                       f->mog_ls, f->boff);
     if ((rc = hip_check(hipGetLastError(), "k_fit_prep_ls"))) return rc;
     hipLaunchKernelGGL(k_fit_lists, dim3(nb_ls), dim3(PREP_TB), 0, st, cap_ls, f->bk_ls,
                        f->bcnt_ls, f->list_ls, cap_ls, f->cnt + LS_CNT);
     if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
+    if ((rc = launch_mog(f, f->mog_ls, f->list_ls, cap_ls, f->cnt + LS_CNT, 0, f->ls_base, 1, 0, st)))
+        return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         if (kSizes[b] < f->min_sz) continue;
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
                                  f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b,
                                  NALPHA * f->n_gal, f->maxrows[b], f->spec[b], nullptr, f->data,
-                                 f->inverr, chi2, st, fitter_hbuf(f, kSizes[b]))))
+                                 f->inverr, chi2, st, fitter_hbuf(f, kSizes[b]),
+                                 f->hyb ? f->ls_base : nullptr)))
             return rc;
     }
     // point sources over their whole windows (no normal equations assumed)

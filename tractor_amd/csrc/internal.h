@@ -41,10 +41,12 @@ int fft_hbuf_slots(const trh_handle *h, int sz);
 int reserve_fft_hbuf(trh_handle *h, int sz);
 // `hbuf`: the row slots of the SZ >= 256 renders (fft_hbuf_slots(h, sz)
 // slots of fft_hbuf_slot_floats(sz) floats), or NULL for the handle's own
+// `base` (TRH_CHI2 only): every job with pad0 > 0 adds the window at
+// base + pad0 - 1 (row stride mw) to its model (the TRH_CHI2_BASE kernel).
 int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
                    float *out, const float *data, const float *inverr, double *chi2,
-                   hipStream_t st, float2 *hbuf = nullptr);
+                   hipStream_t st, float2 *hbuf = nullptr, const float *base = nullptr);
 // Bracket every use of the handle's shared buffers (scratch, hbuf) on `st`:
 // shared_begin orders `st` after the last use on another stream, shared_end
 // records this use.  Streams being captured into a graph are not ordered

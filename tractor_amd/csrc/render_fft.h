@@ -94,6 +94,12 @@ __device__ __forceinline__ bool epilogue_buffers_ok(int mode, const float *out, 
     return (mode == TRH_STORE || mode == TRH_ADD) && out;
 }
 
+// Kernel mode of the chi^2 renders whose unit-flux model is the FFT render
+// plus a pixel-space part already rendered into `base` (the MoG half of a
+// hybrid-PSF galaxy): the job's window of it starts at base + pad0 - 1 with
+// row stride mw when pad0 > 0; pad0 = 0: none.  Used by the batched fitter.
+#define TRH_CHI2_BASE 18
+
 // Epilogue of one output pixel.  Returns the chi^2 contribution (TRH_CHI2).
 __device__ __forceinline__ double epilogue(int mode, float val, float scale, int64_t idx,
                                            float *__restrict__ out,
@@ -343,7 +349,8 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                                const float2 *__restrict__ twg, float2 *ltab,
                                float2 *lbuf, float2 *hbuf, int maxrows, RenderScratch &S,
                                float *__restrict__ out, const float *__restrict__ data,
-                               const float *__restrict__ inverr, double *__restrict__ chi2) {
+                               const float *__restrict__ inverr, double *__restrict__ chi2,
+                               const float *__restrict__ base) {
     using G = WaveGeom<SZ>;
     constexpr int NV = G::NV, R = G::R, SEQL = G::SEQL, NSEQ = G::NSEQ, HS = G::HS;
     constexpr int LOG2 = Log2<SZ>::v;
@@ -360,8 +367,17 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
     const int64_t off = Jp->out_offset;
     const int stride = Jp->out_stride;
     const int mw = Jp->mw, mh = Jp->mh, sx = Jp->sx, sy = Jp->sy;
-    // MODE >= 0: epilogue fixed at compile time; -1: read from the job
-    const int K = min(max(Jp->K, 0), TRH_MAXK), mode = MODE >= 0 ? MODE : Jp->mode;
+    // MODE >= 0: epilogue fixed at compile time; -1: read from the job;
+    // TRH_CHI2_BASE: chi^2 with the job's base plane added to the model
+    constexpr bool BASE = MODE == TRH_CHI2_BASE;
+    constexpr int EMODE = BASE ? TRH_CHI2 : MODE;
+    const int K = min(max(Jp->K, 0), TRH_MAXK), mode = EMODE >= 0 ? EMODE : Jp->mode;
+    const float *__restrict__ bp = BASE && Jp->pad0 > 0 ? base + (Jp->pad0 - 1) : nullptr;
+    // the base value of window pixel (oy, ox)
+    auto basev = [&](int oy, int ox) -> float {
+        if constexpr (BASE) return bp ? bp[(int64_t)oy * mw + ox] : 0.f;
+        return 0.f;
+    };
     // a job whose epilogue has no buffer to use renders nothing (the same
     // decision in every thread of the block, so no barrier is left waiting)
     if (!epilogue_buffers_ok(mode, out, data, inverr, chi2)) return;
@@ -411,7 +427,8 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
             const int r = idx / mw;
             const int ox = idx - r * mw;
             const int oy = r < top ? r : bot + (r - top);
-            acc += epilogue(mode, 0.f, scale, off + (int64_t)oy * stride + ox, out, data, inverr);
+            acc += epilogue(mode, basev(oy, ox), scale, off + (int64_t)oy * stride + ox, out, data,
+                            inverr);
         }
     }
 
@@ -593,7 +610,7 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 }
                 return make_float2(va * inv_norm, vb * inv_norm);
             };
-            if (MODE == TRH_CHI2 && mw <= SZ) {
+            if (EMODE == TRH_CHI2 && mw <= SZ) {
                 // (compile-time chi^2 kernels only: in the kernel that reads
                 // the mode from the job it would raise every mode's register
                 // pressure) the data / inverr loads of TB columns per lane
@@ -603,6 +620,7 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                 // halfsize covers it), the clamped pixels' terms dropped
                 constexpr int NT = SZ / SEQL, TB = SZ >= 256 ? 1 : (NT < 2 ? NT : 2);
                 int64_t rowa[CW], rowb[CW];
+                int oya[CW];
                 bool act[CW], hasb[CW];
 #pragma unroll
                 for (int c = 0; c < CW; ++c) {
@@ -610,12 +628,13 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                     act[c] = pp < npairs;
                     const int qa = act[c] ? 2 * pp : 0;
                     hasb[c] = act[c] && (qa + 1) < nrb;
-                    rowa[c] = off + (int64_t)(rb0 + qa + sy) * stride;
+                    oya[c] = rb0 + qa + sy;
+                    rowa[c] = off + (int64_t)oya[c] * stride;
                     rowb[c] = hasb[c] ? rowa[c] + stride : rowa[c];
                 }
 #pragma unroll
                 for (int t0 = 0; t0 < NT; t0 += TB) {
-                    float dv[CW][TB][2], iv[CW][TB][2];
+                    float dv[CW][TB][2], iv[CW][TB][2], bv[BASE ? CW : 1][BASE ? TB : 1][2];
 #pragma unroll
                     for (int c = 0; c < CW; ++c)
 #pragma unroll
@@ -625,6 +644,10 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                             iv[c][tt][0] = ld_stream(inverr + rowa[c] + oxc);
                             dv[c][tt][1] = ld_stream(data + rowb[c] + oxc);
                             iv[c][tt][1] = ld_stream(inverr + rowb[c] + oxc);
+                            if constexpr (BASE) {
+                                bv[c][tt][0] = basev(oya[c], oxc);
+                                bv[c][tt][1] = basev(hasb[c] ? oya[c] + 1 : oya[c], oxc);
+                            }
                         }
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
@@ -632,7 +655,11 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
 #pragma unroll
                         for (int tt = 0; tt < TB; ++tt) {
                             const int ox = sl + SEQL * (t0 + tt);
-                            const float2 v = pair_value(lb, min(ox, mw - 1) - sx);
+                            float2 v = pair_value(lb, min(ox, mw - 1) - sx);
+                            if constexpr (BASE) {
+                                v.x += bv[c][tt][0];
+                                v.y += bv[c][tt][1];
+                            }
                             const float ca = (dv[c][tt][0] - v.x * scale) * iv[c][tt][0];
                             const float cb = (dv[c][tt][1] - v.y * scale) * iv[c][tt][1];
                             if (act[c] && ox < mw) acc += (double)ca * (double)ca;
@@ -651,10 +678,10 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
                     const int oya = rb0 + qa + sy;
                     for (int ox = sl; ox < mw; ox += SEQL) {
                         const float2 v = pair_value(lb, ox - sx);
-                        acc += epilogue(mode, v.x, scale, off + (int64_t)oya * stride + ox, out,
-                                        data, inverr);
+                        acc += epilogue(mode, v.x + basev(oya, ox), scale,
+                                        off + (int64_t)oya * stride + ox, out, data, inverr);
                         if (hasb)
-                            acc += epilogue(mode, v.y, scale,
+                            acc += epilogue(mode, v.y + basev(oya + 1, ox), scale,
                                             off + (int64_t)(oya + 1) * stride + ox, out, data,
                                             inverr);
                     }
@@ -818,7 +845,7 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 
 // Grid-stride render kernel over a job list.  `count` (device, may be NULL:
 // then njobs) bounds the list; `list` (may be NULL: identity) maps list
-// entries to job indices.  blockDim.x is a multiple of 64; dynamic LDS holds
+// entries to job indices.  base: MODE TRH_CHI2_BASE only.  blockDim.x is a multiple of 64; dynamic LDS holds
 // fft_lds_bytes(SZ, blockDim.x / 64, maxrows).  For SZ >= 256 the rows live
 // in hbuf_g, one slot of fft_hbuf_slot_floats(SZ) floats per block (gridDim.x
 // slots), and every pass takes all SZ rows.
@@ -828,7 +855,8 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
              const int *__restrict__ count, int njobs, const float2 *__restrict__ psf_spec,
              const float2 *__restrict__ twg, int maxrows, float *__restrict__ out,
              const float *__restrict__ data, const float *__restrict__ inverr,
-             double *__restrict__ chi2, float2 *__restrict__ hbuf_g) {
+             double *__restrict__ chi2, float2 *__restrict__ hbuf_g,
+             const float *__restrict__ base) {
     extern __shared__ float2 smem[];
     __shared__ RenderScratch S;
     using G = WaveGeom<SZ>;
@@ -855,10 +883,10 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
         const bool fd = ylo >= 3 && yhi <= SZ - 3 && -sx >= 3 && Jp->mw - sx <= SZ - 3;
         if (fd)
             render_fft_one<SZ, MODE, true>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, S,
-                                           out, data, inverr, chi2);
+                                           out, data, inverr, chi2, base);
         else
             render_fft_one<SZ, MODE, false>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, S,
-                                            out, data, inverr, chi2);
+                                            out, data, inverr, chi2, base);
     }
 }
 

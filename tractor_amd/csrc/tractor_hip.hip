@@ -364,7 +364,7 @@ template <int SZ>
 int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list,
                const int *count, int njobs_bound, int maxrows, const float *psf_spec, float *out,
                const float *data, const float *inverr, double *chi2, hipStream_t st,
-               float2 *hbuf) {
+               float2 *hbuf, const float *base) {
     int nt, rows, grid;
     size_t lds;
     fft_launch_shape(h, SZ, maxrows, &nt, &rows, &lds, &grid, njobs_bound);
@@ -390,15 +390,21 @@ int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list
     switch (mode) {
     case TRH_STORE:
         hipLaunchKernelGGL((k_render_fft<SZ, TRH_STORE>), dim3(grid), dim3(nt), lds, st, jobs, list,
-                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb);
+                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb, nullptr);
         break;
     case TRH_CHI2:
-        hipLaunchKernelGGL((k_render_fft<SZ, TRH_CHI2>), dim3(grid), dim3(nt), lds, st, jobs, list,
-                           count, njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb);
+        if (base)
+            hipLaunchKernelGGL((k_render_fft<SZ, TRH_CHI2_BASE>), dim3(grid), dim3(nt), lds, st,
+                               jobs, list, count, njobs_bound, ps, tw, rows, out, data, inverr,
+                               chi2, hb, base);
+        else
+            hipLaunchKernelGGL((k_render_fft<SZ, TRH_CHI2>), dim3(grid), dim3(nt), lds, st, jobs,
+                               list, count, njobs_bound, ps, tw, rows, out, data, inverr, chi2,
+                               hb, nullptr);
         break;
     default:
         hipLaunchKernelGGL((k_render_fft<SZ, -1>), dim3(grid), dim3(nt), lds, st, jobs, list, count,
-                           njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb);
+                           njobs_bound, ps, tw, rows, out, data, inverr, chi2, hb, nullptr);
         break;
     }
     return hip_check(hipGetLastError(), "k_render_fft");
@@ -407,23 +413,25 @@ int launch_fft(trh_handle *h, int mode, const trh_fft_job *jobs, const int *list
 int launch_fft_any(trh_handle *h, int sz, int mode, const trh_fft_job *jobs, const int *list,
                    const int *count, int njobs_bound, int maxrows, const float *psf_spec,
                    float *out, const float *data, const float *inverr, double *chi2,
-                   hipStream_t st, float2 *hbuf) {
+                   hipStream_t st, float2 *hbuf, const float *base) {
+    if (base && mode != TRH_CHI2)
+        return set_err(TRH_E_ARG, "launch_fft_any: a base plane needs chi^2 mode");
     switch (sz) {
     case 32:
         return launch_fft<32>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                              data, inverr, chi2, st, hbuf);
+                              data, inverr, chi2, st, hbuf, base);
     case 64:
         return launch_fft<64>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                              data, inverr, chi2, st, hbuf);
+                              data, inverr, chi2, st, hbuf, base);
     case 128:
         return launch_fft<128>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                               data, inverr, chi2, st, hbuf);
+                               data, inverr, chi2, st, hbuf, base);
     case 256:
         return launch_fft<256>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                               data, inverr, chi2, st, hbuf);
+                               data, inverr, chi2, st, hbuf, base);
     case 512:
         return launch_fft<512>(h, mode, jobs, list, count, njobs_bound, maxrows, psf_spec, out,
-                               data, inverr, chi2, st, hbuf);
+                               data, inverr, chi2, st, hbuf, base);
     }
     return set_err(TRH_E_UNSUPPORTED, "FFT render size %d not in {32,64,128,256,512}", sz);
 }
