@@ -80,3 +80,45 @@ def test_mog_psf_minval_patch():
     assert (p.x0, p.y0) == (x0 + sx0, y0 + sy0)
     np.testing.assert_allclose(p.patch, res[sy0:sy1, sx0:sx1], rtol=1e-12, atol=0)
     assert abs(p.patch.sum() - 1.) < 1e-3
+
+
+def test_galaxy_mog_path_with_minval():
+    """A galaxy under a GaussianMixturePSF without a modelMask and with
+    img.modelMinval > 0: run_mog -> mixture_to_patch -> approx3 ring growth
+    (tractor/galaxy.py:307-333, mixture_profiles.py:425-438) on the no-mask
+    box (halfsize rule, galaxy.py:273-290, 678-691), minval = modelMinval /
+    counts; the sliced patch equals the oracle's approx3 of the convolved
+    mixture and the model is counts times it."""
+    import tractor_amd as T
+    from oracle import tractor_oracle as O
+    psf = T.GaussianMixturePSF([0.8, 0.2], [[0., 0.], [0.3, -0.2]],
+                               [[[4., 0.5], [0.5, 3.]], [[16., -2.], [-2., 12.]]])
+    H = W = 96
+    tim = T.Image(data=np.zeros((H, W), np.float32), inverr=np.ones((H, W), np.float32), psf=psf)
+    tim.modelMinval = 0.05
+    px, py, flux, logre, ee1, ee2 = 47.3, 45.8, 300., np.log(3.), 0.2, -0.1
+    gal = T.ExpGalaxy(T.PixPos(px, py), T.Flux(flux), T.EllipseESoft(logre, ee1, ee2))
+    p = gal.getModelPatch(tim)
+    print('minval galaxy patch', p.x0, p.y0, p.patch.shape, flush=True)
+    # oracle: the sheared profile convolved with the PSF mixture, on the box
+    amp, var = O.exp_mixture()
+    vp = O.shear_mixture(var, np.eye(2) * 3600., O.ellipse_esoft_basis(logre, ee1, ee2))
+    mog = psf.getMixtureOfGaussians()
+    mean = np.zeros((len(amp), 2)) + np.array([px, py])
+    camp, cmean, cvar = O.mog_convolve(amp, mean, vp, mog.amp, mog.mean, mog.var)
+    hs = O.galaxy_patch_halfsize(4. * 3., px, py, (H, W), psf.getRadius())
+    ox, _ = O.get_overlapping_region(int(np.floor(px - hs)), int(np.ceil(px + hs + 1)), 0, W - 1)
+    oy, _ = O.get_overlapping_region(int(np.floor(py - hs)), int(np.ceil(py + hs + 1)), 0, H - 1)
+    x0, x1, y0, y1 = ox.start, ox.stop, oy.start, oy.stop
+    minval = 0.05 / flux
+    res, _, _, (sx0, sx1, sy0, sy1) = O.mog_grid_approx3(
+        x0, x1, y0, y1, 0., 0., minval, camp, cmean, cvar, int(cmean[0, 0]), int(cmean[0, 1]))
+    ref = res[sy0:sy1, sx0:sx1]
+    assert (p.x0, p.y0) == (x0 + sx0, y0 + sy0)
+    assert p.patch.shape == ref.shape
+    # FP64 approx3, the model cast to float32 after the counts product
+    np.testing.assert_allclose(p.patch, (ref * flux).astype(np.float32), rtol=1e-6,
+                               atol=1e-6 * flux * ref.max())
+    # ring growth stops below 0.05 counts per pixel: the faint wings (about
+    # 1.6% of the flux here) are cut, as in the reference
+    assert 0.95 < p.patch.sum() / flux < 1.

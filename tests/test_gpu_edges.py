@@ -55,7 +55,9 @@ def test_largest_fft_size_matches_oracle(dev):
     torch.cuda.synchronize()
     _, _, G = O.render_galaxy_fft(amp, var, 130.3, 129.6, psf, mm=(0, 0, P, P))
     got = out.cpu().numpy().reshape(P, P)
-    assert np.max(np.abs(got - G)) <= 2e-5 * np.max(np.abs(G))
+    e = np.max(np.abs(got - G)) / np.max(np.abs(G))
+    print('  sz 512 max err / peak %.3e' % e, flush=True)
+    assert e <= 1e-5
 
 
 def test_thin_and_one_pixel_windows(dev):
@@ -80,7 +82,8 @@ def test_thin_and_one_pixel_windows(dev):
         torch.cuda.synchronize()
         _, _, G = O.render_galaxy_fft(amp, var, px, py, psf, mm=mm)
         got = out.cpu().numpy().reshape(mm[3], mm[2])
-        assert np.max(np.abs(got - G)) <= 2e-5 * max(np.max(np.abs(G)), 1e-30) + 1e-12, mm
+        print('  max err %.3e peak %.3e' % (np.max(np.abs(got - G)), np.max(np.abs(G))), flush=True)
+        assert np.max(np.abs(got - G)) <= 1e-5 * max(np.max(np.abs(G)), 1e-30) + 1e-12, mm
 
 
 def test_forced_photometry_sources_outside_the_image(dev):

@@ -1,7 +1,7 @@
 """Batched forced photometry on the device (tractor_amd.forced) against the
 FP64 oracle (oracle.forced_unit_patch + forced_phot_lsq) on a 384x384 scene
 of 240 overlapping sources with a bilinearly interpolated 3x3 PSF grid.
-Unit-flux patches agree to FP32 rendering accuracy (2e-5 of each patch's
+Unit-flux patches agree to FP32 rendering accuracy (1e-5 of each patch's
 peak); fluxes within 1e-4 relative or 0.01 sigma (sigma = iv^-1/2)."""
 import numpy as np
 import pytest
@@ -44,6 +44,7 @@ def test_forced_photometry_matches_oracle():
     print('pcg iterations', fp.iterations, 'resid', fp.resid, 'timing', fp.timing, flush=True)
     assert np.all(fp.info == 0)
     # unit-flux patches
+    worst = 0.
     buf = fp.patches.cpu().numpy()
     inv = np.empty(n, np.int64)
     inv[fp.order] = np.arange(n)
@@ -56,7 +57,9 @@ def test_forced_photometry_matches_oracle():
         rx0, ry0, p = patches[s]
         assert (x0, y0, ww, hh) == (rx0, ry0, p.shape[1], p.shape[0])
         g = buf[fp.poff[i]:fp.poff[i] + ww * hh].reshape(hh, ww)
-        assert np.max(np.abs(g - p)) <= 2e-5 * np.max(np.abs(p)) + 1e-12, s
+        worst = max(worst, np.max(np.abs(g - p)) / np.max(np.abs(p)))
+        assert np.max(np.abs(g - p)) <= 1e-5 * np.max(np.abs(p)) + 1e-12, s
+    print('  unit patches max err / peak %.3e' % worst, flush=True)
     np.testing.assert_allclose(fp.iv, iv_ref, rtol=1e-4, atol=1e-12)
     sig = 1. / np.sqrt(np.maximum(iv_ref, 1e-300))
     err = np.abs(got - ref)
