@@ -1,8 +1,9 @@
-"""Regenerates sersic_table.py from the reference's tractor/sersic.py: the
-`fits` list of SersicMixture.__init__ (sersic.py:40-221) and `self.cores`
-(223-246) are read as literals from the parsed source (nothing is run) and
-written out as data.  Usage:
-    python -m tractor_amd.gen_sersic_table [/root/reference/tractor/sersic.py]
+"""Regenerates tractor_amd/sersic_table.py from the reference's
+tractor/sersic.py: the `fits` list of SersicMixture.__init__ (sersic.py:
+40-221) and `self.cores` (223-246) are read as literals from the parsed
+source (nothing is run) and written out as data.  A data recipe, kept with
+the tests that check the table.  Usage:
+    python tests/golden/gen_sersic_table.py [/root/reference/tractor/sersic.py]
 """
 import ast
 import os
@@ -48,7 +49,8 @@ def render(fits, cores):
 
 if __name__ == '__main__':
     src = sys.argv[1] if len(sys.argv) > 1 else '/root/reference/tractor/sersic.py'
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'sersic_table.py')
+    root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    dst = os.path.join(root, 'tractor_amd', 'sersic_table.py')
     text = render(*read_tables(src))
     with open(dst, 'w') as f:
         f.write(text)

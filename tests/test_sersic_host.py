@@ -153,13 +153,17 @@ def test_oracle_sersic_restatement():
 
 
 def test_table_matches_reference_source():
-    """sersic_table.py is what gen_sersic_table writes from the reference's
-    tractor/sersic.py (parsed as text; skipped where the reference tree is
-    not present)."""
+    """sersic_table.py is what tests/golden/gen_sersic_table.py writes from
+    the reference's tractor/sersic.py (parsed as text; skipped where the
+    reference tree is not present)."""
     import os
     src = '/root/reference/tractor/sersic.py'
     if not os.path.exists(src):
         pytest.skip('reference source not present')
-    from tractor_amd import gen_sersic_table as G
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        'gen_sersic_table', os.path.join(os.path.dirname(__file__), 'golden', 'gen_sersic_table.py'))
+    G = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(G)
     import tractor_amd.sersic_table as ST
     assert G.render(*G.read_tables(src)) == open(ST.__file__).read()
