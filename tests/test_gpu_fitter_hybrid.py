@@ -181,3 +181,51 @@ def test_full_fit_matches_oracle(hyb):
                         prob.kind, prob.P, np.round(err / sig, 4),
                         abs(st['chisq'][s] - chi) / chi, st['iterations'][s], nref)))
     FP.check(res, say)
+
+
+def test_em_fitted_hybrid_psf_end_to_end():
+    """HybridPixelizedPSF(gauss=None) fits its mixture to the stamp on the
+    device (EM, GaussianMixturePSF.fromStamp); that mixture drives the
+    batched fitter's split.  The fitter's renders equal the object API's
+    getModelPatch with the same HybridPixelizedPSF on the same modelMask
+    (two separate code paths: the fitter's k_fit_mog vs the object API's
+    k_render_mog) and the oracle's hybrid render, to 1e-5 of peak."""
+    import tractor_amd as T
+    say('EM-fitted hybrid PSF, fitter vs object API vs oracle')
+    psf_img = W.moffat_stamp(63, 4.0, 3.5)
+    hpsf = T.HybridPixelizedPSF(T.PixelizedPSF(psf_img), N=2)
+    mog = hpsf.getMixtureOfGaussians()
+    hm = (np.asarray(mog.amp, np.float64), np.asarray(mog.mean, np.float64),
+          np.asarray(mog.var, np.float64))
+    kind = np.array([1, 2, 1, 2], np.int32)
+    P = np.array([24, 32, 40, 56], np.int32)
+    params = np.array([[12.3, 11.8, 800., np.log(5.), 0.2, -0.1],
+                       [16.6, 15.9, 1500., np.log(7.), -0.1, 0.3],
+                       [20.2, 19.7, 600., np.log(3.), 0.0, 0.2],
+                       [28.4, 27.6, 2000., np.log(8.), 0.3, 0.1]])
+    pix_off, total = W.patch_offsets(P)
+    data = torch.zeros(total, dtype=torch.float32, device='cuda')
+    inverr = torch.ones_like(data)
+    f = B.BatchFitter(kind, P, pix_off, data, inverr, psf_img)
+    f.set_hybrid(*hm)
+    f.set_params(params)
+    out = torch.zeros_like(data)
+    f.render_models(out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    for s in range(len(kind)):
+        Pz, o = int(P[s]), int(pix_off[s])
+        p = params[s]
+        g = got[o:o + Pz * Pz].reshape(Pz, Pz)
+        tim = T.Image(data=np.zeros((Pz, Pz), np.float32), inverr=np.ones((Pz, Pz), np.float32),
+                      psf=hpsf, wcs=T.PixelWCS())
+        cls = T.ExpGalaxy if kind[s] == 1 else T.DevGalaxy
+        gal = cls(T.PixPos(p[0], p[1]), T.Flux(p[2]), T.EllipseESoft(p[3], p[4], p[5]))
+        api = gal.getModelPatch(tim, modelMask=T.ModelMask(0, 0, Pz, Pz)).patch
+        prob = O.SourceProblem(int(kind[s]), p, Pz, psf_img, hybrid_mog=hm)
+        ref = O.source_render(prob, p) * np.float32(p[2])
+        e_api = np.max(np.abs(g - api)) / np.max(np.abs(ref))
+        e_ref = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+        say('  source %d: fitter vs object API %.2e, vs oracle %.2e' % (s, e_api, e_ref))
+        assert e_api <= 1e-5 and e_ref <= 1e-5, s
+    f.close()

@@ -1619,7 +1619,7 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
     int rc;
     if (!capture) (void)hipEventRecord(f->ev[0], st);
     SLOT_BEGIN(SL_PREP);
-    hipLaunchKernelGGL(k_fit_reset, dim3(1), dim3(64), 0, st, f->cnt, 16);
+    // (no counter reset: each k_fit_lists writes every bucket count it owns)
     const int nb_jac = (cap_jac + PREP_TB - 1) / PREP_TB, nb_ls = (cap_ls + PREP_TB - 1) / PREP_TB;
     hipLaunchKernelGGL(k_fit_prep_jac, dim3(nb_jac), dim3(PREP_TB), 0, st, S, f->nj, f->kind, f->P,
                        f->render_off, f->params, f->active, f->prof, f->ser, f->sstep, f->ph, f->pw, f->jobs_jac,
