@@ -1,0 +1,120 @@
+"""The batched fitter on windows inside one shared image (row_stride = the
+image width; the reference's modelMask windows of an image,
+tractor/engine.py:341-370, galaxy.py:337-351) against the same windows
+copied out as back-to-back patches: every kernel reads and writes through
+pix_off + y * stride + x, so both layouts must give bit-identical fits.
+Also: accumulated model renders of overlapping windows sum, as
+Tractor.getModelImage adds patches (engine.py:424-445)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from tractor_amd import batch as B, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WIDTH = HEIGHT = 512
+
+
+def _scene():
+    rng = np.random.default_rng(W.seed(4, 40))
+    S = 20
+    kind = rng.choice([0, 1, 2], S, p=[0.3, 0.4, 0.3]).astype(np.int32)
+    P = rng.choice([24, 40, 56, 72, 96], S).astype(np.int32)
+    # windows on a 5 x 4 grid of 100 px cells: no two overlap
+    x0 = (np.arange(S) % 5) * 100 + 2
+    y0 = (np.arange(S) // 5) * 100 + 3
+    truth = np.zeros((S, 6))
+    truth[:, 0] = P / 2. + rng.uniform(-1, 1, S)
+    truth[:, 1] = P / 2. + rng.uniform(-1, 1, S)
+    truth[:, 2] = np.exp(rng.uniform(np.log(300.), np.log(1e4), S))
+    for s in range(S):
+        if kind[s]:
+            re = np.exp(rng.uniform(np.log(1.), np.log(6.)))
+            e = rng.uniform(0, 0.5)
+            th = rng.uniform(0, np.pi)
+            truth[s, 3:] = O.esoft_from_e(re, e * np.cos(2 * th), e * np.sin(2 * th))
+    return kind, P, x0, y0, truth
+
+
+def test_strided_windows_fit_like_packed_patches():
+    assert torch.cuda.is_available()
+    print('strided windows vs packed patches', flush=True)
+    kind, P, x0, y0, truth = _scene()
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    g = torch.Generator(device='cuda')
+    g.manual_seed(W.seed(4, 41))
+    tile = torch.randn(WIDTH * HEIGHT, generator=g, device='cuda')
+    inv_tile = torch.ones_like(tile)
+    off = (y0 * WIDTH + x0).astype(np.int64)
+    fs = B.BatchFitter(kind, P, off, tile, inv_tile, psf, row_stride=WIDTH)
+    fs.set_params(truth)
+    fs.render_models(tile, accumulate=True)      # data = noise + models
+    torch.cuda.synchronize()
+    # the same windows, packed back to back
+    pk_off, total = W.patch_offsets(P)
+    th = tile.cpu().numpy().reshape(HEIGHT, WIDTH)
+    packed = np.zeros(total, np.float32)
+    for s in range(len(P)):
+        packed[pk_off[s]:pk_off[s] + P[s] * P[s]] = th[y0[s]:y0[s] + P[s], x0[s]:x0[s] + P[s]].ravel()
+    d_packed = torch.from_numpy(packed).cuda()
+    fp = B.BatchFitter(kind, P, pk_off, d_packed, torch.ones_like(d_packed), psf)
+    start = W.perturb(kind, truth, np.random.default_rng(W.seed(4, 42)))
+    res = []
+    for f in (fs, fp):
+        f.set_params(start)
+        f.fit(max_iter=100)
+        res.append((f.get_params(), f.status()))
+    (pa, sa), (pb, sb) = res
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(sa['chisq'], sb['chisq'])
+    np.testing.assert_array_equal(sa['iterations'], sb['iterations'])
+    print('  iterations', sa['iterations'].tolist(), flush=True)
+    # and the strided fit against the oracle on two windows
+    for s in (0, 7):
+        Pz = int(P[s])
+        prob = O.SourceProblem(int(kind[s]), start[s], Pz, psf,
+                               data=th[y0[s]:y0[s] + Pz, x0[s]:x0[s] + Pz].copy(),
+                               inverr=np.ones((Pz, Pz), np.float32), cdi=B.PIXEL_CDI)
+        ref, nref, chi = O.fit_source(prob, start[s])
+        assert abs(sa['chisq'][s] - chi) <= 1e-4 * chi, (s, sa['chisq'][s], chi)
+    fs.close()
+    fp.close()
+
+
+def test_accumulated_models_of_overlapping_windows_sum():
+    print('overlapping windows accumulate', flush=True)
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    kind = np.array([1, 2, 0], np.int32)
+    P = np.array([64, 48, 32], np.int32)
+    # three windows that overlap each other
+    x0 = np.array([10, 40, 50])
+    y0 = np.array([20, 30, 45])
+    truth = np.array([[32.3, 31.6, 500., np.log(3.), 0.1, 0.2],
+                      [24.1, 23.7, 800., np.log(2.), -0.2, 0.1],
+                      [16.4, 15.8, 300., 0., 0., 0.]])
+    out = torch.zeros(128 * 128, device='cuda')
+    d = torch.zeros_like(out)
+    off = (y0 * 128 + x0).astype(np.int64)
+    f = B.BatchFitter(kind, P, off, d, torch.ones_like(d), psf, row_stride=128)
+    f.set_params(truth)
+    f.render_models(out, accumulate=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(128, 128)
+    ref = np.zeros((128, 128), np.float64)
+    for s in range(3):
+        Pz = int(P[s])
+        prob = O.SourceProblem(int(kind[s]), truth[s], Pz, psf, cdi=B.PIXEL_CDI)
+        m = O.source_render(prob, truth[s]) * np.float32(truth[s, 2])
+        ref[y0[s]:y0[s] + Pz, x0[s]:x0[s] + Pz] += m
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    # stored (not accumulated) renders overwrite: the last window wins where
+    # windows overlap, so check a pixel covered by the first window only
+    out2 = torch.full_like(out, 7.)
+    f.render_models(out2, accumulate=False)
+    torch.cuda.synchronize()
+    g2 = out2.cpu().numpy().reshape(128, 128)
+    assert g2[0, 0] == 7.
+    assert abs(g2[25, 15] - ref[25, 15]) <= 1e-5 * np.max(np.abs(ref))
+    f.close()
