@@ -28,7 +28,9 @@ def solve_least_squares(cols, b, nrows, ncols, damp=0.):
     A = np.zeros((ncols + 1, len(rows)), np.float32)
     for c, (r, v) in enumerate(cols):
         if len(r):
-            np.add.at(A[c], np.searchsorted(rows, r), v)
+            # entries of one column may repeat a row (several patches): sum them
+            A[c] += np.bincount(np.searchsorted(rows, r), weights=v,
+                                minlength=len(rows)).astype(np.float32)
     A[ncols] = b[rows]
     dev = 'cuda'
     dA = torch.from_numpy(A.ravel()).to(dev)
