@@ -129,15 +129,15 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline_cpp(wl, budget_s=10.0, dchisq=0.):
+def cpu_baseline_cpp(wl, budget_s=10.0, dchisq=0., nthreads=None):
     """The oracle's C++ FP64 restatement (oracle/fit_fp64.cpp) fitting the
     workload's sources in order from the same data and start, threaded over
-    sources on cpu_threads() host threads; no source is started after
-    budget_s seconds.  -> (n_done, seconds, params, iterations, chisq,
-    threads)."""
+    sources on nthreads (default cpu_threads()) host threads; no source is
+    started after budget_s seconds.  -> (n_done, seconds, params,
+    iterations, chisq, threads)."""
     from oracle import fit_fp64 as F
     from tractor_amd import batch as B
-    nt = cpu_threads()
+    nt = cpu_threads() if nthreads is None else nthreads
     data = wl['data'].cpu().numpy()
     inverr = wl['inverr'].cpu().numpy()
     t0 = time.time()
@@ -151,17 +151,22 @@ def cpu_baseline_cpp(wl, budget_s=10.0, dchisq=0.):
 
 def cpu_baseline_record(wl, args, gpu_chisq, gpu_iters):
     """cpu_baseline of the bench line (the C++ FP64 port on the host cores,
-    the NumPy oracle on one core beside it) and the parity of the GPU fit's
+    the same port on one thread and the NumPy oracle on one core beside it)
+    and the parity of the GPU fit's
     chi^2 against both samples."""
     from oracle import fit_fp64 as F
     unit = 'sources/s' if wl['cfg'] == 4 else 'source-iterations/s'
     m, cdt, cp, cit, cchi, nt = cpu_baseline_cpp(wl, args.cpu_seconds, dchisq=args.dchisq)
+    m1, cdt1 = cpu_baseline_cpp(wl, args.cpu_seconds / 4, dchisq=args.dchisq, nthreads=1)[:2]
     n, ndt, res = cpu_baseline(wl, args.cpu_seconds, dchisq=args.dchisq)
     cpu = dict(value=round(m / cdt, 4), unit=unit, cores=nt, kind='port',
                sample='first %d sources of the same workload (same data and start), C++ FP64 '
                       'restatement of the oracle (oracle/fit_fp64.cpp), %d threads over '
                       'sources, %.1f s' % (m, nt, cdt),
                cpu_model=F.cpu_model(), os_cpu_count=os.cpu_count(),
+               single_thread=dict(value=round(m1 / cdt1, 4), unit=unit, cores=1,
+                                  sample='first %d sources, the same C++ port on one thread, '
+                                         '%.1f s' % (m1, cdt1)),
                numpy_port=dict(value=round(n / ndt, 4), unit=unit, cores=1,
                                sample='first %d sources, NumPy FP64 oracle, one process' % n))
     parity = None
