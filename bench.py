@@ -534,7 +534,7 @@ def main_c5(args, torch):
     parity = None
     if not args.no_cpu_baseline:
         # the CPU fits start from the standard start and type: the surrogate
-        # pipeline's final chi^2 is judged against their converged chi^2
+        # pipeline's final chi^2 is compared with their converged chi^2
         cpu, parity = cpu_baseline_record(wl, args, res['chisq'], res['iterations'])
 
     out = {

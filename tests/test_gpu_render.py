@@ -1,6 +1,6 @@
 """GPU parity: libtractor_hip render kernels vs the FP64 oracle.
 
-Tolerance (the north-star bar for FP32 renders): max |gpu - oracle| /
+Tolerance (the parity bar for FP32 renders): max |gpu - oracle| /
 max |oracle| <= 1e-5 per patch.
 """
 import math
