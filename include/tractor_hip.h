@@ -564,6 +564,29 @@ int trh_fitter_set_hybrid(trh_fitter *f, int K, const double *amp, const double 
                           const double *var, double inner_nsigma, double outer_nsigma);
 
 /*
+ * Chained restarts (no counterpart in the reference's optimizers: the
+ * fallback of a surrogate-initialised fit, "re-fit from the standard start
+ * when the proposal converges worse", run inside the same batch so that the
+ * re-fits overlap the rest of the fit).  Host arrays of n pairs: source
+ * second[i] waits, stopped, until source first[i] has stopped; it then
+ * starts from the parameters set for it (with its own iteration count) if
+ * first[i]'s chi^2 is above ref_chisq[i] * (1 + rel_tol) or not a number,
+ * and otherwise never runs.  A source may be in one pair only, in one role.
+ * The check runs on the device after every iteration (one launch, part of
+ * the captured graph).  trh_fitter_set_params puts every pair back to
+ * pending; n = 0 removes them.  Ordered after the fitter's graph replays on
+ * `stream`, and synchronises it.
+ */
+int trh_fitter_set_chain(trh_fitter *f, int n, const int32_t *first, const int32_t *second,
+                         const double *ref_chisq, double rel_tol, void *stream);
+
+/*
+ * Per pair of trh_fitter_set_chain (host int32 [n]; synchronises `stream`):
+ * 0 first source still running, 1 second source started, 2 not needed.
+ */
+int trh_fitter_chain_state(trh_fitter *f, int32_t *state, void *stream);
+
+/*
  * Render every source's model (counts x unit-flux patch at the current
  * parameters, as getModelPatch, tractor/pointsource.py:36-60) into `out`,
  * laid out like `data` (window s at pix_off[s], row stride row_stride[s]):

@@ -149,3 +149,17 @@ def test_training_learns_and_helps_the_fit():
         int(res['fallback'].sum()), rel.max(), res['timings']), flush=True)
     assert np.all(rel <= 1e-4)
     assert res['fallback'].sum() == res['timings']['fallback_sources'] < 0.2 * n
+    # the re-fits as a separate batch after the fit (the default) and as
+    # chained restarts in the same batch give the same results: every
+    # source's fit is independent of the others in its batch
+    assert not res['timings']['fallback_overlapped']
+    ovl = S.fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kind=kind,
+                              plain_chisq=plain['chisq'], overlap=True)
+    assert ovl['timings']['fallback_overlapped']
+    print('sequential fallback %.1f ms, overlapped %.1f ms' % (
+        1e3 * res['timings']['total_s'], 1e3 * ovl['timings']['total_s']), flush=True)
+    np.testing.assert_array_equal(ovl['fallback'], res['fallback'])
+    np.testing.assert_array_equal(ovl['chisq'], res['chisq'])
+    np.testing.assert_array_equal(ovl['params'], res['params'])
+    np.testing.assert_array_equal(ovl['kind'], res['kind'])
+    np.testing.assert_array_equal(ovl['iterations'], res['iterations'])

@@ -100,6 +100,8 @@ SIGNATURES = {
     'trh_fitter_get_params': (_I, [_P, _P, _P]),
     'trh_fitter_set_sersic': (_I, [_P, _P, _I]),
     'trh_fitter_set_hybrid': (_I, [_P, _I, _P, _P, _P, _D, _D]),
+    'trh_fitter_set_chain': (_I, [_P, _I, _P, _P, _P, _D, _P]),
+    'trh_fitter_chain_state': (_I, [_P, _P, _P]),
     'trh_fitter_iterate': (_I, [_P, _D, _D, _I, _P]),
     'trh_fitter_iterate_graph': (_I, [_P, _D, _D, _I, _I, _P]),
     'trh_fitter_render_models': (_I, [_P, _P, _I, _P]),

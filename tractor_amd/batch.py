@@ -60,6 +60,7 @@ class BatchFitter(object):
                                          ptr(psf), psf.shape[0], psf.shape[1], ptr(cdi)),
               'trh_fitter_create')
         self.f = f
+        self.n_chain = 0
         self.npar = 7 if np.any(self.kind == 3) else 6
         if self.npar == 7:
             from .sersic import fitter_table
@@ -84,6 +85,26 @@ class BatchFitter(object):
         check(self.lib.trh_fitter_set_hybrid(self.f, K, ptr(amp), ptr(mean), ptr(v3),
                                              float(inner_nsigma), float(outer_nsigma)),
               'trh_fitter_set_hybrid')
+
+    def set_chain(self, first, second, ref_chisq, rel_tol=1e-4):
+        """Chained restarts (trh_fitter_set_chain): source second[i] starts,
+        from the parameters set for it, once source first[i] has stopped with
+        chi^2 above ref_chisq[i] (1 + rel_tol); until then, and if it is not
+        needed, it stays stopped.  Takes effect from the next set_params."""
+        first = np.ascontiguousarray(first, np.int32)
+        second = np.ascontiguousarray(second, np.int32)
+        ref = np.ascontiguousarray(ref_chisq, np.float64)
+        if not (len(first) == len(second) == len(ref)):
+            raise ValueError('set_chain: first, second and ref_chisq differ in length')
+        check(self.lib.trh_fitter_set_chain(self.f, len(first), ptr(first), ptr(second), ptr(ref),
+                                            float(rel_tol), self.stream), 'set_chain')
+        self.n_chain = len(first)
+
+    def chain_state(self):
+        """Per pair: 0 first source running, 1 second started, 2 not needed."""
+        st = np.zeros(max(self.n_chain, 1), np.int32)
+        check(self.lib.trh_fitter_chain_state(self.f, ptr(st), self.stream), 'chain_state')
+        return st[:self.n_chain]
 
     def close(self):
         if self.f:
@@ -170,9 +191,12 @@ class BatchFitter(object):
             check_every = 4 if graph else 1
         if max_iter < 1:
             raise ValueError('max_iter must be >= 1')
+        # a chained second source can start as late as its first's last
+        # iteration and then runs max_iter of its own
+        total = 2 * max_iter if self.n_chain else max_iter
         it = 0
-        while it < max_iter:
-            k = min(check_every, max_iter - it)
+        while it < total:
+            k = min(check_every, total - it)
             if graph:
                 self.iterate_graph(k, lam, dchisq, max_iter)
             else:

@@ -295,6 +295,13 @@ struct trh_fitter {
     // P x P windows at ls_base + boff[s] + a P^2
     FitHybrid *hyb = nullptr;       // null: pure pixelized PSF
     FitHybrid *hyb_dev = nullptr;   // its device copy, kept while hybrid is off
+    // chained restarts (trh_fitter_set_chain): pair i starts source
+    // ch_second[i] once source ch_first[i] has stopped with chi^2 above
+    // ch_ref[i] (1 + ch_tol); ch_state[i]: 0 pending, 1 started, 2 not needed
+    int n_chain = 0;
+    int32_t *ch_first = nullptr, *ch_second = nullptr, *ch_state = nullptr;
+    double *ch_ref = nullptr;
+    double ch_tol = 0.0;
     FitMogJob *mog_jac = nullptr, *mog_ls = nullptr;
     float *ls_base = nullptr;
     int64_t *boff = nullptr;
@@ -1247,6 +1254,36 @@ __global__ void k_fit_accept(int S, const double *__restrict__ neq,
     if (rule || it >= max_iter) active[s] = 0;
 }
 
+// Chained restarts, after k_fit_accept: a pending pair whose first source
+// has stopped starts its second one (from the parameters set for it) when
+// the first's chi^2 is above the reference, NaN included; otherwise the
+// pair is settled without it.
+__global__ void k_fit_chain(int n, const int32_t *__restrict__ first,
+                            const int32_t *__restrict__ second, const double *__restrict__ ref,
+                            double tol, int32_t *__restrict__ state, int32_t *__restrict__ active,
+                            int32_t *__restrict__ iters, int32_t *__restrict__ flags,
+                            const double *__restrict__ chisq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || state[i] != 0 || active[first[i]]) return;
+    const bool go = !(chisq[first[i]] <= ref[i] * (1.0 + tol));
+    state[i] = go ? 1 : 2;
+    if (go) {
+        const int b = second[i];
+        active[b] = 1;
+        iters[b] = 0;
+        flags[b] = 0;
+    }
+}
+
+// Every pair pending again and its second source stopped until then.
+__global__ void k_fit_chain_reset(int n, const int32_t *__restrict__ second,
+                                  int32_t *__restrict__ state, int32_t *__restrict__ active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    state[i] = 0;
+    active[second[i]] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1482,6 +1519,12 @@ extern "C" int trh_fitter_set_params(trh_fitter *f, const double *params, void *
         (rc = hip_check(hipMemcpyAsync(f->flags, zeros.data(), 4 * (size_t)f->S,
                                        hipMemcpyHostToDevice, st), "reset flags")))
         return rc;
+    if (f->n_chain > 0) {
+        // the second source of every chained pair waits for its first
+        hipLaunchKernelGGL(k_fit_chain_reset, dim3((f->n_chain + 255) / 256), dim3(256), 0, st,
+                           f->n_chain, f->ch_second, f->ch_state, f->active);
+        if ((rc = hip_check(hipGetLastError(), "k_fit_chain_reset"))) return rc;
+    }
     return hip_check(hipStreamSynchronize(st), "set params sync");
 }
 
@@ -1706,6 +1749,12 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
                        f->steplim, f->chi2ls, f->params, f->active, f->iters, f->chisq, f->dlnp,
                        f->flags, dchisq, max_iter);
     if ((rc = hip_check(hipGetLastError(), "k_fit_accept"))) return rc;
+    if (f->n_chain > 0) {
+        hipLaunchKernelGGL(k_fit_chain, dim3((f->n_chain + 255) / 256), dim3(256), 0, st,
+                           f->n_chain, f->ch_first, f->ch_second, f->ch_ref, f->ch_tol,
+                           f->ch_state, f->active, f->iters, f->flags, f->chisq);
+        if ((rc = hip_check(hipGetLastError(), "k_fit_chain"))) return rc;
+    }
     if (!capture) (void)hipEventRecord(f->ev[5], st);
     if (prof) {
         // accumulate this iteration's per-slot times (synchronises the stream)
@@ -1824,6 +1873,65 @@ extern "C" int trh_fitter_profile_read(trh_fitter *f, double *out) {
         out[4 * k + 3] = (double)a[2 * k + 1] + f->tot_extra_bytes[k];
     }
     return TRH_OK;
+}
+
+extern "C" int trh_fitter_set_chain(trh_fitter *f, int n, const int32_t *first,
+                                    const int32_t *second, const double *ref_chisq,
+                                    double rel_tol, void *stream) {
+    if (!f || n < 0 || (n > 0 && (!first || !second || !ref_chisq)) || !(rel_tol >= 0.0))
+        return set_err(TRH_E_ARG, "trh_fitter_set_chain: bad arguments");
+    const int S = f->S;
+    if (n > S / 2)
+        return set_err(TRH_E_ARG, "trh_fitter_set_chain: %d pairs for %d sources", n, S);
+    // each source in at most one pair, and in one role
+    std::vector<int8_t> used(S, 0);
+    for (int i = 0; i < n; ++i) {
+        const int a = first[i], b = second[i];
+        if (a < 0 || a >= S || b < 0 || b >= S || a == b || used[a] || used[b])
+            return set_err(TRH_E_ARG, "trh_fitter_set_chain: pair %d (%d, %d) out of range or "
+                                      "reusing a source", i, a, b);
+        used[a] = used[b] = 1;
+    }
+    int rc;
+    if ((rc = hip_check(hipSetDevice(f->h->device), "hipSetDevice"))) return rc;
+    // the captured iteration gains or loses the chain launch: capture again
+    if (f->gexec) {
+        (void)hipGraphExecDestroy(f->gexec);
+        f->gexec = nullptr;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (f->have_gev && (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "order after replays")))
+        return rc;
+    if (n > 0 && !f->ch_first) {
+        const size_t cap = (size_t)S / 2;
+        if ((rc = falloc(f, &f->ch_first, cap)) || (rc = falloc(f, &f->ch_second, cap)) ||
+            (rc = falloc(f, &f->ch_state, cap)) || (rc = falloc(f, &f->ch_ref, cap)))
+            return rc;
+    }
+    f->n_chain = 0;
+    if (n == 0) return TRH_OK;
+    if ((rc = hip_check(hipMemcpyAsync(f->ch_first, first, 4 * (size_t)n, hipMemcpyHostToDevice, st),
+                        "chain firsts")) ||
+        (rc = hip_check(hipMemcpyAsync(f->ch_second, second, 4 * (size_t)n, hipMemcpyHostToDevice,
+                                       st), "chain seconds")) ||
+        (rc = hip_check(hipMemcpyAsync(f->ch_ref, ref_chisq, 8 * (size_t)n, hipMemcpyHostToDevice,
+                                       st), "chain references")))
+        return rc;
+    f->n_chain = n;
+    f->ch_tol = rel_tol;
+    hipLaunchKernelGGL(k_fit_chain_reset, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       f->ch_second, f->ch_state, f->active);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_chain_reset"))) return rc;
+    return hip_check(hipStreamSynchronize(st), "set chain sync");
+}
+
+extern "C" int trh_fitter_chain_state(trh_fitter *f, int32_t *state, void *stream) {
+    if (!f || !state) return set_err(TRH_E_ARG, "trh_fitter_chain_state: null argument");
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "sync"))) return rc;
+    if (f->n_chain == 0) return TRH_OK;
+    return hip_check(hipMemcpy(state, f->ch_state, 4 * (size_t)f->n_chain, hipMemcpyDeviceToHost),
+                     "read chain state");
 }
 
 extern "C" int trh_fitter_set_hybrid(trh_fitter *f, int K, const double *amp, const double *mean,

@@ -310,7 +310,7 @@ def selfcheck_bad(chisq, P, nparams, nsig=5.):
 
 def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kind=None,
                       max_iter=100, dchisq=0., cdi=Bt.PIXEL_CDI, type_conf=0.0, timing=None,
-                      profile=None, plain_chisq=None, rel_tol=1e-4):
+                      profile=None, plain_chisq=None, rel_tol=1e-4, overlap=False):
     """Surrogate-initialised batched fit.
 
     1. cutouts -> CNN -> proposed type and start;
@@ -319,9 +319,19 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
        likely type too, in the same batch, and keeps the lower chi^2;
     3. fallback: with `plain_chisq` (the final chi^2 of the plain fit of the
        same sources), every source whose chi^2 exceeds it by more than
-       `rel_tol` relative is re-fit from the standard start (and, if given,
-       the standard type); without it, the sources failing the chi-square
-       self-check are.  The lower chi^2 is kept.
+       `rel_tol` relative (or is not a number) is re-fit from the standard
+       start (and, if given, the standard type); without it, the sources
+       failing the chi-square self-check are.  The lower chi^2 is kept.
+       With `plain_chisq`, `overlap` and no second type hypotheses, the
+       re-fits run in the same batch as chained restarts
+       (BatchFitter.set_chain): each source's re-fit starts on the device as
+       soon as its proposal's fit has stopped worse than the plain fit, so
+       the re-fits overlap the rest of the fit instead of following it; the
+       results are those of the separate re-fit (every source's fit is
+       independent of the others in its batch).  That pays when the sources
+       that need a re-fit stop early; a proposal that runs to max_iter and
+       ends worse starts its re-fit only then, which stretches the batch to
+       up to 2 max_iter iterations, so the separate re-fit is the default.
     Returns dict(kind, params, chisq, iterations, fallback mask, timings)."""
     t = {}
     n = len(P)
@@ -348,9 +358,19 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
     kind_all = np.concatenate([kind, k2]).astype(np.int32)
     start_all = np.concatenate([start, s2])
     src = np.concatenate([np.arange(n), amb])
+    chained = overlap and plain_chisq is not None and not len(amb)
+    if chained:
+        # sources n..2n-1: the standard start and type, each chained to its
+        # proposal's fit
+        pk = np.asarray(plain_kind if plain_kind is not None else kind).astype(np.int32)
+        kind_all = np.concatenate([kind_all, pk]).astype(np.int32)
+        start_all = np.concatenate([start_all, np.asarray(plain_start, np.float64)])
+        src = np.concatenate([src, np.arange(n)])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     f = Bt.BatchFitter(kind_all, P[src], pix_off[src], data, inverr, psf, cdi=cdi)
+    if chained:
+        f.set_chain(np.arange(n), n + np.arange(n), plain_chisq, rel_tol)
     f.set_params(start_all)
     if profile is not None:
         f.profile(True)
@@ -360,6 +380,7 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
         f.profile(False)
     st = f.status()
     pall = f.get_params()
+    cstate = f.chain_state() if chained else None
     f.close()
     chisq = st['chisq'][:n].copy()
     iters = st['iterations'][:n].copy()
@@ -375,12 +396,23 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
         iters[amb] = np.maximum(iters[amb], st['iterations'][n:])
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    if plain_chisq is not None:
-        bad = chisq > np.asarray(plain_chisq, np.float64) * (1. + rel_tol)
+    if chained:
+        # the restarts that ran: keep the lower chi^2 of the pair
+        bad = cstate == 1
+        idx = np.flatnonzero(bad)
+        c2 = st['chisq'][n:][idx]
+        better = c2 < chisq[idx]
+        sel = idx[better]
+        chisq[sel] = c2[better]
+        params[sel] = pall[n:][idx][better]
+        fk[sel] = pk[idx][better]
+        iters[idx] += st['iterations'][n:][idx]
+    elif plain_chisq is not None:
+        bad = ~(chisq <= np.asarray(plain_chisq, np.float64) * (1. + rel_tol))
     else:
         bad = selfcheck_bad(chisq, P, np.where(fk == 0, 3, 6))
     nb = int(bad.sum())
-    if nb:
+    if nb and not chained:
         idx = np.flatnonzero(bad)
         kb = (np.asarray(plain_kind)[idx] if plain_kind is not None else fk[idx]).astype(np.int32)
         fb = Bt.BatchFitter(kb, P[idx], pix_off[idx], data, inverr, psf, cdi=cdi)
@@ -398,7 +430,7 @@ def fit_with_fallback(net, data, inverr, pix_off, P, psf, plain_start, plain_kin
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     t.update(cnn_s=t1 - t0, fit_s=t2 - t1, fallback_s=t3 - t2, total_s=t3 - t0, lm_iterations=it,
-             second_hypotheses=int(len(amb)), fallback_sources=nb)
+             second_hypotheses=int(len(amb)), fallback_sources=nb, fallback_overlapped=chained)
     if timing is not None:
         timing.update(t)
     return dict(kind=fk, params=params, chisq=chisq, iterations=iters, fallback=bad,
