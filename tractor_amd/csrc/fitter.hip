@@ -1287,6 +1287,15 @@ __global__ void k_fit_chain_reset(int n, const int32_t *__restrict__ second,
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Drop the captured iteration (its launches change, or the fitter goes):
+// after the replays already issued on the fitter's stream.
+static void drop_graph(trh_fitter *f) {
+    if (!f->gexec) return;
+    if (f->gstream) (void)hipStreamSynchronize(f->gstream);
+    (void)hipGraphExecDestroy(f->gexec);
+    f->gexec = nullptr;
+}
+
 template <typename T>
 static int falloc(trh_fitter *f, T **p, size_t n) {
     void *q = nullptr;
@@ -1337,13 +1346,15 @@ static void host_profiles(FitProfiles &pr) {
 extern "C" int trh_fitter_destroy(trh_fitter *f) {
     if (!f) return TRH_OK;
     (void)hipSetDevice(f->h->device);
+    // replays still in flight read the buffers: let them finish first
+    drop_graph(f);
+    if (f->gstream) (void)hipStreamSynchronize(f->gstream);
     for (void *p : f->allocs) (void)hipFree(p);
     if (f->have_ev) {
         for (int i = 0; i < 6; ++i) (void)hipEventDestroy(f->ev[i]);
         for (int i = 0; i < TRH_NSLOT; ++i)
             for (int j = 0; j < 2; ++j) (void)hipEventDestroy(f->pev[i][j]);
     }
-    if (f->gexec) (void)hipGraphExecDestroy(f->gexec);
     if (f->have_gev)
         for (int i = 0; i < 2; ++i) (void)hipEventDestroy(f->gev[i]);
     if (f->gstream) (void)hipStreamDestroy(f->gstream);
@@ -1810,10 +1821,7 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
         f->have_gev = true;
     }
     if (!f->gexec || lambda != f->g_lambda || dchisq != f->g_dchisq || max_iter != f->g_max_iter) {
-        if (f->gexec) {
-            (void)hipGraphExecDestroy(f->gexec);
-            f->gexec = nullptr;
-        }
+        drop_graph(f);
         // (the row slots of the SZ >= 256 renders are the fitter's own,
         // allocated at creation)
         if ((rc = hip_check(hipStreamBeginCapture(f->gstream, hipStreamCaptureModeThreadLocal),
@@ -1895,10 +1903,7 @@ extern "C" int trh_fitter_set_chain(trh_fitter *f, int n, const int32_t *first,
     int rc;
     if ((rc = hip_check(hipSetDevice(f->h->device), "hipSetDevice"))) return rc;
     // the captured iteration gains or loses the chain launch: capture again
-    if (f->gexec) {
-        (void)hipGraphExecDestroy(f->gexec);
-        f->gexec = nullptr;
-    }
+    drop_graph(f);
     hipStream_t st = (hipStream_t)stream;
     if (f->have_gev && (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "order after replays")))
         return rc;
@@ -1945,10 +1950,7 @@ extern "C" int trh_fitter_set_hybrid(trh_fitter *f, int K, const double *amp, co
     int rc;
     if ((rc = hip_check(hipSetDevice(f->h->device), "hipSetDevice"))) return rc;
     // the captured iteration's launches change: capture again on next use
-    if (f->gexec) {
-        (void)hipGraphExecDestroy(f->gexec);
-        f->gexec = nullptr;
-    }
+    drop_graph(f);
     if (K == 0) {
         f->hyb = nullptr;
         return TRH_OK;
