@@ -240,12 +240,87 @@ def sersic_leg(torch, n=4096, steps=3, n_check=8):
             'max_rel_chisq_vs_fp64_first_%d' % len(errs): float(max(errs))}
 
 
+def tiles_leg(torch, steps=3, cpu_seconds=2.0, n_check=2000):
+    """The config-4 sources in place on 64 independent 512 x 512 float32
+    tiles, 256 per tile: each source's P x P window at a random position in
+    its tile, the tiles holding noise plus every source's model (so a window
+    also holds its neighbours' light, which no source's fit models), fitted
+    in place (row stride 512) from the perturbed start; timed over `steps`
+    fits after one warm-up.  The C++ FP64 port fits the same windows (copied
+    out on the host) for about cpu_seconds, and the GPU fit's chi^2 is
+    checked against it."""
+    from oracle import fit_fp64 as F
+    from tractor_amd import batch as B
+    from tractor_amd import workloads as W
+    n, ntile, side = 16384, 64, 512
+    kind, P, truth = W.draw_sources(n, config=4)
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    rng = np.random.default_rng(W.seed(4, 50))
+    tile = np.arange(n) % ntile
+    x0 = rng.integers(0, side - P + 1)
+    y0 = rng.integers(0, side - P + 1)
+    off = (tile.astype(np.int64) * side * side + y0 * side + x0).astype(np.int64)
+    data = torch.zeros(ntile * side * side, dtype=torch.float32, device='cuda')
+    inverr = torch.ones_like(data)
+    f = B.BatchFitter(kind, P, off, data, inverr, psf, row_stride=side)
+    f.set_params(truth)
+    f.render_models(data, accumulate=True)
+    g = torch.Generator(device='cuda')
+    g.manual_seed(W.seed(4, 51))
+    data.add_(torch.randn(data.numel(), generator=g, device='cuda'))
+    start = W.perturb(kind, truth, np.random.default_rng(W.seed(4, 52)))
+    f.set_params(start)
+    f.fit(max_iter=100)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.set_params(start)
+        f.fit(max_iter=100)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = f.status()
+    f.close()
+    out = {'workload': '16384 sources drawn as config 4, 256 in place on each of 64 512x512 '
+                       'tiles (noise + every model; windows overlap), row stride 512',
+           'ms_per_fit': round(dt * 1e3, 3), 'sources_per_s': round(n / dt, 1),
+           'mean_iterations': round(float(st['iterations'].mean()), 2),
+           'at_iteration_cap': int(np.sum(st['iterations'] >= 100)),
+           'us_per_source_iteration': round(dt * 1e6 / max(1, int(st['iterations'].sum())), 4),
+           'note': 'every window also holds the light of the neighbours overlapping it, '
+                   'which no fit models: a harder fit than the headline layout (one source '
+                   'per window), so more iterations and more FP32/FP64 trajectories that part',
+           'parity': None}
+    if cpu_seconds <= 0:
+        return out
+    # the same windows, copied out for the CPU port
+    m = min(n, n_check)
+    dh = data.cpu().numpy().reshape(ntile, side, side)
+    poff, tot = W.patch_offsets(P[:m])
+    packed = np.zeros(tot, np.float32)
+    for s in range(m):
+        packed[poff[s]:poff[s] + P[s] * P[s]] = \
+            dh[tile[s], y0[s]:y0[s] + P[s], x0[s]:x0[s] + P[s]].ravel()
+    done, _, cit, cchi = F.fit_sources(kind[:m], P[:m], poff, packed, np.ones(tot, np.float32),
+                                       start[:m], psf, cdi=B.PIXEL_CDI, nthreads=cpu_threads(),
+                                       budget_s=cpu_seconds)
+    ok = cit[:done] > 0
+    e = np.abs(st['chisq'][:done][ok] - cchi[:done][ok]) / cchi[:done][ok]
+    capped = (st['iterations'][:done][ok] >= 100) | (cit[:done][ok] >= 100)
+    out['parity'] = {'sources': int(ok.sum()), 'reference': 'C++ FP64 port on the same windows',
+                     'frac_within_1e-4': float(np.mean(e <= 1e-4)),
+                     'max_rel_chisq': float(e.max()),
+                     'frac_within_1e-4_converged': float(np.mean(e[~capped] <= 1e-4))
+                     if np.any(~capped) else None}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', default='c4', choices=['c4', 'c1', 'c2', 'c3', 'c5', 'sersic'])
+    ap.add_argument('--workload', default='c4',
+                    choices=['c4', 'c1', 'c2', 'c3', 'c5', 'sersic', 'tiles'])
     ap.add_argument('--cnn-train', type=int, default=1 << 20,
                     help='c5: number of rendered training cutouts')
     ap.add_argument('--cnn-epochs', type=int, default=4)
@@ -261,6 +336,9 @@ def main():
                          'rocprofv3 so the trace holds the timed fit only); roofline is then null')
     ap.add_argument('--no-surrogate', action='store_true',
                     help='skip the surrogate-initialised fit (CNN) reported beside the config-4 '
+                         'line')
+    ap.add_argument('--no-tiles', action='store_true',
+                    help='skip the in-place fit on 512x512 tiles reported beside the config-4 '
                          'line')
     ap.add_argument('--no-autotune', action='store_true',
                     help='keep the default render launch shapes')
@@ -290,6 +368,16 @@ def main():
         return main_c1(args, torch)
     if args.workload == 'c3':
         return main_c3(args, torch)
+    if args.workload == 'tiles':
+        leg = tiles_leg(torch, steps=args.steps)
+        print(json.dumps({
+            'metric': 'in-place batched fit on 512x512 tiles (side measurement)',
+            'value': leg['sources_per_s'], 'unit': 'sources/s', 'n_gpus': 1, 'steps': args.steps,
+            'warmup': 1, 'ms_per_step': leg['ms_per_fit'], 'higher_is_better': True,
+            'scaling': 'weak', 'vs_baseline': None,
+            'dtype': 'f32 (renders, J^T W J) + f64 (Cholesky, chi2)', 'data': 'synthetic',
+            'config': {'workload': leg['workload']}, 'tiles_fit': leg}), flush=True)
+        return
     if args.workload == 'sersic':
         leg = sersic_leg(torch, steps=args.steps)
         print(json.dumps({
@@ -375,6 +463,9 @@ def main():
     if wl['cfg'] == 4 and not args.no_sersic:
         log('sersic leg')
         out['sersic_fit'] = sersic_leg(torch)
+    if wl['cfg'] == 4 and not args.no_tiles:
+        log('tiles leg')
+        out['tiles_fit'] = tiles_leg(torch, cpu_seconds=0.0 if args.no_cpu_baseline else 3.0)
     print(json.dumps(out), flush=True)
 
 
