@@ -118,3 +118,61 @@ def test_accumulated_models_of_overlapping_windows_sum():
     assert g2[0, 0] == 7.
     assert abs(g2[25, 15] - ref[25, 15]) <= 1e-5 * np.max(np.abs(ref))
     f.close()
+
+
+def test_overlapping_windows_in_place_fit_like_packed_copies():
+    """Windows that overlap inside one image, as on the 512x512 tiles of the
+    in-place benchmark: every source's window holds its neighbours' light
+    too.  Fitting them in place (shared read-only data, row stride = image
+    width) equals fitting copies of the same windows packed back to back,
+    bit for bit."""
+    print('overlapping windows in place vs packed copies', flush=True)
+    rng = np.random.default_rng(W.seed(4, 43))
+    S, side = 24, 160
+    kind = rng.choice([0, 1, 2], S, p=[0.3, 0.4, 0.3]).astype(np.int32)
+    P = rng.choice([24, 40, 56, 72], S).astype(np.int32)
+    x0 = rng.integers(0, side - P + 1)
+    y0 = rng.integers(0, side - P + 1)
+    truth = np.zeros((S, 6))
+    truth[:, 0] = P / 2. + rng.uniform(-1, 1, S)
+    truth[:, 1] = P / 2. + rng.uniform(-1, 1, S)
+    truth[:, 2] = np.exp(rng.uniform(np.log(300.), np.log(1e4), S))
+    for s in range(S):
+        if kind[s]:
+            e = rng.uniform(0, 0.5)
+            th = rng.uniform(0, np.pi)
+            truth[s, 3:] = O.esoft_from_e(np.exp(rng.uniform(0., np.log(6.))), e * np.cos(2 * th),
+                                          e * np.sin(2 * th))
+    psf = W.moffat_stamp(63, 4.0, 3.5)
+    off = (y0 * side + x0).astype(np.int64)
+    g = torch.Generator(device='cuda')
+    g.manual_seed(W.seed(4, 44))
+    img = torch.randn(side * side, generator=g, device='cuda')
+    fs = B.BatchFitter(kind, P, off, img, torch.ones_like(img), psf, row_stride=side)
+    fs.set_params(truth)
+    fs.render_models(img, accumulate=True)
+    torch.cuda.synchronize()
+    cover = np.zeros((side, side), np.int32)
+    for s in range(S):
+        cover[y0[s]:y0[s] + P[s], x0[s]:x0[s] + P[s]] += 1
+    assert cover.max() >= 3   # the windows do overlap
+    ih = img.cpu().numpy().reshape(side, side)
+    pk_off, total = W.patch_offsets(P)
+    packed = np.zeros(total, np.float32)
+    for s in range(S):
+        packed[pk_off[s]:pk_off[s] + P[s] * P[s]] = ih[y0[s]:y0[s] + P[s], x0[s]:x0[s] + P[s]].ravel()
+    d_packed = torch.from_numpy(packed).cuda()
+    fp = B.BatchFitter(kind, P, pk_off, d_packed, torch.ones_like(d_packed), psf)
+    start = W.perturb(kind, truth, np.random.default_rng(W.seed(4, 45)))
+    res = []
+    for f in (fs, fp):
+        f.set_params(start)
+        f.fit(max_iter=100)
+        res.append((f.get_params(), f.status()))
+    (pa, sa), (pb, sb) = res
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(sa['chisq'], sb['chisq'])
+    np.testing.assert_array_equal(sa['iterations'], sb['iterations'])
+    print('  iterations', sa['iterations'].tolist(), flush=True)
+    fs.close()
+    fp.close()
