@@ -542,6 +542,21 @@ int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int ma
                              int n_iter, void *stream);
 
 /*
+ * Branched replays for the last iterations of a fit (no reference
+ * counterpart: launch scheduling).  Once the last trh_fitter_status read
+ * counted fewer than `below_active` active sources, trh_fitter_iterate_graph
+ * replays a second captured iteration in which the render launches of each
+ * phase (one per FFT size, one for the point sources) run on their own
+ * library-owned streams between a fork and a join, so that their launches,
+ * latency-bound when few renders remain, overlap; under full load one
+ * stream is faster, and the first graph stays in use.  Results are identical
+ * either way (every render job writes its own window or chi^2 slot).
+ * trh_fitter_set_params goes back to the first graph.  0 (the default):
+ * never branch.
+ */
+int trh_fitter_set_branching(trh_fitter *f, int below_active);
+
+/*
  * Hybrid PSF (HybridPixelizedPSF, tractor/psf.py:537-625, as the GPU plugin
  * requires, tractor/gpu_optimizer.py:827): every galaxy render splits its
  * sheared components by the FFT box size sz (ProfileGalaxy, tractor/

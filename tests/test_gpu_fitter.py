@@ -164,6 +164,37 @@ def test_graph_replay_matches_eager(problem):
     assert np.array_equal(a, f.get_params())
 
 
+def test_branched_graph_matches_unbranched(problem):
+    """The branched iteration graph (render buckets on parallel streams,
+    trh_fitter_set_branching) reaches bit-identical parameters, chi^2 and
+    iteration counts to the single-stream graph and to eager launches: from
+    the first replay on (threshold above every active count), and switching
+    over part-way through a fit (threshold inside the range the active count
+    passes through)."""
+    say('branched graph vs unbranched')
+    f = problem['fitter']
+    start = problem['start']
+    S = len(start)
+
+    def fit(th, graph=True):
+        f.set_branching(th)
+        f.set_params(start)
+        f.fit(max_iter=100, graph=graph)
+        st = f.status()
+        return f.get_params(), st['chisq'], st['iterations']
+
+    try:
+        ref = fit(0)
+        eager = fit(0, graph=False)
+        for th in (S + 1, max(1, S // 2)):
+            got = fit(th)
+            for a, b, c in zip(ref, got, eager):
+                assert np.array_equal(a, b), th
+                assert np.array_equal(b, c), th
+    finally:
+        f.set_branching(B.DEFAULT_BRANCH_BELOW)
+
+
 def test_per_launch_profile(problem):
     """Per-launch profiling over several eager iterations: no HIP error
     (the 63-pixel PSF makes every FFT at least 64, so the 32 bucket is never

@@ -286,6 +286,18 @@ struct trh_fitter {
     bool have_gev = false;
     double g_lambda = 0.0, g_dchisq = 0.0;
     int g_max_iter = 0;
+    // the same iteration with the render buckets of each phase forked onto
+    // side streams (parallel graph branches), replayed instead of gexec once
+    // the last status read counted fewer than branch_below active sources
+    // (trh_fitter_set_branching; 0: never)
+    hipGraphExec_t gexec_br = nullptr;
+    double gb_lambda = 0.0, gb_dchisq = 0.0;
+    int gb_max_iter = 0;
+    int branch_below = 0;
+    int last_active = -1;   // -1: not read since the last set_params
+    hipStream_t bside[NBUCKET];
+    hipEvent_t bfork[2], bjoin[2][NBUCKET];
+    bool have_br = false;
     // the fitter's own row slots of the SZ >= 256 renders (sizes 256, 512),
     // so that fitters on different streams never share them
     float2 *hbuf[2] = {nullptr, nullptr};
@@ -1290,10 +1302,12 @@ __global__ void k_fit_chain_reset(int n, const int32_t *__restrict__ second,
 // Drop the captured iteration (its launches change, or the fitter goes):
 // after the replays already issued on the fitter's stream.
 static void drop_graph(trh_fitter *f) {
-    if (!f->gexec) return;
+    if (!f->gexec && !f->gexec_br) return;
     if (f->gstream) (void)hipStreamSynchronize(f->gstream);
-    (void)hipGraphExecDestroy(f->gexec);
+    if (f->gexec) (void)hipGraphExecDestroy(f->gexec);
+    if (f->gexec_br) (void)hipGraphExecDestroy(f->gexec_br);
     f->gexec = nullptr;
+    f->gexec_br = nullptr;
 }
 
 template <typename T>
@@ -1357,6 +1371,16 @@ extern "C" int trh_fitter_destroy(trh_fitter *f) {
     }
     if (f->have_gev)
         for (int i = 0; i < 2; ++i) (void)hipEventDestroy(f->gev[i]);
+    if (f->have_br) {
+        for (int i = 0; i < NBUCKET; ++i) {
+            (void)hipStreamSynchronize(f->bside[i]);
+            (void)hipStreamDestroy(f->bside[i]);
+        }
+        for (int p = 0; p < 2; ++p) {
+            (void)hipEventDestroy(f->bfork[p]);
+            for (int i = 0; i < NBUCKET; ++i) (void)hipEventDestroy(f->bjoin[p][i]);
+        }
+    }
     if (f->gstream) (void)hipStreamDestroy(f->gstream);
     delete f;
     return TRH_OK;
@@ -1507,8 +1531,16 @@ extern "C" int trh_fitter_create(trh_handle *h, trh_fitter **fp, int S, const in
     return TRH_OK;
 }
 
+extern "C" int trh_fitter_set_branching(trh_fitter *f, int below_active) {
+    if (!f || below_active < 0)
+        return set_err(TRH_E_ARG, "trh_fitter_set_branching: bad arguments");
+    f->branch_below = below_active;
+    return TRH_OK;
+}
+
 extern "C" int trh_fitter_set_params(trh_fitter *f, const double *params, void *stream) {
     if (!f || !params) return set_err(TRH_E_ARG, "trh_fitter_set_params: null argument");
+    f->last_active = -1;   // every source starts again: the unbranched graph
     std::vector<double> p8((size_t)8 * f->S, 0.0);
     for (int s = 0; s < f->S; ++s)
         for (int i = 0; i < TRH_FIT_NPARAM; ++i)
@@ -1661,8 +1693,34 @@ static int launch_mog(trh_fitter *f, const FitMogJob *jobs, const int32_t *lists
     return hip_check(hipGetLastError(), "k_fit_mog");
 }
 
+// Fork point of a render phase in a branched capture: the side streams
+// start after everything enqueued on `st` so far.
+static int branch_fork(trh_fitter *f, int ph, hipStream_t st) {
+    int rc = hip_check(hipEventRecord(f->bfork[ph], st), "branch fork");
+    for (int i = 0; i < NBUCKET && !rc; ++i)
+        rc = hip_check(hipStreamWaitEvent(f->bside[i], f->bfork[ph], 0), "branch wait");
+    return rc;
+}
+
+// Join of a render phase: `st` continues after every side stream's work.
+static int branch_join(trh_fitter *f, int ph, hipStream_t st) {
+    int rc = TRH_OK;
+    for (int i = 0; i < NBUCKET && !rc; ++i)
+        rc = hip_check(hipEventRecord(f->bjoin[ph][i], f->bside[i]), "branch join");
+    for (int i = 0; i < NBUCKET && !rc; ++i)
+        rc = hip_check(hipStreamWaitEvent(st, f->bjoin[ph][i], 0), "branch join wait");
+    return rc;
+}
+
+// The launches of one LM iteration.  `branch` (graph capture only): the FFT
+// size buckets and the point-source bucket of each render phase run on
+// their own side streams between a fork and a join (they write disjoint
+// outputs: every job its own render window or chi^2 slot, every FFT size
+// its own row slots), so under few active sources the latency-bound small
+// launches overlap instead of running one after another.  Every job's
+// arithmetic is the same either way, so the results are identical.
 static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_iter, hipStream_t st,
-                        bool capture) {
+                        bool capture, bool branch = false) {
     const int S = f->S;
     const int cap_jac = f->nj * S, cap_ls = NALPHA * S;
     const int tb = 256, nb = (S + tb - 1) / tb;
@@ -1683,27 +1741,32 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
                        f->bcnt_jac, f->list_jac, cap_jac, f->cnt);
     if ((rc = hip_check(hipGetLastError(), "k_fit_lists"))) return rc;
     SLOT_END(SL_PREP);
+    branch = branch && capture && f->have_br;
+    if (branch && (rc = branch_fork(f, 0, st))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         if (kSizes[b] < f->min_sz) continue;
         SLOT_BEGIN(b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
                                  f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal + f->n_ser, f->maxrows[b], f->spec[b], f->renders,
-                                 nullptr, nullptr, nullptr, st, fitter_hbuf(f, kSizes[b]))))
+                                 nullptr, nullptr, nullptr, branch ? f->bside[b] : st,
+                                 fitter_hbuf(f, kSizes[b]))))
             return rc;
         SLOT_END(b);
     }
-    // hybrid PSF: the pixel-space parts added to the galaxies' FFT renders
-    if ((rc = launch_mog(f, f->mog_jac, f->list_jac, cap_jac, f->cnt, 1, f->renders, 1, 0, st)))
-        return rc;
     SLOT_BEGIN(SL_JPT);
     if ((rc = launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt,
-                           f->renders, nullptr, st)))
+                           f->renders, nullptr, branch ? f->bside[PTB] : st)))
         return rc;
     SLOT_END(SL_JPT);
+    if (branch && (rc = branch_join(f, 0, st))) return rc;
+    // hybrid PSF: the pixel-space parts added to the galaxies' FFT renders
+    // (after them: they add into the same planes)
+    if ((rc = launch_mog(f, f->mog_jac, f->list_jac, cap_jac, f->cnt, 1, f->renders, 1, 0, st)))
+        return rc;
     if (!capture) (void)hipEventRecord(f->ev[1], st);
     SLOT_BEGIN(SL_NEQ);
     {
-        const int grid = S < f->h->n_cu * 8 ? S : f->h->n_cu * 8;
+        const int grid = S < f->h->n_cu * 16 ? S : f->h->n_cu * 16;
         if (S > f->n_ser) {
             hipLaunchKernelGGL(k_fit_normal_eq<6>, dim3(grid), dim3(NEQ_TB), 0, st, S, f->kind, f->P,
                                f->stride, f->pix_off, f->render_off, f->data, f->inverr,
@@ -1740,21 +1803,24 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
     // which its chi^2 render adds to the FFT model
     if ((rc = launch_mog(f, f->mog_ls, f->list_ls, cap_ls, f->cnt + LS_CNT, 0, f->ls_base, 1, 0, st)))
         return rc;
+    if (branch && (rc = branch_fork(f, 1, st))) return rc;
     for (int b = 0; b < NFFT && f->n_gal; ++b) {
         if (kSizes[b] < f->min_sz) continue;
         SLOT_BEGIN(SL_LS + b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
                                  f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
-                                 nullptr, f->data, f->inverr, f->chi2ls, st, fitter_hbuf(f, kSizes[b]),
+                                 nullptr, f->data, f->inverr, f->chi2ls,
+                                 branch ? f->bside[b] : st, fitter_hbuf(f, kSizes[b]),
                                  f->hyb ? f->ls_base : nullptr)))
             return rc;
         SLOT_END(SL_LS + b);
     }
     SLOT_BEGIN(SL_LPT);
     if ((rc = launch_point(f, f->jobs_ls, f->list_ls + (size_t)PTB * cap_ls, f->cnt + LS_CNT + PTB,
-                           NALPHA * f->n_pt, nullptr, f->chi2ls, st)))
+                           NALPHA * f->n_pt, nullptr, f->chi2ls, branch ? f->bside[PTB] : st)))
         return rc;
     SLOT_END(SL_LPT);
+    if (branch && (rc = branch_join(f, 1, st))) return rc;
     if (!capture) (void)hipEventRecord(f->ev[4], st);
     hipLaunchKernelGGL(k_fit_accept, dim3(nb), dim3(tb), 0, st, S, f->neq, f->trial, f->nsteps,
                        f->steplim, f->chi2ls, f->params, f->active, f->iters, f->chisq, f->dlnp,
@@ -1820,14 +1886,40 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
                 return rc;
         f->have_gev = true;
     }
-    if (!f->gexec || lambda != f->g_lambda || dchisq != f->g_dchisq || max_iter != f->g_max_iter) {
-        drop_graph(f);
+    // the branched iteration once few sources remain (see trh_fitter_set_branching)
+    const bool br = f->branch_below > 0 && f->last_active >= 0 && f->last_active < f->branch_below;
+    if (br && !f->have_br) {
+        for (int i = 0; i < NBUCKET; ++i)
+            if ((rc = hip_check(hipStreamCreateWithFlags(&f->bside[i], hipStreamNonBlocking),
+                                "branch stream")))
+                return rc;
+        for (int p = 0; p < 2; ++p) {
+            if ((rc = hip_check(hipEventCreateWithFlags(&f->bfork[p], hipEventDisableTiming),
+                                "branch event")))
+                return rc;
+            for (int i = 0; i < NBUCKET; ++i)
+                if ((rc = hip_check(hipEventCreateWithFlags(&f->bjoin[p][i], hipEventDisableTiming),
+                                    "branch event")))
+                    return rc;
+        }
+        f->have_br = true;
+    }
+    hipGraphExec_t &gx = br ? f->gexec_br : f->gexec;
+    double &kl = br ? f->gb_lambda : f->g_lambda;
+    double &kd = br ? f->gb_dchisq : f->g_dchisq;
+    int &km = br ? f->gb_max_iter : f->g_max_iter;
+    if (!gx || lambda != kl || dchisq != kd || max_iter != km) {
+        if (gx) {
+            (void)hipStreamSynchronize(f->gstream);
+            (void)hipGraphExecDestroy(gx);
+            gx = nullptr;
+        }
         // (the row slots of the SZ >= 256 renders are the fitter's own,
         // allocated at creation)
         if ((rc = hip_check(hipStreamBeginCapture(f->gstream, hipStreamCaptureModeThreadLocal),
                             "begin capture")))
             return rc;
-        const int brc = iterate_body(f, lambda, dchisq, max_iter, f->gstream, true);
+        const int brc = iterate_body(f, lambda, dchisq, max_iter, f->gstream, true, br);
         hipGraph_t g = nullptr;
         rc = hip_check(hipStreamEndCapture(f->gstream, &g), "end capture");
         if (brc) {
@@ -1835,19 +1927,19 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
             return brc;
         }
         if (rc) return rc;
-        rc = hip_check(hipGraphInstantiate(&f->gexec, g, nullptr, nullptr, 0), "graph instantiate");
+        rc = hip_check(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0), "graph instantiate");
         (void)hipGraphDestroy(g);
         if (rc) return rc;
-        f->g_lambda = lambda;
-        f->g_dchisq = dchisq;
-        f->g_max_iter = max_iter;
+        kl = lambda;
+        kd = dchisq;
+        km = max_iter;
     }
     hipStream_t st = (hipStream_t)stream;
     if ((rc = hip_check(hipEventRecord(f->gev[0], st), "graph order in")) ||
         (rc = hip_check(hipStreamWaitEvent(f->gstream, f->gev[0], 0), "graph wait in")))
         return rc;
     for (int i = 0; i < n_iter; ++i)
-        if ((rc = hip_check(hipGraphLaunch(f->gexec, f->gstream), "graph launch"))) return rc;
+        if ((rc = hip_check(hipGraphLaunch(gx, f->gstream), "graph launch"))) return rc;
     if ((rc = hip_check(hipEventRecord(f->gev[1], f->gstream), "graph order out")) ||
         (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "graph wait out")))
         return rc;
@@ -2048,11 +2140,10 @@ extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterat
     if ((rc = hip_check(hipMemcpy(a.data(), f->active, 4 * (size_t)f->S, hipMemcpyDeviceToHost), "read active")))
         return rc;
     if (active) std::copy(a.begin(), a.end(), active);
-    if (n_active) {
-        int n = 0;
-        for (int v : a) n += v ? 1 : 0;
-        *n_active = n;
-    }
+    int n = 0;
+    for (int v : a) n += v ? 1 : 0;
+    if (n_active) *n_active = n;
+    f->last_active = n;
     if (iterations &&
         (rc = hip_check(hipMemcpy(iterations, f->iters, 4 * (size_t)f->S, hipMemcpyDeviceToHost), "read iters")))
         return rc;
