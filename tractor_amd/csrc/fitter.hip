@@ -1889,18 +1889,26 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
     // the branched iteration once few sources remain (see trh_fitter_set_branching)
     const bool br = f->branch_below > 0 && f->last_active >= 0 && f->last_active < f->branch_below;
     if (br && !f->have_br) {
-        for (int i = 0; i < NBUCKET; ++i)
-            if ((rc = hip_check(hipStreamCreateWithFlags(&f->bside[i], hipStreamNonBlocking),
-                                "branch stream")))
-                return rc;
+        // all side streams and events, or none (what was made is released)
+        int ns = 0, ne = 0;
+        hipEvent_t evs[2 * (NBUCKET + 1)];
+        rc = TRH_OK;
+        while (!rc && ns < NBUCKET)
+            if (!(rc = hip_check(hipStreamCreateWithFlags(&f->bside[ns], hipStreamNonBlocking),
+                                 "branch stream")))
+                ++ns;
+        while (!rc && ne < 2 * (NBUCKET + 1))
+            if (!(rc = hip_check(hipEventCreateWithFlags(&evs[ne], hipEventDisableTiming),
+                                 "branch event")))
+                ++ne;
+        if (rc) {
+            for (int i = 0; i < ne; ++i) (void)hipEventDestroy(evs[i]);
+            for (int i = 0; i < ns; ++i) (void)hipStreamDestroy(f->bside[i]);
+            return rc;
+        }
         for (int p = 0; p < 2; ++p) {
-            if ((rc = hip_check(hipEventCreateWithFlags(&f->bfork[p], hipEventDisableTiming),
-                                "branch event")))
-                return rc;
-            for (int i = 0; i < NBUCKET; ++i)
-                if ((rc = hip_check(hipEventCreateWithFlags(&f->bjoin[p][i], hipEventDisableTiming),
-                                    "branch event")))
-                    return rc;
+            f->bfork[p] = evs[p * (NBUCKET + 1)];
+            for (int i = 0; i < NBUCKET; ++i) f->bjoin[p][i] = evs[p * (NBUCKET + 1) + 1 + i];
         }
         f->have_br = true;
     }

@@ -843,6 +843,36 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
     __syncthreads();
 }
 
+// The spatial-Lanczos render (a job whose taps reach past the box edges) as
+// a call of its own in the chi^2 kernels from SZ = 256 on.  Inlined beside
+// the spectrum-filter path it sets the register allocation of the whole
+// kernel (sz = 256 chi^2: 380 bytes of spills per lane, 56 without it), and
+// every render pays for it: as a call, the spectrum-filter renders take
+// 182 ns instead of 192 and the spatial ones 175 instead of 186 (none of a
+// batched fit's sz = 256 windows needs them).  Below 256 and in the STORE
+// kernels the call costs more than it saves (sz = 128 chi^2: -1.4 ns on the
+// spectrum-filter path, +17 ns on the spatial path, which 10-13% of a
+// batched fit's windows take), so those keep it inline.  The call builds
+// its own twiddle registers; results are the same bit for bit.
+template <int SZ, int MODE>
+constexpr bool SPATIAL_CALL = SZ >= 256 && (MODE == TRH_CHI2 || MODE == TRH_CHI2_BASE);
+
+template <int SZ, int MODE>
+__device__ __noinline__ void render_fft_spatial(const trh_fft_job *__restrict__ Jp,
+                                                const float2 *__restrict__ psf_spec,
+                                                const float2 *__restrict__ twg, float2 *ltab,
+                                                float2 *lbuf, float2 *hbuf, int maxrows,
+                                                RenderScratch *S, float *__restrict__ out,
+                                                const float *__restrict__ data,
+                                                const float *__restrict__ inverr,
+                                                double *__restrict__ chi2,
+                                                const float *__restrict__ base) {
+    WaveTw<SZ> T;
+    T.load(twg, (threadIdx.x & 63) & (WaveGeom<SZ>::SEQL - 1));
+    render_fft_one<SZ, MODE, false>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, *S, out,
+                                    data, inverr, chi2, base);
+}
+
 // Grid-stride render kernel over a job list.  `count` (device, may be NULL:
 // then njobs) bounds the list; `list` (may be NULL: identity) maps list
 // entries to job indices.  base: MODE TRH_CHI2_BASE only.  blockDim.x is a multiple of 64; dynamic LDS holds
@@ -884,6 +914,9 @@ k_render_fft(const trh_fft_job *__restrict__ jobs, const int *__restrict__ list,
         if (fd)
             render_fft_one<SZ, MODE, true>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, S,
                                            out, data, inverr, chi2, base);
+        else if constexpr (SPATIAL_CALL<SZ, MODE>)
+            render_fft_spatial<SZ, MODE>(Jp, psf_spec, twg, ltab, lbuf, hbuf, maxrows, &S, out,
+                                         data, inverr, chi2, base);
         else
             render_fft_one<SZ, MODE, false>(Jp, psf_spec, T, twg, ltab, lbuf, hbuf, maxrows, S,
                                             out, data, inverr, chi2, base);
