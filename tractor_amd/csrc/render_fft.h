@@ -844,7 +844,7 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 }
 
 // The spatial-Lanczos render (a job whose taps reach past the box edges) as
-// a call of its own in the chi^2 kernels from SZ = 256 on.  Inlined beside
+// a call of its own in the SZ = 256 chi^2 kernels.  Inlined beside
 // the spectrum-filter path it sets the register allocation of the whole
 // kernel (sz = 256 chi^2: 380 bytes of spills per lane, 56 without it), and
 // every render pays for it: as a call, the spectrum-filter renders take
@@ -852,10 +852,12 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
 // batched fit's sz = 256 windows needs them).  Below 256 and in the STORE
 // kernels the call costs more than it saves (sz = 128 chi^2: -1.4 ns on the
 // spectrum-filter path, +17 ns on the spatial path, which 10-13% of a
-// batched fit's windows take), so those keep it inline.  The call builds
+// batched fit's windows take), so those keep it inline; at SZ = 512 (rare
+// jobs) the call's stack frame costs 2 us per launch, so it stays inline
+// there too.  The call builds
 // its own twiddle registers; results are the same bit for bit.
 template <int SZ, int MODE>
-constexpr bool SPATIAL_CALL = SZ >= 256 && (MODE == TRH_CHI2 || MODE == TRH_CHI2_BASE);
+constexpr bool SPATIAL_CALL = SZ == 256 && (MODE == TRH_CHI2 || MODE == TRH_CHI2_BASE);
 
 template <int SZ, int MODE>
 __device__ __noinline__ void render_fft_spatial(const trh_fft_job *__restrict__ Jp,
