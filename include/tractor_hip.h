@@ -531,11 +531,12 @@ int trh_fitter_iterate(trh_fitter *f, double lambda, double dchisq, int max_iter
                        void *stream);
 
 /*
- * `n_iter` LM iterations replayed from a hipGraph that captures one
- * iteration's launches (captured on first use, and again when lambda,
- * dchisq or max_iter change), on a library-owned stream ordered after and
- * before `stream`.  Same results as calling trh_fitter_iterate n_iter times;
- * not available while per-launch profiling is on.  No reference counterpart
+ * `n_iter` LM iterations replayed from a hipGraph that captures the
+ * launches of four iterations when n_iter is a multiple of four, else of
+ * one (each captured on first use, and again when lambda, dchisq or
+ * max_iter change), on a library-owned stream ordered after and before
+ * `stream`.  Same results as calling trh_fitter_iterate n_iter times; not
+ * available while per-launch profiling is on.  No reference counterpart
  * (launch scheduling).
  */
 int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int max_iter,
@@ -555,6 +556,20 @@ int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int ma
  * never branch.
  */
 int trh_fitter_set_branching(trh_fitter *f, int below_active);
+
+/*
+ * The number of active sources without synchronising the fit (no reference
+ * counterpart: the check between graph replays of BatchFitter.fit).
+ * trh_fitter_count_active enqueues a count on `stream`, ordered after the
+ * fitter's replays so far; its result reaches pinned host memory on a
+ * library-owned stream while later replays run (an earlier count that was
+ * not read is dropped).  trh_fitter_count_read returns it: with `wait`
+ * after waiting for it, without `wait` -1 if it has not arrived yet.  The
+ * count it returns is also what trh_fitter_set_branching's threshold is
+ * compared with.
+ */
+int trh_fitter_count_active(trh_fitter *f, void *stream);
+int trh_fitter_count_read(trh_fitter *f, int wait, int32_t *n_active);
 
 /*
  * Hybrid PSF (HybridPixelizedPSF, tractor/psf.py:537-625, as the GPU plugin

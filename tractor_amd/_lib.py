@@ -105,6 +105,8 @@ SIGNATURES = {
     'trh_fitter_iterate': (_I, [_P, _D, _D, _I, _P]),
     'trh_fitter_iterate_graph': (_I, [_P, _D, _D, _I, _I, _P]),
     'trh_fitter_set_branching': (_I, [_P, _I]),
+    'trh_fitter_count_active': (_I, [_P, _P]),
+    'trh_fitter_count_read': (_I, [_P, _I, _P]),
     'trh_fitter_render_models': (_I, [_P, _P, _I, _P]),
     'trh_fitter_status': (_I, [_P, _P, _P, _P, _P, _P, _P]),
     'trh_fitter_flags': (_I, [_P, _P, _P]),

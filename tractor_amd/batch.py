@@ -208,17 +208,43 @@ class BatchFitter(object):
         # iteration and then runs max_iter of its own
         total = 2 * max_iter if self.n_chain else max_iter
         it = 0
+        pending = False
         while it < total:
             k = min(check_every, total - it)
-            if graph:
-                self.iterate_graph(k, lam, dchisq, max_iter)
-            else:
+            if not graph:
                 for _ in range(k):
                     self.iterate(lam, dchisq, max_iter)
+                it += k
+                if self.n_active() == 0:
+                    break
+                continue
+            self.iterate_graph(k, lam, dchisq, max_iter)
             it += k
-            if self.n_active() == 0:
-                break
+            # the check lags one chunk: the count of the previous chunk is
+            # read while this one runs (trh_fitter_count_active), so the
+            # device never waits for the host; a fit that has finished runs
+            # one more chunk of iterations with nothing left to do
+            if pending:
+                n = self.count_read(wait=True)
+                pending = False
+                if n == 0:
+                    break
+            self.count_active()
+            pending = True
+        if pending:
+            self.count_read(wait=True)
         return it
+
+    def count_active(self):
+        """Enqueue a count of the active sources (trh_fitter_count_active)."""
+        check(self.lib.trh_fitter_count_active(self.f, self.stream), 'count_active')
+
+    def count_read(self, wait=True):
+        """The enqueued count (-1 without `wait` if it has not arrived)."""
+        n = ctypes.c_int32()
+        check(self.lib.trh_fitter_count_read(self.f, 1 if wait else 0, ctypes.byref(n)),
+              'count_read')
+        return n.value
 
     # ---- debugging views of device state (tests) ----
     def debug_arrays(self):

@@ -39,6 +39,7 @@ using namespace trh;
 #define SL_LPT 13
 #define SL_PREP 14
 #define TRH_NSLOT 15
+#define GRAPH_ITERS 4   // iterations per captured graph (BatchFitter.fit checks every 4)
 
 // Normal equations of one source: neq[NEQ_W s + .] holds the packed upper
 // triangle of the NC x NC matrix J^T W J (NC = 6; 7 for a Sersic galaxy),
@@ -279,25 +280,36 @@ struct trh_fitter {
     double tot_ms[TRH_NSLOT];
     double tot_launch[TRH_NSLOT];
     double tot_extra_bytes[TRH_NSLOT];   // PSF spectrum reads, counted per launch
-    // one LM iteration captured as a hipGraph, replayed on gstream
+    // LM iterations captured as hipGraphs, replayed on gstream: one or
+    // GRAPH_ITERS iterations per graph (the gap between two graph launches
+    // is ~7.5 us, between the kernels inside one none), each on one stream
+    // or with the render launches of each phase forked onto side streams
+    // (parallel graph branches, replayed once the last count of active
+    // sources was below branch_below; trh_fitter_set_branching, 0: never).
+    // Slot (branched ? 2 : 0) + (GRAPH_ITERS iterations ? 1 : 0), each with
+    // the lambda, dchisq and max_iter it was captured with.
     hipStream_t gstream = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    struct GraphSlot {
+        hipGraphExec_t x = nullptr;
+        double lambda = 0.0, dchisq = 0.0;
+        int max_iter = 0;
+    } gs[4];
     hipEvent_t gev[2];
     bool have_gev = false;
-    double g_lambda = 0.0, g_dchisq = 0.0;
-    int g_max_iter = 0;
-    // the same iteration with the render buckets of each phase forked onto
-    // side streams (parallel graph branches), replayed instead of gexec once
-    // the last status read counted fewer than branch_below active sources
-    // (trh_fitter_set_branching; 0: never)
-    hipGraphExec_t gexec_br = nullptr;
-    double gb_lambda = 0.0, gb_dchisq = 0.0;
-    int gb_max_iter = 0;
     int branch_below = 0;
     int last_active = -1;   // -1: not read since the last set_params
     hipStream_t bside[NBUCKET];
     hipEvent_t bfork[2], bjoin[2][NBUCKET];
     bool have_br = false;
+    // active-source count read without synchronising the fit
+    // (trh_fitter_count_active / trh_fitter_count_read): a one-block count
+    // kernel on the caller's stream, its result copied to pinned host memory
+    // on a side stream while the next replays run
+    int32_t *nact_dev = nullptr;
+    int32_t *nact_pin = nullptr;
+    hipStream_t cstream = nullptr;
+    hipEvent_t cev[2];
+    bool have_count = false, count_pending = false;
     // the fitter's own row slots of the SZ >= 256 renders (sizes 256, 512),
     // so that fitters on different streams never share them
     float2 *hbuf[2] = {nullptr, nullptr};
@@ -1302,12 +1314,14 @@ __global__ void k_fit_chain_reset(int n, const int32_t *__restrict__ second,
 // Drop the captured iteration (its launches change, or the fitter goes):
 // after the replays already issued on the fitter's stream.
 static void drop_graph(trh_fitter *f) {
-    if (!f->gexec && !f->gexec_br) return;
+    bool any = false;
+    for (auto &g : f->gs) any = any || g.x;
+    if (!any) return;
     if (f->gstream) (void)hipStreamSynchronize(f->gstream);
-    if (f->gexec) (void)hipGraphExecDestroy(f->gexec);
-    if (f->gexec_br) (void)hipGraphExecDestroy(f->gexec_br);
-    f->gexec = nullptr;
-    f->gexec_br = nullptr;
+    for (auto &g : f->gs) {
+        if (g.x) (void)hipGraphExecDestroy(g.x);
+        g.x = nullptr;
+    }
 }
 
 template <typename T>
@@ -1380,6 +1394,12 @@ extern "C" int trh_fitter_destroy(trh_fitter *f) {
             (void)hipEventDestroy(f->bfork[p]);
             for (int i = 0; i < NBUCKET; ++i) (void)hipEventDestroy(f->bjoin[p][i]);
         }
+    }
+    if (f->have_count) {
+        (void)hipStreamSynchronize(f->cstream);
+        (void)hipStreamDestroy(f->cstream);
+        for (int i = 0; i < 2; ++i) (void)hipEventDestroy(f->cev[i]);
+        (void)hipHostFree(f->nact_pin);
     }
     if (f->gstream) (void)hipStreamDestroy(f->gstream);
     delete f;
@@ -1912,10 +1932,14 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
         }
         f->have_br = true;
     }
-    hipGraphExec_t &gx = br ? f->gexec_br : f->gexec;
-    double &kl = br ? f->gb_lambda : f->g_lambda;
-    double &kd = br ? f->gb_dchisq : f->g_dchisq;
-    int &km = br ? f->gb_max_iter : f->g_max_iter;
+    // GRAPH_ITERS iterations per graph launch when n_iter is a multiple
+    const bool multi = n_iter > 0 && n_iter % GRAPH_ITERS == 0;
+    const int per = multi ? GRAPH_ITERS : 1;
+    trh_fitter::GraphSlot &slot = f->gs[(br ? 2 : 0) + (multi ? 1 : 0)];
+    hipGraphExec_t &gx = slot.x;
+    double &kl = slot.lambda;
+    double &kd = slot.dchisq;
+    int &km = slot.max_iter;
     if (!gx || lambda != kl || dchisq != kd || max_iter != km) {
         if (gx) {
             (void)hipStreamSynchronize(f->gstream);
@@ -1927,7 +1951,9 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
         if ((rc = hip_check(hipStreamBeginCapture(f->gstream, hipStreamCaptureModeThreadLocal),
                             "begin capture")))
             return rc;
-        const int brc = iterate_body(f, lambda, dchisq, max_iter, f->gstream, true, br);
+        int brc = TRH_OK;
+        for (int q = 0; q < per && !brc; ++q)
+            brc = iterate_body(f, lambda, dchisq, max_iter, f->gstream, true, br);
         hipGraph_t g = nullptr;
         rc = hip_check(hipStreamEndCapture(f->gstream, &g), "end capture");
         if (brc) {
@@ -1946,7 +1972,7 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
     if ((rc = hip_check(hipEventRecord(f->gev[0], st), "graph order in")) ||
         (rc = hip_check(hipStreamWaitEvent(f->gstream, f->gev[0], 0), "graph wait in")))
         return rc;
-    for (int i = 0; i < n_iter; ++i)
+    for (int i = 0; i < n_iter; i += per)
         if ((rc = hip_check(hipGraphLaunch(gx, f->gstream), "graph launch"))) return rc;
     if ((rc = hip_check(hipEventRecord(f->gev[1], f->gstream), "graph order out")) ||
         (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "graph wait out")))
@@ -2159,6 +2185,92 @@ extern "C" int trh_fitter_status(trh_fitter *f, int32_t *active, int32_t *iterat
         return rc;
     if (dlnp && (rc = hip_check(hipMemcpy(dlnp, f->dlnp, 8 * (size_t)f->S, hipMemcpyDeviceToHost), "read dlnp")))
         return rc;
+    return TRH_OK;
+}
+
+// Number of active sources into out[0] (one block).
+__global__ void __launch_bounds__(1024) k_fit_count_active(int S, const int32_t *__restrict__ active,
+                                                           int32_t *__restrict__ out) {
+    __shared__ int32_t part[16];
+    int n = 0;
+    for (int s = threadIdx.x; s < S; s += blockDim.x) n += active[s] ? 1 : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) n += __shfl_xor(n, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        out[0] = t;
+    }
+}
+
+extern "C" int trh_fitter_count_active(trh_fitter *f, void *stream) {
+    if (!f) return set_err(TRH_E_ARG, "trh_fitter_count_active: null fitter");
+    int rc;
+    // an unread earlier count is dropped (its copy finishes first: the
+    // pinned word is written by one copy at a time)
+    if (f->count_pending && (rc = hip_check(hipEventSynchronize(f->cev[1]), "count drop")))
+        return rc;
+    f->count_pending = false;
+    if (!f->have_count) {
+        if ((rc = falloc(f, &f->nact_dev, 1))) return rc;
+        if ((rc = hip_check(hipHostMalloc((void **)&f->nact_pin, sizeof(int32_t)), "pinned count")))
+            return rc;
+        if ((rc = hip_check(hipStreamCreateWithFlags(&f->cstream, hipStreamNonBlocking),
+                            "count stream"))) {
+            (void)hipHostFree(f->nact_pin);
+            return rc;
+        }
+        int ne = 0;
+        while (!rc && ne < 2)
+            if (!(rc = hip_check(hipEventCreateWithFlags(&f->cev[ne], hipEventDisableTiming),
+                                 "count event")))
+                ++ne;
+        if (rc) {
+            for (int i = 0; i < ne; ++i) (void)hipEventDestroy(f->cev[i]);
+            (void)hipStreamDestroy(f->cstream);
+            (void)hipHostFree(f->nact_pin);
+            return rc;
+        }
+        f->have_count = true;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    // after the replays issued so far (trh_fitter_iterate_graph leaves
+    // `stream` ordered after them; again here for a different stream)
+    if (f->have_gev && (rc = hip_check(hipStreamWaitEvent(st, f->gev[1], 0), "count order")))
+        return rc;
+    hipLaunchKernelGGL(k_fit_count_active, dim3(1), dim3(1024), 0, st, f->S, f->active,
+                       f->nact_dev);
+    if ((rc = hip_check(hipGetLastError(), "k_fit_count_active")) ||
+        (rc = hip_check(hipEventRecord(f->cev[0], st), "count event")) ||
+        (rc = hip_check(hipStreamWaitEvent(f->cstream, f->cev[0], 0), "count wait")) ||
+        (rc = hip_check(hipMemcpyAsync(f->nact_pin, f->nact_dev, sizeof(int32_t),
+                                       hipMemcpyDeviceToHost, f->cstream), "count copy")) ||
+        (rc = hip_check(hipEventRecord(f->cev[1], f->cstream), "count done")))
+        return rc;
+    f->count_pending = true;
+    return TRH_OK;
+}
+
+extern "C" int trh_fitter_count_read(trh_fitter *f, int wait, int32_t *n_active) {
+    if (!f || !n_active) return set_err(TRH_E_ARG, "trh_fitter_count_read: null argument");
+    if (!f->count_pending)
+        return set_err(TRH_E_ARG, "trh_fitter_count_read: no count enqueued");
+    int rc;
+    if (wait) {
+        if ((rc = hip_check(hipEventSynchronize(f->cev[1]), "count sync"))) return rc;
+    } else {
+        const hipError_t q = hipEventQuery(f->cev[1]);
+        if (q == hipErrorNotReady) {
+            *n_active = -1;
+            return TRH_OK;
+        }
+        if ((rc = hip_check(q, "count query"))) return rc;
+    }
+    *n_active = *f->nact_pin;
+    f->last_active = *n_active;
+    f->count_pending = false;
     return TRH_OK;
 }
 
