@@ -504,24 +504,47 @@ class BatchForcedPhotometry(object):
         status = torch.empty(2, dtype=torch.int32, device=self.dev)
         pending = []
         it0 = 0
+        res_prev = None   # relative residual at the end of the previous chunk read
+        hold = False      # the chunk in flight is predicted to be the last one needed
+        # per chunk: status and (r.r, b.b) copied to pinned host memory, and
+        # an event; two slots (at most two chunks are in flight), allocated
+        # once per solve
+        hs_ring = torch.empty((2, 2), dtype=torch.int32, pin_memory=True)
+        hn_ring = torch.empty((2, 2), dtype=torch.float64, pin_memory=True)
+        ev_ring = [torch.cuda.Event(), torch.cuda.Event()]
+        nchunk = 0
         while True:
-            if it0 < maxiter:
+            if it0 < maxiter and (not hold or not pending):
                 k = min(chunk, maxiter - it0)
                 check(lib.trh_fp_pcg(h, n, ptr(d_rp), ptr(d_col), ptr(self.vals), ptr(d_ts),
                                      ptr(d_bo), nt, self.max_tile, ptr(blocks), ptr(self.b),
                                      float(tol), it0, k, ptr(x), ptr(work), ptr(status), st),
                       'fp_pcg')
                 it0 += k
-                hs = torch.empty(2, dtype=torch.int32, pin_memory=True)
+                slot = nchunk & 1
+                nchunk += 1
+                hs, hn, ev = hs_ring[slot], hn_ring[slot], ev_ring[slot]
                 hs.copy_(status, non_blocking=True)
-                ev = torch.cuda.Event()
+                hn.copy_(work[-2:], non_blocking=True)
                 ev.record()
-                pending.append((hs, ev))
-            if len(pending) > 1 or it0 >= maxiter:
-                hs, ev = pending.pop(0)
+                pending.append((hs, hn, ev))
+            if len(pending) > 1 or it0 >= maxiter or hold:
+                hs, hn, ev = pending.pop(0)
                 ev.synchronize()
-                if int(hs[0]) != 0 or not pending:
+                if int(hs[0]) != 0 or (not pending and it0 >= maxiter):
                     break
+                # the residual's decay over the chunk just read predicts the
+                # iterations still needed; while the chunk in flight covers
+                # them, no further chunk is enqueued (it would be `chunk`
+                # pairs of launches that only return), and a wrong guess
+                # costs one host round trip
+                rr, bb = float(hn[0]), float(hn[1])
+                res = np.sqrt(rr / bb) if bb > 0 else 0.0
+                hold = False
+                if pending and res_prev is not None and 0 < res < res_prev and res > tol:
+                    rate = (res / res_prev) ** (1.0 / chunk)
+                    hold = bool(np.log(tol / res) / np.log(rate) <= chunk - 0.5)
+                res_prev = res
         torch.cuda.synchronize(self.dev)
         self.status = int(hs[0])
         self.iterations = int(hs[1])
