@@ -195,6 +195,30 @@ def test_branched_graph_matches_unbranched(problem):
         f.set_branching(B.DEFAULT_BRANCH_BELOW)
 
 
+def test_active_count_without_synchronising(problem):
+    """trh_fitter_count_active / trh_fitter_count_read: the count enqueued
+    after some graph replays equals the synchronising status read's; without
+    waiting it returns -1 or the count; an unread count is dropped by the next
+    one; reading with none enqueued is an error."""
+    say('active count')
+    f = problem['fitter']
+    f.set_params(problem['start'])
+    f.iterate_graph(4, 0., 0.)
+    f.count_active()
+    n = f.count_read(wait=True)
+    assert n == f.status()['n_active']
+    f.iterate_graph(4, 0., 0.)
+    f.count_active()
+    f.count_active()          # drops the first, unread
+    got = f.count_read(wait=False)
+    torch.cuda.synchronize()
+    if got < 0:
+        got = f.count_read(wait=True)
+    assert got == f.status()['n_active']
+    with pytest.raises(RuntimeError):
+        f.count_read(wait=True)
+
+
 def test_per_launch_profile(problem):
     """Per-launch profiling over several eager iterations: no HIP error
     (the 63-pixel PSF makes every FFT at least 64, so the 32 bucket is never

@@ -91,6 +91,9 @@ def test_every_entry_point_rejects_bad_arguments_without_a_device():
         'trh_fitter_iterate': (N, 0., 0., 100, N),
         'trh_fitter_iterate_graph': (N, 0., 0., 100, 1, N),
         'trh_fitter_flags': (N, N, N),
+        'trh_fitter_set_branching': (N, 100),
+        'trh_fitter_count_active': (N, N),
+        'trh_fitter_count_read': (N, 1, N),
         'trh_jacobian_normal_eq_batch': (N, 1) + (N,) * 11 + (0.01, N, N),
         'trh_solve_damped_cholesky_f64_batch': (N, 1, N, N, 0., N, N, N, N, N, N),
         'trh_line_search_chi2_batch': (N, N, N, N, N),
