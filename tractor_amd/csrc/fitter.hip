@@ -860,6 +860,69 @@ struct NeqPix {
         in.e = E[di];
         in.d = D[di];
     }
+    // Four pixels of one row (x .. x + 3, x a multiple of 4) from one 16-byte
+    // load per plane and two scalar loads for the outer neighbours: the same
+    // inputs as four load() calls, with a quarter of the load instructions.
+    // Needs P, the row stride and the plane and window starts to be
+    // multiples of 4 floats (vec4_ok).
+    template <int I>
+    __device__ static __forceinline__ float lane4(const float4 &v) {
+        if constexpr (I == 0) return v.x;
+        else if constexpr (I == 1) return v.y;
+        else if constexpr (I == 2) return v.z;
+        else return v.w;
+    }
+    template <int I>
+    __device__ __forceinline__ void pix4(NeqIn<NC> &in, bool ry, int x, float l, float r,
+                                         const float4 &c, const float4 &up, const float4 &dn,
+                                         const float4 &e4, const float4 &d4) const {
+        const bool inner = ry && x + I > 0 && x + I < P - 1;
+        in.r0 = lane4<I>(c);
+        const float li = inner ? (I == 0 ? l : lane4<(I > 0 ? I - 1 : 0)>(c)) : 0.f;
+        const float ri = inner ? (I == 3 ? r : lane4<(I < 3 ? I + 1 : 3)>(c)) : 0.f;
+        const float ui = inner ? lane4<I>(up) : 0.f, wi = inner ? lane4<I>(dn) : 0.f;
+        in.dx = (li - ri) / 2.f;
+        in.dy = (ui - wi) / 2.f;
+        in.e = lane4<I>(e4);
+        in.d = lane4<I>(d4);
+    }
+    __device__ __forceinline__ void load4(int idx, NeqIn<NC> &i0, NeqIn<NC> &i1, NeqIn<NC> &i2,
+                                          NeqIn<NC> &i3) const {
+        const int y = (int)(((float)idx + 0.5f) * invP);
+        const int x = idx - y * P;
+        const bool ry = y > 0 && y < P - 1;
+        const float4 c = *reinterpret_cast<const float4 *>(R0 + idx);
+        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 up = zero, dn = zero;
+        float l = 0.f, r = 0.f;
+        if (ry) {
+            up = *reinterpret_cast<const float4 *>(R0 + idx - P);
+            dn = *reinterpret_cast<const float4 *>(R0 + idx + P);
+            if (x > 0) l = R0[idx - 1];
+            if (x + 4 < P) r = R0[idx + 4];
+        }
+        const int di = y * ds + x;
+        const float4 e4 = *reinterpret_cast<const float4 *>(E + di);
+        const float4 d4 = *reinterpret_cast<const float4 *>(D + di);
+        // the derivative planes of a galaxy (a point source has none)
+#pragma unroll
+        for (int k = 0; k < NC - 3; ++k) {
+            float4 gq = zero;
+            if (kd) gq = *reinterpret_cast<const float4 *>(R0 + (k + 1) * PP + idx);
+            i0.g[k] = gq.x;
+            i1.g[k] = gq.y;
+            i2.g[k] = gq.z;
+            i3.g[k] = gq.w;
+        }
+        pix4<0>(i0, ry, x, l, r, c, up, dn, e4, d4);
+        pix4<1>(i1, ry, x, l, r, c, up, dn, e4, d4);
+        pix4<2>(i2, ry, x, l, r, c, up, dn, e4, d4);
+        pix4<3>(i3, ry, x, l, r, c, up, dn, e4, d4);
+    }
+    __device__ __forceinline__ bool vec4_ok() const {
+        return (P & 3) == 0 && (ds & 3) == 0 && ((uintptr_t)R0 & 15) == 0 &&
+               ((uintptr_t)D & 15) == 0 && ((uintptr_t)E & 15) == 0;
+    }
     __device__ __forceinline__ void add(const NeqIn<NC> &in, float (&acc)[NQ], double &chi2,
                                         double &chi0) const {
         const float e = in.e, r0 = in.r0;
@@ -889,15 +952,14 @@ struct NeqPix {
 // J^T W J, J^T W r (FP32 products and sums) and chi^2 (FP64) for one source
 // per NEQ_TB-thread block, grid-stride over the sources in order of decreasing
 // P (sources span 64x in pixels; big ones first keeps the tail short).  Each
-// thread takes four pixels per step and issues all their loads before any of
-// their arithmetic (written as four inlined per-pixel steps, the compiler
-// interleaved loads and arithmetic: 2.9 TB/s against 4.0 on one config-2
-// iteration); each wave reduce-scatters its first 32 sums
+// thread takes runs of four consecutive pixels of a row and issues all their
+// loads before any of their arithmetic, as one 16-byte load per plane when
+// the window is aligned (a quarter of the load instructions of per-pixel
+// loads: 9.4 -> 8.6 ms of a 16384-source fit); each wave reduce-scatters its first 32 sums
 // (wave_reduce_scatter32), adds up the rest lane-wide, and the waves combine
 // through LDS.  NC = 6 takes every source but the Sersic galaxies, NC = 7
 // only those.
 #define NEQ_TB 512   // threads per source in k_fit_normal_eq
-#define NEQ_U 4      // pixels per thread in flight
 
 template <int NC>
 __global__ void __launch_bounds__(NEQ_TB)
@@ -937,19 +999,27 @@ k_fit_normal_eq(int S, const int32_t *__restrict__ kind, const int32_t *__restri
 #pragma unroll
         for (int k = 0; k < NQ; ++k) acc[k] = 0.f;
         double chi2 = 0.0, chi0 = 0.0;
-        int idx = threadIdx.x;
-        for (; idx + (NEQ_U - 1) * NEQ_TB < px.PP; idx += NEQ_U * NEQ_TB) {
-            // all the pixels' loads issue before any of their arithmetic
-            NeqIn<NC> in[NEQ_U];
-#pragma unroll
-            for (int u = 0; u < NEQ_U; ++u) px.load(idx + u * NEQ_TB, in[u]);
-#pragma unroll
-            for (int u = 0; u < NEQ_U; ++u) px.add(in[u], acc, chi2, chi0);
-        }
-        for (; idx < px.PP; idx += NEQ_TB) {
-            NeqIn<NC> in;
-            px.load(idx, in);
-            px.add(in, acc, chi2, chi0);
+        // each thread takes runs of four consecutive pixels (runs tid, tid +
+        // NEQ_TB, ...) and adds them in pixel order: with 16-byte loads when
+        // the window allows them, else four scalar loads, so that the sums
+        // (and the fit) are the same bit for bit wherever the window lies
+        const bool v4 = px.vec4_ok();
+        const int G4 = (px.PP + 3) >> 2;
+        for (int g = threadIdx.x; g < G4; g += NEQ_TB) {
+            const int idx = 4 * g;
+            NeqIn<NC> a0, a1, a2, a3;
+            if (v4) {
+                px.load4(idx, a0, a1, a2, a3);
+            } else {
+                px.load(idx, a0);
+                px.load(min(idx + 1, px.PP - 1), a1);
+                px.load(min(idx + 2, px.PP - 1), a2);
+                px.load(min(idx + 3, px.PP - 1), a3);
+            }
+            px.add(a0, acc, chi2, chi0);
+            if (idx + 1 < px.PP) px.add(a1, acc, chi2, chi0);
+            if (idx + 2 < px.PP) px.add(a2, acc, chi2, chi0);
+            if (idx + 3 < px.PP) px.add(a3, acc, chi2, chi0);
         }
         float v32[32];
 #pragma unroll
