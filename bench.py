@@ -674,6 +674,18 @@ def surrogate_leg(wl, torch, args, plain_dt, plain):
     prof = {}
     if not args.no_slot_profile:
         S.fit_with_fallback(*args_fit, profile=prof, **kw)
+    # the same pipeline without the plain fit's chi^2 (what a pipeline that
+    # has not run the plain fit can do): fall back on the chi-square
+    # self-check instead
+    kw_sc = dict(kw, plain_chisq=None)
+    S.fit_with_fallback(*args_fit, **kw_sc)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    for _ in range(args.steps):
+        res_sc = S.fit_with_fallback(*args_fit, **kw_sc)
+    torch.cuda.synchronize()
+    dt_sc = (time.perf_counter() - tb) / args.steps
+    rel_sc = (res_sc['chisq'] - plain['chisq']) / plain['chisq']
     rel = (res['chisq'] - plain['chisq']) / plain['chisq']
     acc = float(np.mean(res['proposed_kind'] == wl['kind']))
     log('surrogate %.0f sources/s vs plain %.0f; timings %s; iterations %.2f vs %.2f' % (
@@ -693,6 +705,14 @@ def surrogate_leg(wl, torch, args, plain_dt, plain):
                       for k, v in res['timings'].items()},
         'frac_not_worse_than_plain_by_1e-4': round(float(np.mean(rel <= 1e-4)), 5),
         'frac_within_1e-4_of_plain': round(float(np.mean(np.abs(rel) <= 1e-4)), 5),
+        'selfcheck_fallback': {
+            'rule': 'final chi2 > N_pix - N_par + 5 sqrt(2 N_pix) -> re-fit from the standard '
+                    'start (no plain fit needed)',
+            'sources_per_s': round(n / dt_sc, 3), 'speedup_vs_plain': round(plain_dt / dt_sc, 4),
+            'fallback_sources': int(res_sc['timings']['fallback_sources']),
+            'mean_iterations': round(float(res_sc['iterations'].mean()), 3),
+            'frac_not_worse_than_plain_by_1e-4': round(float(np.mean(rel_sc <= 1e-4)), 5),
+            'frac_within_1e-4_of_plain': round(float(np.mean(np.abs(rel_sc) <= 1e-4)), 5)},
     }
     return dict(summary=summary, res=res, dt=dt, prof=prof.get('slots'))
 
