@@ -17,11 +17,18 @@ pytestmark = pytest.mark.gpu
 WIDTH = HEIGHT = 512
 
 
-def _scene():
+# Even sizes: packed windows (P a multiple of 4, offsets 16-byte aligned)
+# take the normal equations' four-pixel vector loads, the strided ones
+# (x0 = 2 mod 4) the scalar loads.  Odd sizes: both take the scalar loads
+# and P*P leaves a partial last run of four pixels.
+SIZES = {'even': [24, 40, 56, 72, 96], 'odd': [25, 37, 51, 63, 95]}
+
+
+def _scene(sizes=SIZES['even']):
     rng = np.random.default_rng(W.seed(4, 40))
     S = 20
     kind = rng.choice([0, 1, 2], S, p=[0.3, 0.4, 0.3]).astype(np.int32)
-    P = rng.choice([24, 40, 56, 72, 96], S).astype(np.int32)
+    P = rng.choice(sizes, S).astype(np.int32)
     # windows on a 5 x 4 grid of 100 px cells: no two overlap
     x0 = (np.arange(S) % 5) * 100 + 2
     y0 = (np.arange(S) // 5) * 100 + 3
@@ -38,10 +45,11 @@ def _scene():
     return kind, P, x0, y0, truth
 
 
-def test_strided_windows_fit_like_packed_patches():
+@pytest.mark.parametrize('sizes', ['even', 'odd'])
+def test_strided_windows_fit_like_packed_patches(sizes):
     assert torch.cuda.is_available()
-    print('strided windows vs packed patches', flush=True)
-    kind, P, x0, y0, truth = _scene()
+    print('strided windows vs packed patches,', sizes, 'sizes', flush=True)
+    kind, P, x0, y0, truth = _scene(SIZES[sizes])
     psf = W.moffat_stamp(63, 4.0, 3.5)
     g = torch.Generator(device='cuda')
     g.manual_seed(W.seed(4, 41))
