@@ -647,6 +647,34 @@ __device__ __forceinline__ int gal_profile(const FitProfiles &pr, const SersicTa
 // steps of a galaxy and the index step of a Sersic galaxy.
 __device__ __forceinline__ int jac_planes(int kd) { return kd == 0 ? 1 : (kd == 3 ? 5 : 4); }
 
+// Per-launch profiling sums (jobs and algorithmic bytes per launch slot) of
+// one block of a job-building kernel: added up in LDS, then one global add
+// per counter the block touched.  One global atomic per thread would hit
+// the same 30 addresses from every wave and serialise (3 ms for a 213k-
+// thread line-search launch against 35 us without profiling).
+struct BlockAcc {
+    unsigned long long *g;
+    unsigned long long *l;
+    // every thread of the block calls this (a barrier when g is set)
+    __device__ __forceinline__ void init(unsigned long long *glob, unsigned long long *lds) {
+        g = glob;
+        l = lds;
+        if (!g) return;
+        for (int i = threadIdx.x; i < 2 * TRH_NSLOT; i += blockDim.x) l[i] = 0;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void add(int sl, unsigned long long bytes) const {
+        atomicAdd(&l[2 * sl], 1ull);
+        atomicAdd(&l[2 * sl + 1], bytes);
+    }
+    // after a barrier that follows the last add()
+    __device__ __forceinline__ void flush() const {
+        if (!g) return;
+        for (int i = threadIdx.x; i < 2 * TRH_NSLOT; i += blockDim.x)
+            if (l[i]) atomicAdd(&g[i], l[i]);
+    }
+};
+
 // One thread per (source, render), nj slots per source: render 0 of every
 // active source, renders 1..3 (EllipseE steps) of the galaxies and render 4
 // (index step, whose size goes to sstep[s]) of the Sersic galaxies.
@@ -661,6 +689,9 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
                                unsigned long long *__restrict__ acc,
                                const FitHybrid *__restrict__ hyb, FitMogJob *__restrict__ mog) {
     __shared__ trh_fft_job sj[PREP_TB];
+    __shared__ unsigned long long lacc[2 * TRH_NSLOT];
+    BlockAcc A;
+    A.init(acc, lacc);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = t / nj, k = t - s * nj;
     int b = -1;
@@ -673,10 +704,8 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
             // normal equations read the renders + data + inverr and write
             // the packed sums; the solve reads them and writes X and trials
             const unsigned long long nc = kd == 3 ? 7 : 6, nq = nc * (nc + 1) / 2 + nc + 2;
-            atomicAdd(&acc[2 * SL_NEQ], 1ull);
-            atomicAdd(&acc[2 * SL_NEQ + 1], (4ull * jac_planes(kd) + 8) * PP + nq * 8);
-            atomicAdd(&acc[2 * SL_SOLVE], 1ull);
-            atomicAdd(&acc[2 * SL_SOLVE + 1], nq * 8 + 8ull * 8 * NALPHA + 8 * 8);
+            A.add(SL_NEQ, (4ull * jac_planes(kd) + 8) * PP + nq * 8);
+            A.add(SL_SOLVE, nq * 8 + 8ull * 8 * NALPHA + 8 * 8);
         }
         trh_fft_job &J = sj[threadIdx.x];
         if (kd == 0) {
@@ -720,15 +749,14 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_jac(int S, int nj, const i
         J.scale = 1.f;
         J.slot = 0;
         if (acc && b >= 0) {
-            const int sl = b == PTB ? SL_JPT : b;
-            atomicAdd(&acc[2 * sl], 1ull);
-            atomicAdd(&acc[2 * sl + 1], 4ull * PP);  // render written
+            A.add(b == PTB ? SL_JPT : b, 4ull * PP);  // render written
         }
     }
     __shared__ int8_t built[PREP_TB];
     built[threadIdx.x] = b >= 0;
     block_bucket_counts(b, t, nj * S, bk, bcnt);
     __syncthreads();
+    A.flush();
     const int first = blockIdx.x * blockDim.x;
     copy_jobs_out(sj, built, jobs + first, min((int)blockDim.x, nj * S - first));
 }
@@ -1233,6 +1261,9 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
                               const FitHybrid *__restrict__ hyb, FitMogJob *__restrict__ mog,
                               const int64_t *__restrict__ boff) {
     __shared__ trh_fft_job sj[PREP_TB];
+    __shared__ unsigned long long lacc[2 * TRH_NSLOT];
+    BlockAcc A;
+    A.init(acc, lacc);
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = slot / NALPHA, a = slot - s * NALPHA;
     int b = -1;
@@ -1281,7 +1312,6 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
         J.slot = slot;
         if (acc && b >= 0) {
             const int sl = b == PTB ? SL_LPT : SL_LS + b;
-            atomicAdd(&acc[2 * sl], 1ull);
             // data + inverr read: the window, or for a point source the
             // stamp's support in it (3 pixels of Lanczos taps around it)
             unsigned long long px = (unsigned long long)P * P;
@@ -1290,13 +1320,14 @@ __global__ void __launch_bounds__(PREP_TB) k_fit_prep_ls(int S, const int32_t *_
                 const int hh = max(0, min(P, J.sy + ph + 3) - max(0, J.sy - 3));
                 px = (unsigned long long)w * hh;
             }
-            atomicAdd(&acc[2 * sl + 1], 8ull * px);
+            A.add(sl, 8ull * px);
         }
     }
     __shared__ int8_t built[PREP_TB];
     built[threadIdx.x] = b >= 0;
     block_bucket_counts(b, slot, NALPHA * S, bk, bcnt);
     __syncthreads();
+    A.flush();
     const int first = blockIdx.x * blockDim.x;
     copy_jobs_out(sj, built, jobs + first, min((int)blockDim.x, NALPHA * S - first));
 }
