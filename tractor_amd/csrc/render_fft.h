@@ -49,14 +49,17 @@ struct RenderScratch {
     double red[16];
 };
 
-// Lanczos taps for (mux, muy) into S.taps (all threads call; 2 barriers).
+// Lanczos taps for (mux, muy) into S.taps (all threads call; 1 barrier).
+// Lanes 0..13 of wave 0 evaluate the 14 raw taps and normalise them with
+// sums read across the wave, so no barrier separates the two steps.
 __device__ __forceinline__ void compute_taps(double mux, double muy, RenderScratch &S) {
     const int tid = threadIdx.x;
-    if (tid < 14) {
+    if (tid < 64) {
         const double dx = tid < 7 ? mux : muy;
         const int i = tid % 7;
         double val = 0.0;
-        if (dx == 0.0) {
+        if (tid >= 14) {
+        } else if (dx == 0.0) {
             val = (i == 3) ? 1.0 : 0.0;
         } else {
             const bool zero = dx > 0.0 ? (i == 6) : (i == 0);
@@ -65,18 +68,32 @@ __device__ __forceinline__ void compute_taps(double mux, double muy, RenderScrat
                 val = 3.0 * sin(pix) * sin(pix / 3.0) / (pix * pix);
             }
         }
-        S.dtaps[tid] = val;
-    }
-    __syncthreads();
-    if (tid < 14) {
         const int g = tid < 7 ? 0 : 7;
-        const double dx = tid < 7 ? mux : muy;
         double s = 0.0;
         // same summation order as lanczos_filter_3 (mp_fourier.i:380-400)
-        for (int i = 0; i < 7; ++i) s += S.dtaps[g + i];
-        S.taps[tid] = (dx == 0.0) ? (float)S.dtaps[tid] : (float)(S.dtaps[tid] / s);
+        for (int k = 0; k < 7; ++k) s += __shfl(val, g + k);
+        if (tid < 14) {
+            S.dtaps[tid] = val;
+            S.taps[tid] = (dx == 0.0) ? (float)val : (float)(val / s);
+        }
     }
     __syncthreads();
+}
+
+// A render's chi^2 into *dst: the waves' sums added in wave order by thread
+// 0 (block_sum's order) after one barrier, which also ends every wave's
+// reads of the render's LDS.  S.red is written again only after the next
+// render's first barrier, which thread 0 reaches after this read.
+__device__ __forceinline__ void chi2_out(double acc, RenderScratch &S, double *dst) {
+    const double v = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) S.red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        const int nw = blockDim.x >> 6;
+        for (int i = 0; i < nw; ++i) tot += S.red[i];
+        *dst = tot;
+    }
 }
 
 // Image pixels of the FFT renders (data, inverr) are loaded non-temporally
@@ -689,13 +706,16 @@ __device__ void render_fft_one(const trh_fft_job *__restrict__ Jp,
             }
             wave_sync();
         }
-        __syncthreads();
+        // The next pass's phase A overwrites the rows.  After the last pass
+        // the next render's first LDS writes before a barrier are S.comp
+        // (read in phase A only) and S.taps (read in phase B by the spatial
+        // path): the last barrier is needed only there, and a chi^2 render
+        // has chi2_out's.
+        if (rb0 + maxrows < yhi || (!FD && mode != TRH_CHI2)) __syncthreads();
     }
-    if (mode == TRH_CHI2) {
-        const double tot = block_sum(acc, S.red);
-        if (tid == 0) chi2[Jp->slot] = tot;
-    }
-    __syncthreads();
+    // No barrier follows the render: its LDS was last read before the
+    // barrier that ends phase B, or chi2_out's.
+    if (mode == TRH_CHI2) chi2_out(acc, S, chi2 + Jp->slot);
 }
 
 // Pixelized-PSF point-source render (PixelizedPSF.getPointSourcePatch with a
@@ -836,11 +856,10 @@ __device__ void render_point_one(const trh_fft_job *__restrict__ Jp,
                             inverr);
         }
     }
-    if (mode == TRH_CHI2) {
-        const double tot = block_sum(acc, S.red);
-        if (tid == 0) chi2[Jp->slot] = tot;
-    }
-    __syncthreads();
+    // the barrier of chi2_out, or this one, ends the reads of tmp and S.taps
+    // before the next job writes them
+    if (mode == TRH_CHI2) chi2_out(acc, S, chi2 + Jp->slot);
+    else __syncthreads();
 }
 
 // The spatial-Lanczos render (a job whose taps reach past the box edges) as
