@@ -546,11 +546,11 @@ int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dchisq, int ma
  * Branched replays for the last iterations of a fit (no reference
  * counterpart: launch scheduling).  Once the last trh_fitter_status read
  * counted fewer than `below_active` active sources, trh_fitter_iterate_graph
- * replays a second captured iteration in which the render launches of each
- * phase (one per FFT size, one for the point sources) run on their own
- * library-owned streams between a fork and a join, so that their launches,
- * latency-bound when few renders remain, overlap; under full load one
- * stream is faster, and the first graph stays in use.  Results are identical
+ * replays a second captured iteration in which, in each render phase, the
+ * sz = 128 launch runs on the fit's stream while the other launches (the
+ * other FFT sizes, then the point sources) run on one library-owned side
+ * stream between a fork and a join, so that launches that are
+ * latency-bound when few renders remain overlap.  Results are identical
  * either way (every render job writes its own window or chi^2 slot).
  * trh_fitter_set_params goes back to the first graph.  0 (the default):
  * never branch.

@@ -16,11 +16,11 @@ from ._lib import check, ptr
 NULLWCS_CDI = np.eye(2) * 3600.   # NullWCS(pixscale=1).cd_inverse (tractor/wcs.py:24-28)
 PIXEL_CDI = np.eye(2)             # position parameters in pixels
 # Active sources below which fit() replays the branched iteration graph
-# (render buckets on parallel streams; same results).  On MI355X, with the
-# 16384-source batched-fit workload: 120.6 ms per fit never branching, 117.1
-# ms branching below 1024-4096 active sources, 124.5 ms branching
-# throughout (the buckets then compete for the same CUs).
-DEFAULT_BRANCH_BELOW = 2048
+# (the sz = 128 renders on the fit's stream, the other buckets on a side
+# stream; same results).  On MI355X, with the 16384-source batched-fit
+# workload: 116.0 ms per fit never branching, 110.5 ms branching below
+# 2048-8192 active sources, 110.7 ms branching throughout.
+DEFAULT_BRANCH_BELOW = 4096
 
 
 class BatchFitter(object):

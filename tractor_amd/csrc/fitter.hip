@@ -283,7 +283,7 @@ struct trh_fitter {
     // LM iterations captured as hipGraphs, replayed on gstream: one or
     // GRAPH_ITERS iterations per graph (the gap between two graph launches
     // is ~7.5 us, between the kernels inside one none), each on one stream
-    // or with the render launches of each phase forked onto side streams
+    // or with render launches of each phase forked onto a side stream
     // (parallel graph branches, replayed once the last count of active
     // sources was below branch_below; trh_fitter_set_branching, 0: never).
     // Slot (branched ? 2 : 0) + (GRAPH_ITERS iterations ? 1 : 0), each with
@@ -298,8 +298,8 @@ struct trh_fitter {
     bool have_gev = false;
     int branch_below = 0;
     int last_active = -1;   // -1: not read since the last set_params
-    hipStream_t bside[NBUCKET];
-    hipEvent_t bfork[2], bjoin[2][NBUCKET];
+    hipStream_t bside = nullptr;
+    hipEvent_t bfork[2], bjoin[2];
     bool have_br = false;
     // active-source count read without synchronising the fit
     // (trh_fitter_count_active / trh_fitter_count_read): a one-block count
@@ -1487,13 +1487,11 @@ extern "C" int trh_fitter_destroy(trh_fitter *f) {
     if (f->have_gev)
         for (int i = 0; i < 2; ++i) (void)hipEventDestroy(f->gev[i]);
     if (f->have_br) {
-        for (int i = 0; i < NBUCKET; ++i) {
-            (void)hipStreamSynchronize(f->bside[i]);
-            (void)hipStreamDestroy(f->bside[i]);
-        }
+        (void)hipStreamSynchronize(f->bside);
+        (void)hipStreamDestroy(f->bside);
         for (int p = 0; p < 2; ++p) {
             (void)hipEventDestroy(f->bfork[p]);
-            for (int i = 0; i < NBUCKET; ++i) (void)hipEventDestroy(f->bjoin[p][i]);
+            (void)hipEventDestroy(f->bjoin[p]);
         }
     }
     if (f->have_count) {
@@ -1814,30 +1812,38 @@ static int launch_mog(trh_fitter *f, const FitMogJob *jobs, const int32_t *lists
     return hip_check(hipGetLastError(), "k_fit_mog");
 }
 
-// Fork point of a render phase in a branched capture: the side streams
-// start after everything enqueued on `st` so far.
+// Fork point of a render phase in a branched capture: the side stream
+// starts after everything enqueued on `st` so far.
 static int branch_fork(trh_fitter *f, int ph, hipStream_t st) {
     int rc = hip_check(hipEventRecord(f->bfork[ph], st), "branch fork");
-    for (int i = 0; i < NBUCKET && !rc; ++i)
-        rc = hip_check(hipStreamWaitEvent(f->bside[i], f->bfork[ph], 0), "branch wait");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(f->bside, f->bfork[ph], 0), "branch wait");
     return rc;
 }
 
-// Join of a render phase: `st` continues after every side stream's work.
+// Join of a render phase: `st` continues after the side stream's work.
 static int branch_join(trh_fitter *f, int ph, hipStream_t st) {
-    int rc = TRH_OK;
-    for (int i = 0; i < NBUCKET && !rc; ++i)
-        rc = hip_check(hipEventRecord(f->bjoin[ph][i], f->bside[i]), "branch join");
-    for (int i = 0; i < NBUCKET && !rc; ++i)
-        rc = hip_check(hipStreamWaitEvent(st, f->bjoin[ph][i], 0), "branch join wait");
+    int rc = hip_check(hipEventRecord(f->bjoin[ph], f->bside), "branch join");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(st, f->bjoin[ph], 0), "branch join wait");
     return rc;
 }
 
-// The launches of one LM iteration.  `branch` (graph capture only): the FFT
-// size buckets and the point-source bucket of each render phase run on
-// their own side streams between a fork and a join (they write disjoint
+// Stream of render bucket b in a branched iteration: the sz = 128 bucket,
+// the largest share of the renders, stays on the iteration's stream and
+// every other bucket runs, in order, on the one side stream.  Measured on a
+// 16384-source fit: 110.5 ms, against 112.4 with a side stream per bucket
+// (each cross-stream edge of a captured graph costs ~10 us at the fork and
+// the join, and streams beyond the hardware queues share them) and 116.0
+// with no branching.
+static hipStream_t bucket_stream(const trh_fitter *f, int b, bool branch, hipStream_t st) {
+    const bool stays = b < NFFT && kSizes[b] == 128;
+    return branch && !stays ? f->bside : st;
+}
+
+// The launches of one LM iteration.  `branch` (graph capture only): in each
+// render phase the sz = 128 bucket runs beside the other buckets, which run
+// on the side stream between a fork and a join (they write disjoint
 // outputs: every job its own render window or chi^2 slot, every FFT size
-// its own row slots), so under few active sources the latency-bound small
+// its own row slots), so under few active sources the latency-bound
 // launches overlap instead of running one after another.  Every job's
 // arithmetic is the same either way, so the results are identical.
 static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_iter, hipStream_t st,
@@ -1869,14 +1875,14 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
         SLOT_BEGIN(b);
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_STORE, f->jobs_jac,
                                  f->list_jac + (size_t)b * cap_jac, f->cnt + b, 4 * f->n_gal + f->n_ser, f->maxrows[b], f->spec[b], f->renders,
-                                 nullptr, nullptr, nullptr, branch ? f->bside[b] : st,
+                                 nullptr, nullptr, nullptr, bucket_stream(f, b, branch, st),
                                  fitter_hbuf(f, kSizes[b]))))
             return rc;
         SLOT_END(b);
     }
     SLOT_BEGIN(SL_JPT);
     if ((rc = launch_point(f, f->jobs_jac, f->list_jac + (size_t)PTB * cap_jac, f->cnt + PTB, f->n_pt,
-                           f->renders, nullptr, branch ? f->bside[PTB] : st)))
+                           f->renders, nullptr, bucket_stream(f, PTB, branch, st))))
         return rc;
     SLOT_END(SL_JPT);
     if (branch && (rc = branch_join(f, 0, st))) return rc;
@@ -1931,14 +1937,14 @@ static int iterate_body(trh_fitter *f, double lambda, double dchisq, int max_ite
         if ((rc = launch_fft_any(f->h, kSizes[b], TRH_CHI2, f->jobs_ls,
                                  f->list_ls + (size_t)b * cap_ls, f->cnt + LS_CNT + b, NALPHA * f->n_gal, f->maxrows[b], f->spec[b],
                                  nullptr, f->data, f->inverr, f->chi2ls,
-                                 branch ? f->bside[b] : st, fitter_hbuf(f, kSizes[b]),
+                                 bucket_stream(f, b, branch, st), fitter_hbuf(f, kSizes[b]),
                                  f->hyb ? f->ls_base : nullptr)))
             return rc;
         SLOT_END(SL_LS + b);
     }
     SLOT_BEGIN(SL_LPT);
     if ((rc = launch_point(f, f->jobs_ls, f->list_ls + (size_t)PTB * cap_ls, f->cnt + LS_CNT + PTB,
-                           NALPHA * f->n_pt, nullptr, f->chi2ls, branch ? f->bside[PTB] : st)))
+                           NALPHA * f->n_pt, nullptr, f->chi2ls, bucket_stream(f, PTB, branch, st))))
         return rc;
     SLOT_END(SL_LPT);
     if (branch && (rc = branch_join(f, 1, st))) return rc;
@@ -2010,27 +2016,24 @@ extern "C" int trh_fitter_iterate_graph(trh_fitter *f, double lambda, double dch
     // the branched iteration once few sources remain (see trh_fitter_set_branching)
     const bool br = f->branch_below > 0 && f->last_active >= 0 && f->last_active < f->branch_below;
     if (br && !f->have_br) {
-        // all side streams and events, or none (what was made is released)
-        int ns = 0, ne = 0;
-        hipEvent_t evs[2 * (NBUCKET + 1)];
-        rc = TRH_OK;
-        while (!rc && ns < NBUCKET)
-            if (!(rc = hip_check(hipStreamCreateWithFlags(&f->bside[ns], hipStreamNonBlocking),
-                                 "branch stream")))
-                ++ns;
-        while (!rc && ne < 2 * (NBUCKET + 1))
+        // the side stream and its four events, or none (what was made is released)
+        int ne = 0;
+        hipEvent_t evs[4];
+        rc = hip_check(hipStreamCreateWithFlags(&f->bside, hipStreamNonBlocking), "branch stream");
+        while (!rc && ne < 4)
             if (!(rc = hip_check(hipEventCreateWithFlags(&evs[ne], hipEventDisableTiming),
                                  "branch event")))
                 ++ne;
         if (rc) {
             for (int i = 0; i < ne; ++i) (void)hipEventDestroy(evs[i]);
-            for (int i = 0; i < ns; ++i) (void)hipStreamDestroy(f->bside[i]);
+            if (f->bside) (void)hipStreamDestroy(f->bside);
+            f->bside = nullptr;
             return rc;
         }
-        for (int p = 0; p < 2; ++p) {
-            f->bfork[p] = evs[p * (NBUCKET + 1)];
-            for (int i = 0; i < NBUCKET; ++i) f->bjoin[p][i] = evs[p * (NBUCKET + 1) + 1 + i];
-        }
+        f->bfork[0] = evs[0];
+        f->bfork[1] = evs[1];
+        f->bjoin[0] = evs[2];
+        f->bjoin[1] = evs[3];
         f->have_br = true;
     }
     // GRAPH_ITERS iterations per graph launch when n_iter is a multiple
