@@ -305,12 +305,12 @@ static int sz_index(int sz) {
 static const int kDefThreads[5] = {256, 128, 512, 512, 1024};
 static const int kDefRows[5] = {0, 0, 120, 0, 0};
 
-// Global row slots of the SZ >= 256 renders (one per block of the grid):
-// 2 per CU at SZ = 256 (136 MB) and at SZ = 512 (541 MB).  Two blocks of
-// 512 threads per CU are resident at SZ = 256; with no more slots than that
-// the rows stay warmer in the caches than with 4 per CU (a sz = 256 chi^2
-// render 185 -> 179 ns; one per CU: 205 ns).
-int fft_hbuf_slots(const trh_handle *h, int sz) { return h->n_cu * 2; }
+// Global row slots of the SZ >= 256 renders (one per block of the grid),
+// as many per CU as blocks of the default shape are resident: 2 at SZ = 256
+// (512 threads; 136 MB), 1 at SZ = 512 (1024 threads; 270 MB).  With no
+// more slots than resident blocks the rows stay warmer in the caches (a
+// sz = 256 chi^2 render 185 -> 179 ns against 4 per CU; 1 per CU: 205 ns).
+int fft_hbuf_slots(const trh_handle *h, int sz) { return h->n_cu * (sz >= 512 ? 1 : 2); }
 
 void fft_launch_shape(trh_handle *h, int sz, int maxrows, int *threads, int *rows, size_t *lds,
                       int *grid, int njobs) {
